@@ -1,0 +1,169 @@
+/*
+ * peridem_gpu.h -- C-ABI of the MI355X explicit-step library (libperidem_gpu.so).
+ *
+ * The reference has no plugin API: its explicit step is the C++ template
+ *   run_timeloop<dim>(PArr, TL, CN, Wall, CFGV)          (src/particle/timeloop.h:231-965)
+ * called once from main() (run/simulate3d.cpp:142).  Its inputs are built by
+ *   load_particles / load_contact / load_wall             (src/read/load_hdf5.h:41-181)
+ * and its outputs are read back from ParticleN / RectWall every `modulo` steps
+ *   (src/particle/timeloop.h:799-953).
+ * This header replaces that template boundary with plain C entry points: the
+ * host driver keeps the reference's config keys, time switches and output
+ * layout, and hands the particle arrays across as plain pointers.
+ *
+ * Conventions
+ *  - every call returns int: 0 on success, a negative PD_E* code on failure;
+ *    pd_last_error(ctx) returns a message for the last failure.  Nothing
+ *    exits the process and no exception crosses the boundary (the reference
+ *    prints and exits: src/debug/err.h:21-24).
+ *  - node arrays are row-major double[n*3] exactly as the HDF5 datasets
+ *    (Pos, disp, vel, acc, extforce) are laid out; the library copies them.
+ *  - a context owns one HIP stream on the current device; a context is used
+ *    from one host thread at a time; distinct contexts are independent.
+ *  - all arithmetic is IEEE FP64 without contraction; results are bitwise
+ *    equal to a single-rank run of the reference algorithm.
+ */
+#ifndef PERIDEM_GPU_H
+#define PERIDEM_GPU_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PD_OK 0
+#define PD_E_ARG (-1)         /* bad argument / size mismatch */
+#define PD_E_STATE (-2)       /* call not valid in this state (e.g. before pd_commit) */
+#define PD_E_HIP (-3)         /* HIP runtime failure */
+#define PD_E_UNSUPPORTED (-4) /* reference feature not built in this library */
+#define PD_E_OVERFLOW (-5)    /* a fixed device capacity was exceeded (see message) */
+#define PD_E_NOMEM (-6)       /* device memory budget exceeded */
+
+/* Run-level parameters: Timeloop + Contact + RectWall<3> after apply_config
+ * (src/particle/timeloop.h:71-229, src/particle/contact.h:104-183,185-376).
+ * Wall planes are in the reference's 3-D lrtb order
+ * [x_min, y_min, z_min, x_max, y_max, z_max] (contact.h:221-238). */
+typedef struct pd_run_desc {
+  double dt;
+  uint32_t enable_fracture;       /* TL.enable_fracture */
+  uint32_t self_contact;          /* CN.self_contact */
+  uint32_t allow_damping;         /* CN.allow_damping */
+  uint32_t allow_friction;        /* CN.allow_friction */
+  uint32_t nl_bdry_only;          /* CN.nl_bdry_only */
+  uint32_t compute_wall_reaction; /* CN.compute_wall_reaction */
+  uint32_t gradient_extforce;     /* TL.gradient_extforce */
+  uint32_t extforce_maxstep;      /* TL.extforce_maxstep */
+  uint32_t enable_torque;         /* TL.enable_torque */
+  uint32_t allow_wall;            /* Wall.allow_wall */
+  double contact_rad;             /* CN.contact_rad */
+  double normal_stiffness;        /* CN.normal_stiffness */
+  double friction_coefficient;    /* CN.friction_coefficient */
+  double damping_ratio;           /* CN.damping_ratio */
+  double wall_lrtb[6];            /* Wall.lrtb() */
+  double wall_speed[6];           /* Wall.speed_{x,y,z}_min, speed_{x,y,z}_max */
+  uint32_t first_step;            /* value of t for the first step (1 for a fresh run) */
+  uint32_t reserved;
+} pd_run_desc;
+
+/* One particle as load_particles reads it from P_%05u/ (load_hdf5.h:53-106). */
+typedef struct pd_particle_desc {
+  uint32_t nnodes;
+  uint32_t instance;          /* independent system id (0 for a plain run) */
+  const double* pos;          /* [nnodes*3] Pos */
+  const double* vol;          /* [nnodes]   Vol */
+  const double* disp;         /* [nnodes*3] disp (NULL = zeros) */
+  const double* vel;          /* [nnodes*3] vel  (NULL = zeros) */
+  const double* acc;          /* [nnodes*3] acc  (NULL = zeros) */
+  const double* extforce;     /* [nnodes*3] extforce (NULL = zeros) */
+  uint64_t nconn;             /* rows of Connectivity */
+  const uint32_t* conn;       /* [nconn*2] Connectivity, file order */
+  uint32_t nclamped;
+  const uint32_t* clamped;    /* clamped_nodes */
+  uint32_t nbdry;
+  const uint32_t* bdry;       /* bdry_nodes (used only when nl_bdry_only) */
+  double delta, rho, cnot, snot;
+  int32_t movable, breakable, stoppable;
+  uint32_t torque_axis;
+  double torque_val;
+} pd_particle_desc;
+
+typedef struct pd_counters {
+  uint64_t steps;              /* steps advanced since pd_commit */
+  uint64_t bond_updates;       /* directed CSR entries of movable particles evaluated */
+  uint64_t pair_computations;  /* particle pairs that passed the box test (timeloop.h:636-653) */
+  uint64_t alive_bonds;        /* directed bonds not broken (computed on request) */
+  uint64_t total_bonds;        /* directed bonds at commit */
+  uint64_t nodes;
+  uint64_t particles;
+  uint64_t cell_rebuilds;      /* contact candidate rebuilds */
+} pd_counters;
+
+typedef struct pd_ctx pd_ctx;
+
+/* Create a context: replaces the value copies of Timeloop/Contact/RectWall
+ * passed into run_timeloop (timeloop.h:231-233). */
+int pd_create(pd_ctx** out, const pd_run_desc* run);
+
+/* Add one particle: replaces load_particles' per-particle body
+ * (load_hdf5.h:61-106) including conn2NArr (load_hdf5.h:27-37) and gen_xi
+ * (particle2.h:241-257), which run at pd_commit.  Returns the particle index
+ * through *index when index != NULL. */
+int pd_add_particle(pd_ctx* ctx, const pd_particle_desc* p, uint32_t* index);
+
+/* Build the neighbour rows in conn2NArr order, upload all state. */
+int pd_commit(pd_ctx* ctx);
+
+/* Advance nsteps explicit steps: the body of the loop at timeloop.h:395-797
+ * (Verlet position half, PD force + bond breaking, self/wall/pair contact,
+ * external force, Verlet velocity half, lagged boxes, wall motion).  The host
+ * splits calls at time-switch and output boundaries. */
+int pd_step(pd_ctx* ctx, uint32_t nsteps);
+
+/* Host-side time switches (timeloop.h:405-535). */
+#define PD_FLAG_ENABLE_FRACTURE 1
+#define PD_FLAG_SELF_CONTACT 2
+#define PD_FLAG_ALLOW_DAMPING 3
+#define PD_FLAG_ALLOW_FRICTION 4
+#define PD_FLAG_ENABLE_TORQUE 5
+#define PD_FLAG_COMPUTE_WALL_REACTION 6
+int pd_set_flag(pd_ctx* ctx, int flag, int value);
+int pd_set_wall_speed(pd_ctx* ctx, const double speed[6]);
+int pd_set_wall(pd_ctx* ctx, const double lrtb[6]);
+#define PD_PFLAG_MOVABLE 1
+#define PD_PFLAG_STOPPABLE 2
+#define PD_PFLAG_BREAKABLE 3
+int pd_set_particle_flag(pd_ctx* ctx, uint32_t p, int which, int value);
+/* TL.new_snot override (timeloop.h:314-318). */
+int pd_set_snot(pd_ctx* ctx, uint32_t p, double snot);
+
+/* State read-back for output (timeloop.h:826-851).  Any pointer may be NULL.
+ * Arrays are [nnodes*3].  force is the force density, as in the reference. */
+int pd_get_state(pd_ctx* ctx, uint32_t p, double* disp, double* currpos,
+                 double* vel, double* acc, double* force);
+/* Overwrite state (resume, timeloop.h:262-291).  Any pointer may be NULL. */
+int pd_set_state(pd_ctx* ctx, uint32_t p, const double* disp,
+                 const double* vel, const double* acc);
+/* Intact bonds as NbdArr2conn emits them (timeloop.h:20-44): pairs (i,q)
+ * with i<q in row order.  *m receives the number of pairs; when cap is too
+ * small nothing is written and PD_E_ARG is returned with *m set. */
+int pd_get_connectivity(pd_ctx* ctx, uint32_t p, uint32_t* pairs,
+                        uint64_t cap, uint64_t* m);
+/* Wall planes (lrtb order) and, when compute_wall_reaction is on, the
+ * reaction of the last step as get_reaction() orders it (contact.h:255-272):
+ * [x_min, y_min, z_min, x_max, y_max, z_max] x 3 components. */
+int pd_get_wall(pd_ctx* ctx, uint32_t instance, double lrtb[6], double reaction[18]);
+int pd_get_counters(pd_ctx* ctx, pd_counters* out);
+/* Time of the last pd_step call on the device (HIP events), milliseconds. */
+int pd_last_step_ms(pd_ctx* ctx, double* ms);
+const char* pd_last_error(pd_ctx* ctx);
+void pd_destroy(pd_ctx* ctx);
+
+/* Library version and build target, for logs. */
+const char* pd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PERIDEM_GPU_H */

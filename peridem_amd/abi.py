@@ -1,0 +1,216 @@
+"""ctypes view of the C-ABI in include/peridem_gpu.h.
+
+`System` (peridem_amd.gen) describes a run; `Engine` drives one context of
+a library exporting that ABI.  The HIP library (`libperidem_gpu.so`) exports
+pd_*; the CPU oracle used by the tests exports the same entry points under
+the or_ prefix, so both are driven by the same class.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GPU_LIB = os.path.join(HERE, "libperidem_gpu.so")
+
+PD_OK = 0
+ERRORS = {-1: "PD_E_ARG", -2: "PD_E_STATE", -3: "PD_E_HIP", -4: "PD_E_UNSUPPORTED",
+          -5: "PD_E_OVERFLOW", -6: "PD_E_NOMEM"}
+
+PD_FLAG_ENABLE_FRACTURE = 1
+PD_FLAG_SELF_CONTACT = 2
+PD_FLAG_ALLOW_DAMPING = 3
+PD_FLAG_ALLOW_FRICTION = 4
+PD_FLAG_ENABLE_TORQUE = 5
+PD_FLAG_COMPUTE_WALL_REACTION = 6
+PD_PFLAG_MOVABLE = 1
+PD_PFLAG_STOPPABLE = 2
+PD_PFLAG_BREAKABLE = 3
+
+_u32, _u64, _i32, _dbl = C.c_uint32, C.c_uint64, C.c_int32, C.c_double
+_pd = C.POINTER(C.c_double)
+_pu = C.POINTER(C.c_uint32)
+
+
+class RunDesc(C.Structure):
+    _fields_ = [("dt", _dbl), ("enable_fracture", _u32), ("self_contact", _u32),
+                ("allow_damping", _u32), ("allow_friction", _u32), ("nl_bdry_only", _u32),
+                ("compute_wall_reaction", _u32), ("gradient_extforce", _u32),
+                ("extforce_maxstep", _u32), ("enable_torque", _u32), ("allow_wall", _u32),
+                ("contact_rad", _dbl), ("normal_stiffness", _dbl),
+                ("friction_coefficient", _dbl), ("damping_ratio", _dbl),
+                ("wall_lrtb", _dbl * 6), ("wall_speed", _dbl * 6),
+                ("first_step", _u32), ("reserved", _u32)]
+
+
+class ParticleDesc(C.Structure):
+    _fields_ = [("nnodes", _u32), ("instance", _u32), ("pos", _pd), ("vol", _pd),
+                ("disp", _pd), ("vel", _pd), ("acc", _pd), ("extforce", _pd),
+                ("nconn", _u64), ("conn", _pu), ("nclamped", _u32), ("clamped", _pu),
+                ("nbdry", _u32), ("bdry", _pu), ("delta", _dbl), ("rho", _dbl),
+                ("cnot", _dbl), ("snot", _dbl), ("movable", _i32), ("breakable", _i32),
+                ("stoppable", _i32), ("torque_axis", _u32), ("torque_val", _dbl)]
+
+
+class Counters(C.Structure):
+    _fields_ = [(n, _u64) for n in ("steps", "bond_updates", "pair_computations",
+                                    "alive_bonds", "total_bonds", "nodes", "particles",
+                                    "cell_rebuilds")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+SYMBOLS = ["create", "add_particle", "commit", "step", "set_flag", "set_wall_speed",
+           "set_wall", "set_particle_flag", "set_snot", "get_state", "set_state",
+           "get_connectivity", "get_wall", "get_counters", "last_error", "destroy"]
+
+
+class PDError(RuntimeError):
+    pass
+
+
+def _bind(lib, prefix):
+    P = C.c_void_p
+    sig = {
+        "create": (C.c_int, [C.POINTER(P), C.POINTER(RunDesc)]),
+        "add_particle": (C.c_int, [P, C.POINTER(ParticleDesc), _pu]),
+        "commit": (C.c_int, [P]),
+        "step": (C.c_int, [P, _u32]),
+        "set_flag": (C.c_int, [P, C.c_int, C.c_int]),
+        "set_wall_speed": (C.c_int, [P, _pd]),
+        "set_wall": (C.c_int, [P, _pd]),
+        "set_particle_flag": (C.c_int, [P, _u32, C.c_int, C.c_int]),
+        "set_snot": (C.c_int, [P, _u32, _dbl]),
+        "get_state": (C.c_int, [P, _u32, _pd, _pd, _pd, _pd, _pd]),
+        "set_state": (C.c_int, [P, _u32, _pd, _pd, _pd]),
+        "get_connectivity": (C.c_int, [P, _u32, _pu, _u64, C.POINTER(_u64)]),
+        "get_wall": (C.c_int, [P, _u32, _pd, _pd]),
+        "get_counters": (C.c_int, [P, C.POINTER(Counters)]),
+        "last_error": (C.c_char_p, [P]),
+        "destroy": (None, [P]),
+    }
+    fns = {}
+    for name, (res, args) in sig.items():
+        f = getattr(lib, prefix + name)
+        f.restype = res
+        f.argtypes = args
+        fns[name] = f
+    return fns
+
+
+_LIBS = {}
+
+
+def load_gpu_lib(path=GPU_LIB):
+    """Load the HIP library; raises if it is not built (no CPU fallback)."""
+    if path not in _LIBS:
+        if not os.path.exists(path):
+            raise PDError(f"{path} is not built: run __graft_entry__.build()")
+        lib = C.CDLL(path)
+        lib.pd_version.restype = C.c_char_p
+        lib.pd_last_step_ms.restype = C.c_int
+        lib.pd_last_step_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        _LIBS[path] = (lib, _bind(lib, "pd_"))
+    return _LIBS[path]
+
+
+def _ptr(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+class Engine:
+    """One context of a library exporting the pd_ ABI (or the oracle's or_)."""
+
+    def __init__(self, lib, fns, system, prefix="pd_"):
+        self.lib, self.f, self.sys, self.prefix = lib, fns, system, prefix
+        self.ctx = C.c_void_p()
+        self._keep = []
+        rd = system.run_desc()
+        self._check(self.f["create"](C.byref(self.ctx), C.byref(rd)), "create")
+        for p in system.particles:
+            d = p.desc(self._keep)
+            self._check(self.f["add_particle"](self.ctx, C.byref(d), None), "add_particle")
+        self._check(self.f["commit"](self.ctx), "commit")
+        self._keep = []
+
+    def _check(self, rc, what):
+        if rc != PD_OK:
+            msg = self.f["last_error"](self.ctx) if self.ctx else b""
+            raise PDError(f"{self.prefix}{what}: {ERRORS.get(rc, rc)}: "
+                          f"{(msg or b'').decode(errors='replace')}")
+
+    def step(self, n=1):
+        self._check(self.f["step"](self.ctx, n), "step")
+
+    def set_flag(self, flag, value):
+        self._check(self.f["set_flag"](self.ctx, flag, int(value)), "set_flag")
+
+    def set_wall_speed(self, speed):
+        a = np.ascontiguousarray(speed, dtype=np.float64)
+        self._check(self.f["set_wall_speed"](self.ctx, _ptr(a)), "set_wall_speed")
+
+    def set_wall(self, lrtb):
+        a = np.ascontiguousarray(lrtb, dtype=np.float64)
+        self._check(self.f["set_wall"](self.ctx, _ptr(a)), "set_wall")
+
+    def set_particle_flag(self, p, which, value):
+        self._check(self.f["set_particle_flag"](self.ctx, p, which, int(value)), "set_particle_flag")
+
+    def set_snot(self, p, snot):
+        self._check(self.f["set_snot"](self.ctx, p, float(snot)), "set_snot")
+
+    def state(self, p):
+        n = self.sys.particles[p].n
+        out = {k: np.empty((n, 3)) for k in ("disp", "currpos", "vel", "acc", "force")}
+        self._check(self.f["get_state"](self.ctx, p, *[_ptr(out[k]) for k in
+                                                      ("disp", "currpos", "vel", "acc", "force")]),
+                    "get_state")
+        return out
+
+    def set_state(self, p, disp=None, vel=None, acc=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (disp, vel, acc)]
+        self._check(self.f["set_state"](self.ctx, p, *[_ptr(a) for a in arrs]), "set_state")
+
+    def connectivity(self, p):
+        m = _u64()
+        self._check(self.f["get_connectivity"](self.ctx, p, None, 0, C.byref(m)), "get_connectivity")
+        out = np.empty((m.value, 2), dtype=np.uint32)
+        self._check(self.f["get_connectivity"](self.ctx, p, _ptr(out, C.c_uint32), m.value,
+                                               C.byref(m)), "get_connectivity")
+        return out
+
+    def wall(self, instance=0):
+        lrtb, reac = np.empty(6), np.empty(18)
+        self._check(self.f["get_wall"](self.ctx, instance, _ptr(lrtb), _ptr(reac)), "get_wall")
+        return lrtb, reac.reshape(6, 3)
+
+    def counters(self):
+        c = Counters()
+        self._check(self.f["get_counters"](self.ctx, C.byref(c)), "get_counters")
+        return c.as_dict()
+
+    def last_step_ms(self):
+        ms = C.c_double()
+        self._check(self.lib.pd_last_step_ms(self.ctx, C.byref(ms)), "last_step_ms")
+        return ms.value
+
+    def close(self):
+        if self.ctx:
+            self.f["destroy"](self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gpu_engine(system):
+    lib, fns = load_gpu_lib()
+    return Engine(lib, fns, system, "pd_")

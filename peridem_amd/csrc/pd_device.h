@@ -1,0 +1,246 @@
+// pd_device.h -- device-side data model and per-term arithmetic of the
+// explicit step.  Every formula follows the reference operation for
+// operation (IEEE FP64, compiled with -ffp-contract=off): a term computed
+// here is bitwise equal to the same term of the reference.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pd {
+
+constexpr uint32_t kBroken = 0x80000000u;   // high bit of a bond entry: bond erased
+constexpr uint32_t kPad = 0xFFFFFFFFu;      // slice padding entry
+constexpr uint32_t kIdxMask = 0x7FFFFFFFu;
+constexpr int kSlice = 64;                  // nodes per bond slice (= wavefront width)
+constexpr int kMaxCand = 48;                // contact partners within R_c kept per node
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+
+// error bits raised by kernels, read back by the host after pd_step
+constexpr uint32_t kErrCandOverflow = 1u;
+
+// Per-particle parameters (ParticleN scalars, particle2.h:51-59).
+struct DevPart {
+  uint64_t node_off;
+  uint32_t nnodes;
+  uint32_t inst;
+  double rho, rinv, cnot, snot;   // rinv = 1/rho as the reference forms it (timeloop.h:673)
+  int32_t movable, breakable, stoppable, rows_sorted;
+};
+
+// Per-instance (independent system) parameters: Contact + RectWall
+// (contact.h:104-183, 185-376) and the run switches of run_timeloop.
+struct DevInst {
+  double Rc, Kn, mu, zeta;
+  double cell;              // contact cell edge (>= Rc)
+  double wall[6];           // x_min, y_min, z_min, x_max, y_max, z_max
+  double speed[6];
+  int32_t allow_wall, allow_damping, allow_friction, self_contact;
+  int32_t enable_fracture, gradient_extforce;
+  uint32_t extforce_maxstep;
+  uint32_t pad0;
+};
+
+struct Dev {
+  // node state, global node index g
+  double4* X;        // x, y, z, vol
+  double4* U;        // disp x, y, z, (unused)
+  double* V;         // vel [3N]
+  double* A;         // acc [3N]
+  double* F;         // force density [3N]
+  double* E;         // extforce [3N]
+  uint32_t* pid;     // particle of node
+  uint8_t* nflag;    // bit0 clamped
+  uint32_t* deg;     // row length at commit
+  // bonds: sliced rows, 64 nodes per slice, entry k of lane l at soff[s] + k*64 + l
+  uint64_t* soff;    // [S+1]
+  uint32_t* bond;
+  // particles / instances
+  DevPart* part;
+  DevInst* inst;
+  double* lrtb;      // [P*6] lagged boxes (get_minmax order)
+  unsigned long long* lrtb_next;  // [P*6] ordered-key min/max accumulators
+  uint32_t* pactive; // [P] particle has an active contact pair this step
+  uint32_t* pairs;   // [npairs*2] particle pairs (i<j) of the same instance
+  unsigned long long* counters;   // [0] pair computations
+  uint32_t* err;
+  // wall reaction (per instance, 6 planes x 3) and helpers
+  double* reaction;
+  uint8_t* wmask;    // [N] planes in contact this step (reaction steps only)
+  uint8_t* bflag;    // [N/256+1] block has a wall contact
+  // contact cell list
+  uint32_t* ckey;    // [N]
+  uint32_t* cstart;  // [T+1]
+  uint32_t* ccur;    // [T]
+  uint32_t* csorted; // [N]
+  uint32_t* scan_tmp;
+  uint64_t N, S, P, npairs, ninst;
+  uint32_t T;        // hash table size (power of two)
+  uint32_t pad;
+};
+
+// Eigen 3.3 fixed-size 1x3 norm/dot order.
+__device__ __forceinline__ double nrm3(double a, double b, double c) {
+  return sqrt((a * a + b * b) + c * c);
+}
+
+// monotone map double -> uint64 for atomic min/max
+__device__ __forceinline__ unsigned long long okey(double v) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double okey_inv(unsigned long long k) {
+  unsigned long long b = (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  double d;
+  __builtin_memcpy(&d, &b, 8);
+  return d;
+}
+
+__device__ __forceinline__ uint32_t cell_hash(int64_t cx, int64_t cy, int64_t cz, uint32_t inst, uint32_t T) {
+  uint64_t h = (uint64_t)cx * 0x9E3779B97F4A7C15ull;
+  h ^= (uint64_t)cy * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+  h ^= (uint64_t)cz * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
+  h ^= (uint64_t)inst * 0xD6E8FEB86659FD93ull + (h << 6) + (h >> 2);
+  h ^= h >> 29;
+  return (uint32_t)h & (T - 1);
+}
+
+// intersects_box_fext<3> (contact.h:1143-1183): A extended by c.
+__device__ __forceinline__ bool intersects_box_fext(const double* A, const double* B, double c) {
+  double Aminx = A[0] - c, Aminy = A[1] - c, Aminz = A[2] - c;
+  double Amaxx = A[3] + c, Amaxy = A[4] + c, Amaxz = A[5] + c;
+  return (Aminx <= B[3]) && (Amaxx >= B[0]) && (Amaxy >= B[1]) && (Aminy <= B[4]) &&
+         (Aminz <= B[5]) && (Amaxz >= B[2]);
+}
+
+// within_interior_fext (contact.h:1229-1243): the 3-D box read with 2-D indices.
+__device__ __forceinline__ bool within_interior_fext(const double* A, const double* B, double c) {
+  double AL = A[0] - c, AR = A[1] + c, AT = A[2] + c, AB = A[3] - c;
+  return (B[0] < AL) && (AR < B[1]) && (B[2] > AT) && (B[3] < AB);
+}
+
+// wall_side_force_update<3> (contact.h:385-502) with segment_mass<3>
+// (contact.h:14-39).  Adds the node's terms to f; when reac != nullptr also
+// accumulates the wall reaction in the reference's order.
+__device__ __forceinline__ bool wall_side(const DevInst& I, double dt, double dist, int dir,
+                                          const double* vel, double vol, double rho, double* f,
+                                          double speed, double* reac) {
+  const double Rc = I.Rc;
+  if (!(fabs(dist) <= Rc)) return false;
+  double dd = fabs(dist);
+  double Rd = Rc - dd;
+  double area = ((M_PI / 3) * (Rd * Rd)) * (2 * Rc - dd);
+  double q = Rc * Rc - dd * dd;
+  double Im = (M_PI / 4) * (q * q);
+  double cen = Im / area;
+  double contrib = Rc - cen;
+  double pc = contrib > 0 ? contrib : 0;
+  double Fn = (I.Kn * pc) * area;
+  double sg = dist > 0 ? 1.0 : -1.0;
+  f[dir] += sg * Fn;
+  if (reac) reac[dir] -= (vol * sg) * Fn;
+  double rv[3] = {vel[0], vel[1], vel[2]};
+  rv[dir] -= speed;
+  if (I.allow_friction) {
+    int t1 = (dir + 1) % 3, t2 = (dir + 2) % 3;
+    double p1 = rv[t1], p2 = rv[t2];
+    double pn = sqrt(p1 * p1 + p2 * p2);
+    if (pn > 0) {
+      double u1 = p1 / pn, u2 = p2 / pn;
+      double ff1 = ((-I.mu) * fabs(Fn)) * u1;
+      double ff2 = ((-I.mu) * fabs(Fn)) * u2;
+      if (pn < ((I.mu * fabs(Fn)) * dt) / rho) {
+        ff1 = (((-pn) / rho) / dt) * u1;
+        ff2 = (((-pn) / rho) / dt) * u2;
+      }
+      f[t1] += ff1;
+      f[t2] += ff2;
+      if (reac) {
+        reac[t1] -= vol * ff1;
+        reac[t2] -= vol * ff2;
+      }
+    }
+  }
+  if (I.allow_damping) {
+    double dc = ((2 * I.zeta) * sqrt(I.Kn * rho)) * sqrt(fabs(area));
+    double fd = (-dc) * rv[dir];
+    f[dir] += fd;
+    if (reac) reac[dir] -= vol * fd;
+  }
+  return true;
+}
+
+// The six planes in the reference order x_min, x_max, y_min, y_max, z_min,
+// z_max (contact.h:1104-1121).  Returns the mask of planes in contact.
+__device__ __forceinline__ uint32_t wall_node(const DevInst& I, double dt, const double* cp,
+                                              const double* vel, double vol, double rho, double* f,
+                                              double* reac /* [18] or null */) {
+  uint32_t m = 0;
+  const int plane[6] = {0, 3, 1, 4, 2, 5};
+  for (int s = 0; s < 6; s++) {
+    int pl = plane[s];
+    int dir = pl % 3;
+    if (wall_side(I, dt, cp[dir] - I.wall[pl], dir, vel, vol, rho, f, I.speed[pl],
+                  reac ? reac + 3 * pl : nullptr))
+      m |= 1u << pl;
+  }
+  return m;
+}
+
+// Contact roles of a node pair (update_contact_force_by_boundary,
+// contact.h:746-942): ACTION = our node is P_C's node i; REACTION = our node
+// is P_N's node j; SELF = self-contact (action only, contact.h:779-796).
+enum Role { kAction = 0, kReaction = 1, kSelf = 2 };
+
+// Apply the terms of one contact pair to our node's force f.  `d` is the
+// reference's dir = (u_j - u_i) + (X_j - X_i) (P_C = i, P_N = j), `dn` its
+// norm (already known to satisfy dn <= Rc), len/K the reference length and
+// spring constant, VN = P_N.vol[j], VC = P_C.vol[i], rhoC = P_C.rho,
+// rv = P_C.vel[i] - P_N.vel[j].
+__device__ __forceinline__ void contact_terms(const DevInst& I, double dt, int role,
+                                              const double* d, double dn, double len, double K,
+                                              double VN, double VC, double rhoC,
+                                              const double* rv, double* f) {
+  if (!(dn < len)) return;
+  double u[3] = {d[0] / dn, d[1] / dn, d[2] / dn};
+  double s = (dn - len) / len;
+  double fc[3];
+  for (int k = 0; k < 3; k++) fc[k] = ((K * s) * VN) * u[k];
+  double fcn = nrm3(fc[0], fc[1], fc[2]);
+  if (role == kReaction) {
+    for (int k = 0; k < 3; k++) f[k] -= (fc[k] / VN) * VC;
+  } else {
+    for (int k = 0; k < 3; k++) f[k] += fc[k];
+  }
+  if (I.allow_damping) {
+    double pr = (rv[0] * u[0] + rv[1] * u[1]) + rv[2] * u[2];
+    double dc = ((2 * I.zeta) * sqrt(K * rhoC)) * sqrt(VN);
+    double fd[3];
+    for (int k = 0; k < 3; k++) fd[k] = (-dc) * (pr * u[k]);
+    if (role == kReaction) {
+      double sV = sqrt(VN), sC = sqrt(VC);
+      for (int k = 0; k < 3; k++) f[k] -= (fd[k] / sV) * sC;
+    } else {
+      for (int k = 0; k < 3; k++) f[k] += fd[k];
+    }
+  }
+  if (I.allow_friction) {
+    double pr = (rv[0] * u[0] + rv[1] * u[1]) + rv[2] * u[2];
+    double tv[3];
+    for (int k = 0; k < 3; k++) tv[k] = rv[k] - pr * u[k];
+    double tn = nrm3(tv[0], tv[1], tv[2]);
+    if (tn > 0) {
+      double tu[3], ff[3];
+      for (int k = 0; k < 3; k++) tu[k] = tv[k] / tn;
+      for (int k = 0; k < 3; k++) ff[k] = ((-I.mu) * fcn) * tu[k];
+      if (tn < ((I.mu * fcn) * dt) / rhoC)
+        for (int k = 0; k < 3; k++) ff[k] = (((-tn) / rhoC) / dt) * tu[k];
+      if (role == kReaction) {
+        for (int k = 0; k < 3; k++) f[k] -= ff[k];
+      } else {
+        for (int k = 0; k < 3; k++) f[k] += ff[k];
+      }
+    }
+  }
+}
+
+}  // namespace pd

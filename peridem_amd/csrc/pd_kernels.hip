@@ -1,0 +1,509 @@
+// pd_kernels.hip -- CDNA4 (gfx950) kernels of the explicit step.
+//
+// One step (timeloop.h:395-797) is the launch sequence
+//   k_position   Verlet position half + per-particle box of CurrPos
+//   k_pairs      particle-pair box test on the lagged boxes (timeloop.h:636-653)
+//   cell list    k_cellkey -> k_count -> scan -> k_scatter (contact candidates)
+//   k_force      PD bond force + fused irreversible break + self/wall/ext/pair
+//                contact, summed per node in the reference's order, no atomics
+//   k_update     Verlet velocity half, clamped nodes
+//   k_finalize   lagged boxes, wall motion
+// All floating point is FP64 with the reference's operation order; the build
+// uses -ffp-contract=off so no multiply-add is fused.
+#include "pd_device.h"
+
+namespace pd {
+
+struct StepArgs {
+  Dev d;
+  double dt;
+  double ext_scale;      // extforce ramp min(t/extforce_maxstep, 1) (timeloop.h:582-587)
+  uint32_t t;            // reference step index of this step
+  uint32_t want_reaction;
+};
+
+// ---------------------------------------------------------------------------
+// Verlet position half (timeloop.h:537-540; overloads.h:51-70,149-159):
+//   disp += dt*vel + (dt*dt*0.5)*acc ; CurrPos = pos + disp
+// plus the min/max of CurrPos that becomes the particle's lagged box at the
+// end of the step (timeloop.h:779).
+__global__ __launch_bounds__(256) void k_position(Dev d, double dt) {
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ unsigned long long smin[3][256], smax[3][256];
+  __shared__ uint32_t sp[256];
+  double cp[3] = {0, 0, 0};
+  uint32_t p = 0xFFFFFFFFu;
+  if (g < d.N) {
+    p = d.pid[g];
+    double4 u = d.U[g];
+    const double4 x = d.X[g];
+    const double h = (dt * dt) * 0.5;
+    double v0 = d.V[3 * g], v1 = d.V[3 * g + 1], v2 = d.V[3 * g + 2];
+    double a0 = d.A[3 * g], a1 = d.A[3 * g + 1], a2 = d.A[3 * g + 2];
+    u.x = u.x + ((dt * v0) + (h * a0));
+    u.y = u.y + ((dt * v1) + (h * a1));
+    u.z = u.z + ((dt * v2) + (h * a2));
+    d.U[g] = u;
+    cp[0] = x.x + u.x;
+    cp[1] = x.y + u.y;
+    cp[2] = x.z + u.z;
+    if (!d.part[p].movable) p = 0xFFFFFFFFu;
+  }
+  // block-level min/max per particle: nodes of one block belong to few
+  // particles; reduce runs of equal particle id through LDS.
+  int t = threadIdx.x;
+  sp[t] = p;
+  for (int k = 0; k < 3; k++) {
+    smin[k][t] = (p != 0xFFFFFFFFu) ? okey(cp[k]) : ~0ull;
+    smax[k][t] = (p != 0xFFFFFFFFu) ? okey(cp[k]) : 0ull;
+  }
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    unsigned long long mn[3], mx[3];
+    bool take = (t + off < 256) && sp[t + off] == p;
+    for (int k = 0; k < 3; k++) {
+      mn[k] = smin[k][t];
+      mx[k] = smax[k][t];
+      if (take) {
+        unsigned long long a = smin[k][t + off], b = smax[k][t + off];
+        mn[k] = a < mn[k] ? a : mn[k];
+        mx[k] = b > mx[k] ? b : mx[k];
+      }
+    }
+    __syncthreads();
+    for (int k = 0; k < 3; k++) {
+      smin[k][t] = mn[k];
+      smax[k][t] = mx[k];
+    }
+    __syncthreads();
+  }
+  // the first thread of each run of equal ids publishes
+  if (p != 0xFFFFFFFFu && (t == 0 || sp[t - 1] != p)) {
+    for (int k = 0; k < 3; k++) {
+      atomicMin(&d.lrtb_next[6 * p + k], smin[k][t]);
+      atomicMax(&d.lrtb_next[6 * p + 3 + k], smax[k][t]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pair box test on the lagged boxes, pair list in the reference's pair order
+// restricted to one instance (timeloop.h:243-255, 636-653).
+__global__ __launch_bounds__(256) void k_pairs(Dev d) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.npairs) return;
+  uint32_t i = d.pairs[2 * q], j = d.pairs[2 * q + 1];
+  const DevPart& Pi = d.part[i];
+  const DevPart& Pj = d.part[j];
+  if (!(Pi.movable || Pj.movable)) return;
+  const DevInst& I = d.inst[Pi.inst];
+  if (!intersects_box_fext(d.lrtb + 6 * i, d.lrtb + 6 * j, I.Rc)) return;
+  atomicAdd(&d.counters[0], 1ull);
+  // bit 1: the particle's nodes are contact candidates this step;
+  // bit 0: the particle receives contact terms (only stoppable ones do)
+  atomicOr(&d.pactive[i], 2u | (Pi.stoppable ? 1u : 0u));
+  atomicOr(&d.pactive[j], 2u | (Pj.stoppable ? 1u : 0u));
+}
+
+__device__ __forceinline__ bool self_active(const DevPart& P, const DevInst& I) {
+  return P.movable && I.enable_fracture && P.breakable && I.self_contact && P.stoppable;
+}
+
+__device__ __forceinline__ void cell_of(const double* cp, double cell, int64_t* c) {
+  for (int k = 0; k < 3; k++) c[k] = (int64_t)floor(cp[k] / cell);
+}
+
+// Cell keys of the nodes that can take part in contact this step.
+__global__ __launch_bounds__(256) void k_cellkey(Dev d) {
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.N) return;
+  uint32_t p = d.pid[g];
+  const DevPart& P = d.part[p];
+  const DevInst& I = d.inst[P.inst];
+  // a node is a candidate for others when its particle has an active pair
+  // (pactive marks only stoppable ones, so test the pair flag bit separately)
+  uint32_t key = kNoKey;
+  if (d.pactive[p] & 2u || self_active(P, I)) {
+    double4 x = d.X[g];
+    double4 u = d.U[g];
+    double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
+    int64_t c[3];
+    cell_of(cp, I.cell, c);
+    key = cell_hash(c[0], c[1], c[2], P.inst, d.T);
+    atomicAdd(&d.cstart[key], 1u);
+  }
+  d.ckey[g] = key;
+}
+
+// exclusive scan of cstart[0..T) (counts) in place; cstart[T] = total.
+// Three phases: per-block scan, scan of block sums, add-back.
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_scan1(uint32_t* a, uint32_t n, uint32_t* sums) {
+  __shared__ uint32_t s[kScanBlock];
+  uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+  uint32_t v = i < n ? a[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) a[i] = s[threadIdx.x] - v;
+  if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = s[threadIdx.x];
+}
+__global__ __launch_bounds__(kScanBlock) void k_scan2(uint32_t* sums, uint32_t nb, uint32_t* total) {
+  __shared__ uint32_t s[kScanBlock];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += kScanBlock) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < nb ? sums[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+      uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) sums[i] = s[threadIdx.x] - v + carry;
+    uint32_t blocktot = s[kScanBlock - 1];
+    __syncthreads();
+    carry += blocktot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+__global__ __launch_bounds__(kScanBlock) void k_scan3(uint32_t* a, uint32_t n, const uint32_t* sums,
+                                                      uint32_t* cur) {
+  uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+  if (i < n) {
+    uint32_t v = a[i] + sums[blockIdx.x];
+    a[i] = v;
+    cur[i] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(Dev d) {
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.N) return;
+  uint32_t key = d.ckey[g];
+  if (key == kNoKey) return;
+  uint32_t slot = atomicAdd(&d.ccur[key], 1u);
+  d.csorted[slot] = (uint32_t)g;
+}
+
+// Is j an intact bond of node g's row?  Rows hold the post-break state of
+// this step (the force loop above already erased this step's breaks), as
+// std::find over NbdArr[i] sees it (contact.h:783, timeloop.h:552-557).
+__device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) {
+  uint64_t s = g >> 6;
+  const uint32_t* row = d.bond + d.soff[s] + (g & 63);
+  uint32_t n = d.deg[g];
+  if (sorted) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      uint32_t e = row[(uint64_t)mid * kSlice] & kIdxMask;
+      if (e < j) lo = mid + 1;
+      else hi = mid;
+    }
+    // duplicates of j share one stretch and break together
+    for (uint32_t k = lo; k < n; k++) {
+      uint32_t e = row[(uint64_t)k * kSlice];
+      if ((e & kIdxMask) != j) break;
+      if (!(e & kBroken)) return true;
+    }
+    return false;
+  }
+  for (uint32_t k = 0; k < n; k++) {
+    uint32_t e = row[(uint64_t)k * kSlice];
+    if (e == j) return true;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// The force kernel: one thread per node, 64 consecutive nodes per wavefront
+// share one bond slice so the k-th entries of the 64 rows are one coalesced
+// 256-byte load.
+__global__ __launch_bounds__(256) void k_force(StepArgs a) {
+  const Dev& d = a.d;
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.N) return;
+  const double dt = a.dt;
+  const uint32_t p = d.pid[g];
+  const DevPart P = d.part[p];
+  const DevInst& I = d.inst[P.inst];
+  const double4 xi = d.X[g];
+  const double4 ui = d.U[g];
+  double f[3] = {0.0, 0.0, 0.0};
+
+  // (a) bond-based PD force + critical-stretch break
+  //     get_peridynamic_force (particle2.h:107-193), remove_bonds
+  //     (particle2.h:196-237): the stretch is tested against snot right after
+  //     the term is summed; a broken entry stops contributing from the next
+  //     step on, exactly like the reference's erase after the force pass.
+  if (P.movable) {
+    const bool can_break = I.enable_fracture && P.breakable;
+    const uint64_t s = g >> 6;
+    const uint64_t base = d.soff[s];
+    const uint32_t W = (uint32_t)((d.soff[s + 1] - base) >> 6);
+    uint32_t* bp = d.bond + base + (g & 63);
+    const double cnot = P.cnot, snot = P.snot;
+    for (uint32_t k = 0; k < W; k++) {
+      uint32_t e = bp[(uint64_t)k * kSlice];
+      if (e & kBroken) continue;
+      const double4 xj = d.X[e];
+      const double4 uj = d.U[e];
+      double x0 = xj.x - xi.x, x1 = xj.y - xi.y, x2 = xj.z - xi.z;   // xi (gen_xi)
+      double e0 = (uj.x - ui.x) + x0;
+      double e1 = (uj.y - ui.y) + x1;
+      double e2 = (uj.z - ui.z) + x2;
+      double nn = nrm3(e0, e1, e2);
+      double L = nrm3(x0, x1, x2);                                    // xi_norm
+      double str = (nn - L) / L;
+      double c = (cnot * str) * xj.w;
+      f[0] += c * (e0 / nn);
+      f[1] += c * (e1 / nn);
+      f[2] += c * (e2 / nn);
+      if (can_break && str > snot) bp[(uint64_t)k * kSlice] = e | kBroken;
+    }
+  }
+
+  const bool selfc = self_active(P, I);
+  const bool pairc = (d.pactive[p] & 1u) != 0;
+  const double cpi[3] = {xi.x + ui.x, xi.y + ui.y, xi.z + ui.z};
+  const double vi[3] = {d.V[3 * g], d.V[3 * g + 1], d.V[3 * g + 2]};
+
+  // candidate partners within R_c, kept sorted by the reference's
+  // accumulation order: same particle first (q ascending), then other
+  // particles by global node index (= particle order, then node order).
+  uint32_t cand[kMaxCand];
+  uint32_t ncand = 0;
+  if (selfc || pairc) {
+    int64_t c[3];
+    cell_of(cpi, I.cell, c);
+    uint32_t seen[27];
+    int nseen = 0;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          uint32_t key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
+          bool dup = false;
+          for (int q = 0; q < nseen; q++) dup |= (seen[q] == key);
+          if (dup) continue;
+          seen[nseen++] = key;
+          uint32_t b0 = d.cstart[key], b1 = d.cstart[key + 1];
+          for (uint32_t it = b0; it < b1; it++) {
+            uint32_t j = d.csorted[it];
+            if (j == g) continue;
+            uint32_t pj = d.pid[j];
+            const DevPart& Q = d.part[pj];
+            if (Q.inst != P.inst) continue;
+            if (pj == p) {
+              if (!selfc) continue;
+            } else {
+              if (!pairc) continue;
+              uint32_t lo = pj < p ? pj : p, hi = pj < p ? p : pj;
+              if (!(d.part[lo].movable || d.part[hi].movable)) continue;
+              if (!intersects_box_fext(d.lrtb + 6 * lo, d.lrtb + 6 * hi, I.Rc)) continue;
+            }
+            // exact reference distance test, in the reference orientation
+            const double4 xj = d.X[j];
+            const double4 uj = d.U[j];
+            double dd[3];
+            if (pj < p) {   // we are P_N's node j
+              dd[0] = (ui.x - uj.x) + (xi.x - xj.x);
+              dd[1] = (ui.y - uj.y) + (xi.y - xj.y);
+              dd[2] = (ui.z - uj.z) + (xi.z - xj.z);
+            } else {        // we are P_C's node i
+              dd[0] = (uj.x - ui.x) + (xj.x - xi.x);
+              dd[1] = (uj.y - ui.y) + (xj.y - xi.y);
+              dd[2] = (uj.z - ui.z) + (xj.z - xi.z);
+            }
+            double dn = nrm3(dd[0], dd[1], dd[2]);
+            if (!(dn <= I.Rc)) continue;
+            uint32_t keyj = (pj == p ? 0u : kBroken) | j;
+            if (ncand == kMaxCand) {
+              atomicOr(d.err, kErrCandOverflow);
+              continue;
+            }
+            uint32_t w = ncand++;
+            while (w > 0 && cand[w - 1] > keyj) {
+              cand[w] = cand[w - 1];
+              w--;
+            }
+            cand[w] = keyj;
+          }
+        }
+  }
+
+  uint32_t ci = 0;
+  // (b) self-contact, action only (contact.h:779-796 with is_self_contact=1)
+  if (selfc) {
+    for (; ci < ncand && !(cand[ci] & kBroken); ci++) {
+      uint32_t j = cand[ci];
+      if (bonded_alive(d, g, j, P.rows_sorted)) continue;
+      const double4 xj = d.X[j];
+      const double4 uj = d.U[j];
+      double xx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+      double dd[3] = {(uj.x - ui.x) + xx[0], (uj.y - ui.y) + xx[1], (uj.z - ui.z) + xx[2]};
+      double dn = nrm3(dd[0], dd[1], dd[2]);
+      double xin = nrm3(xx[0], xx[1], xx[2]);
+      double len = I.Rc, K = I.Kn * I.Rc;
+      if (xin < I.Rc) {
+        len = xin;
+        K = P.cnot;
+      }
+      double rv[3] = {vi[0] - d.V[3 * j], vi[1] - d.V[3 * j + 1], vi[2] - d.V[3 * j + 2]};
+      contact_terms(I, dt, kSelf, dd, dn, len, K, xj.w, xi.w, P.rho, rv, f);
+    }
+  }
+  while (ci < ncand && !(cand[ci] & kBroken)) ci++;
+
+  // (c) wall contact (timeloop.h:561-570, contact.h:1060-1124)
+  if (P.movable && P.stoppable && I.allow_wall &&
+      !within_interior_fext(d.lrtb + 6 * p, I.wall, I.Rc)) {
+    uint32_t m = wall_node(I, dt, cpi, vi, xi.w, P.rho, f, nullptr);
+    if (a.want_reaction) {
+      d.wmask[g] = (uint8_t)m;
+      if (m) d.bflag[g >> 8] = 1;
+    }
+  }
+
+  // (d) external force density (timeloop.h:581-597)
+  if (P.movable) {
+    if (I.gradient_extforce) {
+      double sc = a.ext_scale;
+      f[0] += sc * d.E[3 * g];
+      f[1] += sc * d.E[3 * g + 1];
+      f[2] += sc * d.E[3 * g + 2];
+    } else {
+      f[0] += d.E[3 * g];
+      f[1] += d.E[3 * g + 1];
+      f[2] += d.E[3 * g + 2];
+    }
+  }
+
+  // (e) inter-particle contact in pair order (timeloop.h:636-651)
+  for (; ci < ncand; ci++) {
+    uint32_t j = cand[ci] & kIdxMask;
+    uint32_t pj = d.pid[j];
+    const DevPart& Q = d.part[pj];
+    const double4 xj = d.X[j];
+    const double4 uj = d.U[j];
+    const double vj[3] = {d.V[3 * j], d.V[3 * j + 1], d.V[3 * j + 2]};
+    double dd[3], rv[3];
+    if (pj < p) {
+      // partner is P_C (node i), we are P_N (node j): reaction terms
+      for (int k = 0; k < 3; k++) rv[k] = vj[k] - vi[k];
+      dd[0] = (ui.x - uj.x) + (xi.x - xj.x);
+      dd[1] = (ui.y - uj.y) + (xi.y - xj.y);
+      dd[2] = (ui.z - uj.z) + (xi.z - xj.z);
+      double dn = nrm3(dd[0], dd[1], dd[2]);
+      contact_terms(I, dt, kReaction, dd, dn, I.Rc, I.Kn * I.Rc, xi.w, xj.w, Q.rho, rv, f);
+    } else {
+      for (int k = 0; k < 3; k++) rv[k] = vi[k] - vj[k];
+      dd[0] = (uj.x - ui.x) + (xj.x - xi.x);
+      dd[1] = (uj.y - ui.y) + (xj.y - xi.y);
+      dd[2] = (uj.z - ui.z) + (xj.z - xi.z);
+      double dn = nrm3(dd[0], dd[1], dd[2]);
+      contact_terms(I, dt, kAction, dd, dn, I.Rc, I.Kn * I.Rc, xj.w, xi.w, P.rho, rv, f);
+    }
+  }
+
+  d.F[3 * g] = f[0];
+  d.F[3 * g + 1] = f[1];
+  d.F[3 * g + 2] = f[2];
+}
+
+// ---------------------------------------------------------------------------
+// Verlet velocity half (timeloop.h:662-682) and clamped nodes (:767-774).
+__global__ __launch_bounds__(256) void k_update(Dev d, double dt) {
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.N) return;
+  const DevPart& P = d.part[d.pid[g]];
+  if (!P.movable) return;
+  const double hdt = 0.5 * dt;
+  if (d.nflag[g] & 1u) {
+    d.U[g] = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int k = 0; k < 3; k++) d.V[3 * g + k] = d.A[3 * g + k] = 0.0;
+    return;
+  }
+  for (int k = 0; k < 3; k++) {
+    double aold = d.A[3 * g + k];
+    double anew = P.rinv * d.F[3 * g + k];
+    d.A[3 * g + k] = anew;
+    d.V[3 * g + k] = d.V[3 * g + k] + ((aold + anew) * hdt);
+  }
+}
+
+// Lagged boxes of movable particles (timeloop.h:779), reset of the per-step
+// flags, wall motion (timeloop.h:784-797).
+__global__ __launch_bounds__(256) void k_finalize(Dev d, double dt) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < d.P) {
+    if (d.part[q].movable)
+      for (int k = 0; k < 6; k++) d.lrtb[6 * q + k] = okey_inv(d.lrtb_next[6 * q + k]);
+    for (int k = 0; k < 3; k++) {
+      d.lrtb_next[6 * q + k] = ~0ull;
+      d.lrtb_next[6 * q + 3 + k] = 0ull;
+    }
+    d.pactive[q] = 0;
+  }
+  if (q < d.ninst) {
+    DevInst& I = d.inst[q];
+    for (int k = 0; k < 6; k++) I.wall[k] += I.speed[k] * dt;
+  }
+}
+
+// Wall reaction of one step in the reference order (particle, node, plane,
+// term), recomputed from the same device function as the force.  One thread
+// per instance walks its node range; blocks without a wall contact are
+// skipped through bflag.
+__global__ void k_reaction(Dev d, double dt, const uint64_t* inst_nodes /* [ninst+1] */) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.ninst) return;
+  DevInst I = d.inst[q];
+  double reac[18];
+  for (int k = 0; k < 18; k++) reac[k] = 0.0;
+  uint64_t g0 = inst_nodes[q], g1 = inst_nodes[q + 1];
+  for (uint64_t g = g0; g < g1;) {
+    uint64_t blk = g >> 8;
+    if (!d.bflag[blk]) {
+      g = (blk + 1) << 8;
+      continue;
+    }
+    uint64_t gend = ((blk + 1) << 8) < g1 ? ((blk + 1) << 8) : g1;
+    for (; g < gend; g++) {
+      if (!d.wmask[g]) continue;
+      const DevPart& P = d.part[d.pid[g]];
+      double4 x = d.X[g];
+      double4 u = d.U[g];
+      double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
+      double v[3] = {d.V[3 * g], d.V[3 * g + 1], d.V[3 * g + 2]};
+      double fdummy[3] = {0, 0, 0};
+      wall_node(I, dt, cp, v, x.w, P.rho, fdummy, reac);
+      d.wmask[g] = 0;
+    }
+  }
+  for (int k = 0; k < 18; k++) d.reaction[18 * q + k] = reac[k];
+}
+
+__global__ void k_clear_bflag(Dev d, uint64_t n) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) d.bflag[q] = 0;
+}
+
+// alive directed bonds (for counters and read-back)
+__global__ __launch_bounds__(256) void k_count_alive(Dev d, uint64_t nent, unsigned long long* out) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t alive = 0;
+  if (q < nent) alive = (d.bond[q] & kBroken) ? 0u : 1u;
+  // wavefront reduction
+  for (int off = 32; off > 0; off >>= 1) alive += __shfl_down(alive, off, 64);
+  if ((threadIdx.x & 63) == 0 && alive) atomicAdd(out, (unsigned long long)alive);
+}
+
+}  // namespace pd

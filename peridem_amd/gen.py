@@ -1,0 +1,325 @@
+"""Synthetic setups for the explicit step: seeded jittered-lattice particles.
+
+The reference builds its setup file offline (exp_dict.py: gmsh meshes,
+`cKDTree.query_pairs(delta)` bonds sorted lexicographically, exp_dict.py:544-583,
+written by Experiment.save, exp_dict.py:2119-2186).  This module produces
+the same data model without gmsh: nodes are the points of a cubic lattice
+(spacing h) inside a shape, jittered uniformly by +-0.05h with a seeded
+generator, each with volume h^3; bonds are all pairs with |X_j - X_i| <= delta,
+emitted as (i<j) rows in lexicographic order, so conn2NArr rows come out
+ascending exactly as they do for the reference's sorted query_pairs output.
+
+Material constants follow the 3-D PMB closed forms of material_dict.py:266-277
+(peridem_3d): c0 = 18K/(pi delta^4), s0 = sqrt(5 Gc / (9 K delta)), with the
+bulk modulus K taken from the particle's own E and nu.  The contact normal
+stiffness is K_n = 18K/(pi R_c^5), i.e. c0 evaluated at delta = R_c divided by
+R_c, the convention stated at contact.h:775-776.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .abi import ParticleDesc, RunDesc
+
+
+# ---------------------------------------------------------------------------
+# material and contact constants
+def bulk_modulus(E, nu):
+    return E / (3.0 * (1.0 - 2.0 * nu))
+
+
+def pmb3d(E, nu, Gc, delta):
+    """(cnot, snot) of the 3-D bond-based prototype microelastic brittle model."""
+    K = bulk_modulus(E, nu)
+    return 18.0 * K / (math.pi * delta ** 4), math.sqrt(5.0 * Gc / (9.0 * K * delta))
+
+
+def normal_stiffness(E, nu, Rc):
+    return 18.0 * bulk_modulus(E, nu) / (math.pi * Rc ** 5)
+
+
+# ---------------------------------------------------------------------------
+@dataclass
+class Particle:
+    pos: np.ndarray
+    vol: np.ndarray
+    conn: np.ndarray
+    delta: float
+    rho: float
+    cnot: float
+    snot: float
+    disp: np.ndarray | None = None
+    vel: np.ndarray | None = None
+    acc: np.ndarray | None = None
+    ext: np.ndarray | None = None
+    clamped: np.ndarray | None = None
+    bdry: np.ndarray | None = None
+    movable: int = 1
+    breakable: int = 1
+    stoppable: int = 1
+    torque_axis: int = 2
+    torque_val: float = 0.0
+    instance: int = 0
+
+    @property
+    def n(self):
+        return int(self.pos.shape[0])
+
+    @property
+    def nbonds(self):
+        return 2 * int(self.conn.shape[0])
+
+    def desc(self, keep):
+        def f64(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_double))
+
+        def u32(a):
+            if a is None or len(a) == 0:
+                return None, 0
+            a = np.ascontiguousarray(a, dtype=np.uint32)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_uint32)), a.size
+
+        d = ParticleDesc()
+        d.nnodes = self.n
+        d.instance = self.instance
+        d.pos, d.vol = f64(self.pos), f64(self.vol)
+        d.disp, d.vel, d.acc, d.extforce = f64(self.disp), f64(self.vel), f64(self.acc), f64(self.ext)
+        cp, cn = u32(self.conn.reshape(-1))
+        d.conn, d.nconn = cp, cn // 2
+        d.clamped, d.nclamped = u32(self.clamped)
+        d.bdry, d.nbdry = u32(self.bdry)
+        d.delta, d.rho, d.cnot, d.snot = self.delta, self.rho, self.cnot, self.snot
+        d.movable, d.breakable, d.stoppable = self.movable, self.breakable, self.stoppable
+        d.torque_axis, d.torque_val = self.torque_axis, self.torque_val
+        return d
+
+
+@dataclass
+class System:
+    particles: list
+    dt: float = 1e-8
+    contact_rad: float = 0.0
+    normal_stiffness: float = 0.0
+    friction_coefficient: float = 0.3
+    damping_ratio: float = 0.1
+    enable_fracture: int = 1
+    self_contact: int = 0
+    allow_damping: int = 1
+    allow_friction: int = 1
+    nl_bdry_only: int = 0
+    compute_wall_reaction: int = 0
+    gradient_extforce: int = 0
+    extforce_maxstep: int = 0
+    enable_torque: int = 0
+    allow_wall: int = 0
+    wall_lrtb: tuple = (0, 0, 0, 0, 0, 0)
+    wall_speed: tuple = (0, 0, 0, 0, 0, 0)
+    first_step: int = 1
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    def run_desc(self):
+        r = RunDesc()
+        for k in ("dt", "contact_rad", "normal_stiffness", "friction_coefficient", "damping_ratio"):
+            setattr(r, k, float(getattr(self, k)))
+        for k in ("enable_fracture", "self_contact", "allow_damping", "allow_friction",
+                  "nl_bdry_only", "compute_wall_reaction", "gradient_extforce",
+                  "extforce_maxstep", "enable_torque", "allow_wall", "first_step"):
+            setattr(r, k, int(getattr(self, k)))
+        for k in range(6):
+            r.wall_lrtb[k] = float(self.wall_lrtb[k])
+            r.wall_speed[k] = float(self.wall_speed[k])
+        return r
+
+    @property
+    def nodes(self):
+        return sum(p.n for p in self.particles)
+
+    @property
+    def bonds(self):
+        return sum(p.nbonds for p in self.particles)
+
+    def movable_bonds(self):
+        return sum(p.nbonds for p in self.particles if p.movable)
+
+
+# ---------------------------------------------------------------------------
+# lattice shapes (integer coordinates, C order)
+def lattice_sphere(r_over_h):
+    m = int(math.floor(r_over_h))
+    ax = np.arange(-m, m + 1)
+    I, J, K = np.meshgrid(ax, ax, ax, indexing="ij")
+    pts = np.stack([I.ravel(), J.ravel(), K.ravel()], 1)
+    keep = (pts ** 2).sum(1) <= r_over_h ** 2 + 1e-9
+    return pts[keep]
+
+
+def lattice_cube(half_over_h):
+    m = int(math.floor(half_over_h))
+    ax = np.arange(-m, m + 1)
+    I, J, K = np.meshgrid(ax, ax, ax, indexing="ij")
+    return np.stack([I.ravel(), J.ravel(), K.ravel()], 1)
+
+
+def lattice_convex(r_over_h, rng, nplanes=14):
+    """Random convex polyhedron: a ball cut by random tangent-ish planes."""
+    pts = lattice_sphere(r_over_h)
+    normals = rng.normal(size=(nplanes, 3))
+    normals /= np.linalg.norm(normals, axis=1, keepdims=True)
+    offs = r_over_h * rng.uniform(0.55, 0.95, size=nplanes)
+    keep = np.all(pts @ normals.T <= offs[None, :] + 1e-9, axis=1)
+    return pts[keep]
+
+
+def bonds_lattice(L, X, delta, h, jitter=0.05):
+    """All pairs (i<j) with |X_j-X_i| <= delta, lexicographically sorted.
+
+    L: integer lattice coordinates of the nodes, X: jittered positions.
+    Candidates are lattice offsets up to delta/h plus the largest possible
+    jitter change of a distance; the exact test is on the jittered positions.
+    """
+    n = L.shape[0]
+    lo = L.min(0)
+    Ls = L - lo
+    reach = int(math.ceil(delta / h + 2 * jitter * math.sqrt(3) + 1e-9))
+    pad = reach + 1
+    shape = tuple(int(v) for v in (Ls.max(0) + 1 + 2 * pad))
+    grid = -np.ones(shape, dtype=np.int64)
+    Lp = Ls + pad
+    grid[Lp[:, 0], Lp[:, 1], Lp[:, 2]] = np.arange(n)
+    lim2 = (delta / h + 2 * jitter * math.sqrt(3)) ** 2 + 1e-9
+    keys = []
+    rng_ = range(-reach, reach + 1)
+    for ox in rng_:
+        for oy in rng_:
+            for oz in rng_:
+                o = (ox, oy, oz)
+                if o <= (0, 0, 0):
+                    continue
+                if ox * ox + oy * oy + oz * oz > lim2:
+                    continue
+                j = grid[Lp[:, 0] + ox, Lp[:, 1] + oy, Lp[:, 2] + oz]
+                ok = j >= 0
+                i = np.nonzero(ok)[0]
+                j = j[ok]
+                d = X[j] - X[i]
+                dist = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+                sel = dist <= delta
+                i, j = i[sel], j[sel]
+                a, b = np.minimum(i, j), np.maximum(i, j)
+                keys.append((a.astype(np.uint64) << np.uint64(32)) | b.astype(np.uint64))
+    if not keys:
+        return np.zeros((0, 2), dtype=np.uint32)
+    k = np.sort(np.concatenate(keys))
+    return np.stack([(k >> np.uint64(32)).astype(np.uint32), (k & np.uint64(0xFFFFFFFF)).astype(np.uint32)], 1)
+
+
+def make_particle(L, h, center, rng, E=72e9, nu=0.25, rho=2440.0, Gc=135.0, horizon=3.0,
+                  jitter=0.05, **kw):
+    X = center[None, :] + h * (L + rng.uniform(-jitter, jitter, size=L.shape))
+    delta = horizon * h
+    cnot, snot = pmb3d(E, nu, Gc, delta)
+    conn = bonds_lattice(L, X, delta, h, jitter)
+    n = L.shape[0]
+    return Particle(pos=X, vol=np.full(n, h ** 3), conn=conn, delta=delta, rho=rho, cnot=cnot,
+                    snot=snot, **kw)
+
+
+# ---------------------------------------------------------------------------
+# the named workloads
+def two_sphere_impact(R=1e-3, h=1e-4, v=20.0, seed=1, E=72e9, nu=0.25, rho=2440.0, Gc=135.0,
+                      gap_h=1.1, b_movable=0, **sys_kw):
+    """Sphere A at v m/s toward a fixed sphere B (surface gap gap_h*h)."""
+    rng = np.random.default_rng(seed)
+    L = lattice_sphere(R / h)
+    A = make_particle(L, h, np.array([0.0, 0.0, 0.0]), rng, E, nu, rho, Gc)
+    B = make_particle(L, h, np.array([2 * R + gap_h * h, 0.0, 0.0]), rng, E, nu, rho, Gc,
+                      movable=b_movable)
+    A.vel = np.zeros((A.n, 3))
+    A.vel[:, 0] = v
+    Rc = 0.9 * h
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(E, nu, Rc),
+              friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=0,
+              name="two_sphere_impact")
+    kw.update(sys_kw)
+    return System(particles=[A, B], **kw)
+
+
+def _random_centres(n, R, box, gap, rng, margin):
+    cs = []
+    tries = 0
+    while len(cs) < n:
+        tries += 1
+        if tries > 2_000_000:
+            raise RuntimeError("could not place particles")
+        c = rng.uniform(margin + R, box - margin - R, size=3)
+        if all(np.linalg.norm(c - q) >= 2 * R + gap for q in cs):
+            cs.append(c)
+    return np.array(cs)
+
+
+def gravity_settling(nspheres=125, R=1e-3, nodes_per_sphere=8000, box=1.2e-2, seed=2,
+                     E=72e9, nu=0.25, rho=2440.0, Gc=135.0, g=9.81, **sys_kw):
+    """Spheres at random non-overlapping centres in a walled box under gravity."""
+    rng = np.random.default_rng(seed)
+    h = R * (4.0 * math.pi / 3.0 / nodes_per_sphere) ** (1.0 / 3.0)
+    L = lattice_sphere(R / h)
+    Rc = 0.9 * h
+    cs = _random_centres(nspheres, R, box, 2 * h, rng, margin=2 * h)
+    parts = []
+    for c in cs:
+        p = make_particle(L, h, c, rng, E, nu, rho, Gc)
+        p.ext = np.zeros((p.n, 3))
+        p.ext[:, 2] = -g * rho
+        parts.append(p)
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(E, nu, Rc),
+              friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=0,
+              allow_wall=1, wall_lrtb=(0, 0, 0, box, box, box), name="gravity_settling")
+    kw.update(sys_kw)
+    s = System(particles=parts, **kw)
+    s.meta["h"] = h
+    return s
+
+
+def crushing_aggregate(nparticles=512, nodes_range=(2000, 12000), seed=3, E=72e9, nu=0.25,
+                       rho=2440.0, Gc=135.0, h=6e-5, top_speed=-5.0, **sys_kw):
+    """Mixed cubes, spheres and convex polyhedra packed on a grid, top wall moving down."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    sizes = []
+    for k in range(nparticles):
+        target = float(np.exp(rng.uniform(np.log(nodes_range[0]), np.log(nodes_range[1]))))
+        kind = rng.integers(0, 3)
+        if kind == 0:
+            half = 0.5 * target ** (1 / 3) - 0.5
+            L = lattice_cube(half)
+        elif kind == 1:
+            L = lattice_sphere((3 * target / (4 * math.pi)) ** (1 / 3))
+        else:
+            L = lattice_convex((3 * target / (4 * math.pi)) ** (1 / 3) * 1.25, rng)
+        sizes.append(L)
+    ext = max(int(np.abs(L).max()) for L in sizes) * 2 + 4
+    pitch = ext * h
+    m = int(math.ceil(nparticles ** (1 / 3)))
+    Rc = 0.9 * h
+    for k, L in enumerate(sizes):
+        ix, iy, iz = k % m, (k // m) % m, k // (m * m)
+        c = np.array([(ix + 0.5) * pitch, (iy + 0.5) * pitch, (iz + 0.5) * pitch])
+        parts.append(make_particle(L, h, c, rng, E, nu, rho, Gc))
+    top = (nparticles + m * m - 1) // (m * m)
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(E, nu, Rc),
+              friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=1,
+              allow_wall=1, compute_wall_reaction=1,
+              wall_lrtb=(0, 0, 0, m * pitch, m * pitch, top * pitch),
+              wall_speed=(0, 0, 0, 0, 0, top_speed), name="crushing_aggregate")
+    kw.update(sys_kw)
+    return System(particles=parts, **kw)

@@ -1,0 +1,91 @@
+"""Shared helpers of the parity tests: small seeded systems and comparisons."""
+import numpy as np
+
+from peridem_amd import gen
+
+FIELDS = ("disp", "currpos", "vel", "acc", "force")
+
+
+def assert_bitwise(a, b, what):
+    """Bitwise equality of two float arrays (NaNs compare equal by bit pattern)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    ua, ub = a.view(np.uint64), b.view(np.uint64)
+    bad = np.nonzero(ua != ub)
+    if bad[0].size:
+        i = tuple(x[0] for x in bad)
+        rel = np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+        raise AssertionError(f"{what}: {bad[0].size} entries differ; first at {i}: "
+                             f"{a[i]!r} vs {b[i]!r}; max rel diff {rel:.3e}")
+
+
+def compare_engines(ga, oa, sysm, label, conn=True):
+    for p in range(len(sysm.particles)):
+        sg, so = ga.state(p), oa.state(p)
+        for k in FIELDS:
+            assert_bitwise(sg[k], so[k], f"{label} particle {p} {k}")
+        if conn:
+            cg, co = ga.connectivity(p), oa.connectivity(p)
+            assert cg.shape == co.shape and np.array_equal(cg, co), \
+                f"{label} particle {p} connectivity: {cg.shape} vs {co.shape}"
+    lg, rg = ga.wall()
+    lo, ro = oa.wall()
+    assert_bitwise(lg, lo, f"{label} wall planes")
+    cg, co = ga.counters(), oa.counters()
+    for k in ("steps", "bond_updates", "pair_computations", "alive_bonds", "total_bonds"):
+        assert cg[k] == co[k], f"{label} counter {k}: {cg[k]} vs {co[k]}"
+    return rg, ro
+
+
+def small_impact(seed=5, R_h=4.0, v=150.0, self_contact=1, walls=True, b_movable=1,
+                 fracture=1, gap_h=1.0, reaction=1):
+    """Two small spheres colliding fast inside a moving walled box: exercises
+    bond breaking, self-contact, pair contact, wall contact and reaction."""
+    h = 1e-4
+    s = gen.two_sphere_impact(R=R_h * h, h=h, v=v, seed=seed, gap_h=gap_h, b_movable=b_movable,
+                              self_contact=self_contact, enable_fracture=fracture)
+    if walls:
+        lo = -R_h * h - 0.5 * h
+        hi = 2 * R_h * h + gap_h * h + R_h * h + 0.5 * h
+        s.allow_wall = 1
+        s.compute_wall_reaction = reaction
+        s.wall_lrtb = (lo, -R_h * h - 0.6 * h, -R_h * h - 0.7 * h, hi, R_h * h + 0.6 * h, R_h * h + 0.7 * h)
+        s.wall_speed = (0.0, 5.0, 0.0, -20.0, 0.0, -3.0)
+    rng = np.random.default_rng(seed + 100)
+    B = s.particles[1]
+    B.vel = rng.uniform(-5, 5, size=(B.n, 3))
+    return s
+
+
+def mixed_system(seed=7):
+    """Several shapes with mixed flags, clamped nodes, gradient extforce, walls."""
+    rng = np.random.default_rng(seed)
+    h = 1e-4
+    parts = []
+    shapes = [gen.lattice_sphere(3.2), gen.lattice_cube(2.0), gen.lattice_convex(3.6, rng),
+              gen.lattice_sphere(2.6)]
+    centres = [(0, 0, 0), (6.3e-4, 0.2e-4, 0), (0.1e-4, 7.0e-4, 0.3e-4), (6.2e-4, 6.9e-4, -0.2e-4)]
+    flags = [(1, 1, 1), (1, 0, 1), (0, 1, 1), (1, 1, 0)]
+    for L, c, (mv, br, st) in zip(shapes, centres, flags):
+        p = gen.make_particle(L, h, np.array(c), rng, movable=mv, breakable=br, stoppable=st)
+        p.vel = rng.uniform(-2, 2, size=(p.n, 3))
+        p.acc = rng.uniform(-1e4, 1e4, size=(p.n, 3))
+        p.disp = rng.uniform(-1e-6, 1e-6, size=(p.n, 3))
+        p.ext = np.zeros((p.n, 3))
+        p.ext[:, 2] = -9.81 * p.rho
+        parts.append(p)
+    parts[0].clamped = np.array([0, 5, 17], dtype=np.uint32)
+    Rc = 0.9 * h
+    # centres of mass drift toward each other
+    parts[0].vel[:, 0] += 60
+    parts[1].vel[:, 0] -= 60
+    parts[2].vel[:, 1] -= 60
+    parts[3].vel[:, 1] -= 30
+    s = gen.System(particles=parts, dt=1e-8, contact_rad=Rc,
+                   normal_stiffness=gen.normal_stiffness(72e9, 0.25, Rc), friction_coefficient=0.3,
+                   damping_ratio=0.1, enable_fracture=1, self_contact=1, allow_wall=1,
+                   compute_wall_reaction=1, gradient_extforce=1, extforce_maxstep=37,
+                   wall_lrtb=(-4.2e-4, -4.1e-4, -3.6e-4, 11.3e-4, 11.4e-4, 3.7e-4),
+                   wall_speed=(1.0, 0.0, 0.0, 0.0, -2.0, -1.0), name="mixed")
+    return s
