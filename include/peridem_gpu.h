@@ -157,6 +157,23 @@ int pd_get_wall(pd_ctx* ctx, uint32_t instance, double lrtb[6], double reaction[
 int pd_get_counters(pd_ctx* ctx, pd_counters* out);
 /* Time of the last pd_step call on the device (HIP events), milliseconds. */
 int pd_last_step_ms(pd_ctx* ctx, double* ms);
+
+/* Per-phase device time, measured with HIP events on the context's stream
+ * while profiling is on (each step then ends with a host wait, so profiled
+ * runs are for kernel durations, not for throughput).  No counterpart in
+ * the reference, which only times whole modulo blocks (timeloop.h:909-914). */
+#define PD_PHASE_POSITION 0 /* Verlet position half + boxes */
+#define PD_PHASE_CONTACT_LIST 1 /* pair box test + cell-list rebuild */
+#define PD_PHASE_BOND 2     /* bond force + fused break (the hot kernel) */
+#define PD_PHASE_CONTACT 3  /* self/wall/ext/pair contact */
+#define PD_PHASE_UPDATE 4   /* Verlet velocity half + lagged boxes + walls */
+#define PD_NPHASE 5
+typedef struct pd_profile {
+  double ms[PD_NPHASE];
+  uint64_t launches[PD_NPHASE];
+} pd_profile;
+int pd_set_profiling(pd_ctx* ctx, int on);
+int pd_get_profile(pd_ctx* ctx, pd_profile* out); /* resets the accumulators */
 const char* pd_last_error(pd_ctx* ctx);
 void pd_destroy(pd_ctx* ctx);
 

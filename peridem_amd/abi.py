@@ -69,6 +69,15 @@ SYMBOLS = ["create", "add_particle", "commit", "step", "set_flag", "set_wall_spe
            "get_connectivity", "get_wall", "get_counters", "last_error", "destroy"]
 
 
+class Profile(C.Structure):
+    _fields_ = [("ms", _dbl * 5), ("launches", _u64 * 5)]
+    PHASES = ("position", "contact_list", "bond", "contact", "update")
+
+    def as_dict(self):
+        return {n: {"ms": self.ms[i], "launches": int(self.launches[i])}
+                for i, n in enumerate(self.PHASES)}
+
+
 class PDError(RuntimeError):
     pass
 
@@ -114,6 +123,10 @@ def load_gpu_lib(path=GPU_LIB):
         lib.pd_version.restype = C.c_char_p
         lib.pd_last_step_ms.restype = C.c_int
         lib.pd_last_step_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        lib.pd_set_profiling.argtypes = [C.c_void_p, C.c_int]
+        lib.pd_set_profiling.restype = C.c_int
+        lib.pd_get_profile.argtypes = [C.c_void_p, C.POINTER(Profile)]
+        lib.pd_get_profile.restype = C.c_int
         _LIBS[path] = (lib, _bind(lib, "pd_"))
     return _LIBS[path]
 
@@ -198,6 +211,14 @@ class Engine:
         ms = C.c_double()
         self._check(self.lib.pd_last_step_ms(self.ctx, C.byref(ms)), "last_step_ms")
         return ms.value
+
+    def set_profiling(self, on=True):
+        self._check(self.lib.pd_set_profiling(self.ctx, int(on)), "set_profiling")
+
+    def profile(self):
+        pr = Profile()
+        self._check(self.lib.pd_get_profile(self.ctx, C.byref(pr)), "get_profile")
+        return pr.as_dict()
 
     def close(self):
         if self.ctx:

@@ -66,6 +66,9 @@ struct pd_ctx {
   bool reaction_valid = false;
   std::vector<uint64_t> movable_bonds;  // per particle directed bonds
   uint64_t steps_since_set = 0;
+  bool prof = false;
+  hipEvent_t pev[PD_NPHASE + 1] = {};
+  pd_profile acc{};
 };
 
 static int fail(pd_ctx* c, int code, const char* fmt, ...) {
@@ -443,7 +446,14 @@ static int launch_step(pd_ctx* c, bool want_reaction) {
   hipStream_t st = c->stream;
   const double dt = c->insts[0].run.dt;
   const uint64_t Np = d.S * kSlice;
+  auto mark = [&](int k) -> int {
+    if (c->prof) HIPCHK(c, hipEventRecord(c->pev[k], st));
+    return PD_OK;
+  };
+  int rc;
+  if ((rc = mark(0))) return rc;
   hipLaunchKernelGGL(k_position, dim3(grid_of(Np, 256)), dim3(256), 0, st, d, dt);
+  if ((rc = mark(1))) return rc;
   if (d.npairs) hipLaunchKernelGGL(k_pairs, dim3(grid_of(d.npairs, 256)), dim3(256), 0, st, d);
   if (any_contact_possible(c)) {
     HIPCHK(c, hipMemsetAsync(d.cstart, 0, ((size_t)d.T + 1) * sizeof(uint32_t), st));
@@ -466,13 +476,27 @@ static int launch_step(pd_ctx* c, bool want_reaction) {
   if (want_reaction)
     hipLaunchKernelGGL(k_clear_bflag, dim3(grid_of(Np / 256 + 1, 256)), dim3(256), 0, st, d,
                        (uint64_t)(Np / 256 + 1));
-  hipLaunchKernelGGL(k_force, dim3(grid_of(Np, 256)), dim3(256), 0, st, a);
+  if ((rc = mark(2))) return rc;
+  hipLaunchKernelGGL(k_bond, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
+  if ((rc = mark(3))) return rc;
+  hipLaunchKernelGGL(k_contact, dim3(grid_of(Np, 256)), dim3(256), 0, st, a);
+  if ((rc = mark(4))) return rc;
   if (want_reaction)
     hipLaunchKernelGGL(k_reaction, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d, dt, c->inst_nodes);
   hipLaunchKernelGGL(k_update, dim3(grid_of(Np, 256)), dim3(256), 0, st, d, dt);
   hipLaunchKernelGGL(k_finalize, dim3(grid_of(std::max<uint64_t>(d.P, d.ninst), 256)), dim3(256), 0,
                      st, d, dt);
+  if ((rc = mark(5))) return rc;
   HIPCHK(c, hipGetLastError());
+  if (c->prof) {
+    HIPCHK(c, hipEventSynchronize(c->pev[PD_NPHASE]));
+    for (int k = 0; k < PD_NPHASE; k++) {
+      float ms = 0;
+      HIPCHK(c, hipEventElapsedTime(&ms, c->pev[k], c->pev[k + 1]));
+      c->acc.ms[k] += ms;
+      c->acc.launches[k] += 1;
+    }
+  }
   // host mirror of the wall planes (timeloop.h:784-797)
   for (auto& I : c->insts)
     for (int k = 0; k < 6; k++) I.wall[k] += I.speed[k] * dt;
@@ -744,13 +768,30 @@ int pd_get_counters(pd_ctx* c, pd_counters* out) {
     unsigned long long* dcount = c->d.counters + 1;
     HIPCHK(c, hipMemsetAsync(dcount, 0, sizeof(unsigned long long), c->stream));
     if (c->nbond_entries)
-      hipLaunchKernelGGL(k_count_alive, dim3(grid_of(c->nbond_entries, 256)), dim3(256), 0, c->stream,
+      hipLaunchKernelGGL(k_count_alive, dim3(std::min<uint64_t>(grid_of(c->nbond_entries, 256), 4096)),
+                         dim3(256), 0, c->stream,
                          c->d, c->nbond_entries, dcount);
     unsigned long long alive = 0;
     HIPCHK(c, hipMemcpyAsync(&alive, dcount, sizeof alive, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     out->alive_bonds = alive;
   }
+  return PD_OK;
+}
+
+int pd_set_profiling(pd_ctx* c, int on) {
+  if (!c) return PD_E_ARG;
+  if (on && !c->pev[0])
+    for (int k = 0; k <= PD_NPHASE; k++) HIPCHK(c, hipEventCreate(&c->pev[k]));
+  c->prof = on != 0;
+  c->acc = pd_profile{};
+  return PD_OK;
+}
+
+int pd_get_profile(pd_ctx* c, pd_profile* out) {
+  if (!c || !out) return PD_E_ARG;
+  *out = c->acc;
+  c->acc = pd_profile{};
   return PD_OK;
 }
 
@@ -768,6 +809,8 @@ void pd_destroy(pd_ctx* c) {
   for (void* p : c->allocs) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (int k = 0; k <= PD_NPHASE; k++)
+    if (c->pev[k]) hipEventDestroy(c->pev[k]);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }

@@ -78,6 +78,59 @@ struct Dev {
   uint32_t pad;
 };
 
+// ---------------------------------------------------------------------------
+// Correctly rounded FP64 sqrt and division, guarded fast path.
+//
+// sqrt_core / rcp_core + div_core are the instruction sequences the AMDGPU
+// backend emits for IEEE sqrt (v_rsq_f64 and two Newton-Raphson corrections)
+// and IEEE division (v_rcp_f64, two Newton-Raphson steps, quotient and one
+// remainder correction), without the exponent pre-scaling (v_ldexp,
+// v_div_scale, v_div_fmas) and the special-value fix-up (v_cmp_class,
+// v_div_fixup).  For operands inside the guarded ranges those wrappers are
+// the identity, so the results are bitwise those of `sqrt(x)` and `a / b`;
+// callers test sqrt_ok/div_ok and use the plain operators otherwise.  The
+// refined reciprocal depends only on the denominator, so the three divisions
+// e_k / |e| of one bond share it.
+__device__ __forceinline__ bool sqrt_ok(double x) { return (x >= 0x1p-767) & (x <= 0x1p+900); }
+__device__ __forceinline__ bool den_ok(double b) { return (b >= 0x1p-380) & (b <= 0x1p+450); }
+__device__ __forceinline__ bool num_ok(double a) {
+  return (__builtin_fabs(a) >= 0x1p-380) & (__builtin_fabs(a) <= 0x1p+450) | (a == 0.0);
+}
+
+__device__ __forceinline__ double sqrt_core(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = y * 0.5;
+  double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return g;
+}
+
+struct Rcp {
+  double nb;  // -b
+  double r;   // refined reciprocal of b
+};
+__device__ __forceinline__ Rcp rcp_core(double b) {
+  double r0 = __builtin_amdgcn_rcp(b);
+  double e0 = __builtin_fma(-b, r0, 1.0);
+  double r1 = __builtin_fma(r0, e0, r0);
+  double e1 = __builtin_fma(-b, r1, 1.0);
+  double r2 = __builtin_fma(r1, e1, r1);
+  return {-b, r2};
+}
+// a / b for b > 0; the sign of a zero quotient is a's, as IEEE division gives
+__device__ __forceinline__ double div_core(double a, Rcp R) {
+  double q = a * R.r;
+  double rem = __builtin_fma(R.nb, q, a);
+  double res = __builtin_fma(rem, R.r, q);
+  return __builtin_copysign(res, a);
+}
+
 // Eigen 3.3 fixed-size 1x3 norm/dot order.
 __device__ __forceinline__ double nrm3(double a, double b, double c) {
   return sqrt((a * a + b * b) + c * c);

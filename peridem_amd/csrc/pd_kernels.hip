@@ -4,8 +4,8 @@
 //   k_position   Verlet position half + per-particle box of CurrPos
 //   k_pairs      particle-pair box test on the lagged boxes (timeloop.h:636-653)
 //   cell list    k_cellkey -> k_count -> scan -> k_scatter (contact candidates)
-//   k_force      PD bond force + fused irreversible break + self/wall/ext/pair
-//                contact, summed per node in the reference's order, no atomics
+//   k_bond       PD bond force + fused irreversible break, per-node row sums
+//   k_contact    self/wall/ext/pair contact added in the reference's order
 //   k_update     Verlet velocity half, clamped nodes
 //   k_finalize   lagged boxes, wall motion
 // All floating point is FP64 with the reference's operation order; the build
@@ -224,10 +224,112 @@ __device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) 
 }
 
 // ---------------------------------------------------------------------------
-// The force kernel: one thread per node, 64 consecutive nodes per wavefront
-// share one bond slice so the k-th entries of the 64 rows are one coalesced
-// 256-byte load.
-__global__ __launch_bounds__(256) void k_force(StepArgs a) {
+// Bond-based PD force + critical-stretch break (get_peridynamic_force,
+// particle2.h:107-193; remove_bonds, particle2.h:196-237).  One thread per
+// node; the 64 nodes of a wavefront share one bond slice, so the k-th entries
+// of their rows are one coalesced 256-byte load.  Each thread sums its row in
+// row order, exactly like the reference, so no atomics and no reordering.
+// The stretch is tested against snot right after the term is summed; a broken
+// entry stops contributing from the next step on, as the reference's erase
+// after the force pass does.  Four entries are fetched per iteration so each
+// wavefront keeps several gathers in flight.
+constexpr int kBondUnroll = 4;
+
+struct BondTerm {
+  double s;       // stretch
+  double t0, t1, t2;
+};
+
+// one term of the PD sum in the reference's operation order (particle2.h:139-153)
+__device__ __forceinline__ BondTerm bond_term(const double4& xi, const double4& ui, const double4& xj,
+                                              const double4& uj, double cnot) {
+  double x0 = xj.x - xi.x, x1 = xj.y - xi.y, x2 = xj.z - xi.z;   // xi (gen_xi, particle2.h:253)
+  double e0 = (uj.x - ui.x) + x0;
+  double e1 = (uj.y - ui.y) + x1;
+  double e2 = (uj.z - ui.z) + x2;
+  double n2 = (e0 * e0 + e1 * e1) + e2 * e2;
+  double l2 = (x0 * x0 + x1 * x1) + x2 * x2;
+  BondTerm r;
+  bool fast = sqrt_ok(n2) & sqrt_ok(l2);
+  double nn, L;
+  if (fast) {
+    nn = sqrt_core(n2);
+    L = sqrt_core(l2);
+    double diff = nn - L;
+    fast = num_ok(diff) & num_ok(e0) & num_ok(e1) & num_ok(e2) & den_ok(nn) & den_ok(L);
+    if (fast) {
+      double str = div_core(diff, rcp_core(L));
+      double c = (cnot * str) * xj.w;
+      Rcp rn = rcp_core(nn);
+      r.s = str;
+      r.t0 = c * div_core(e0, rn);
+      r.t1 = c * div_core(e1, rn);
+      r.t2 = c * div_core(e2, rn);
+      return r;
+    }
+  }
+  nn = sqrt(n2);
+  L = sqrt(l2);
+  double str = (nn - L) / L;
+  double c = (cnot * str) * xj.w;
+  r.s = str;
+  r.t0 = c * (e0 / nn);
+  r.t1 = c * (e1 / nn);
+  r.t2 = c * (e2 / nn);
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_bond(Dev d) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // padded node range: S slices of 64; whole wavefronts exit together
+  if (g >= d.S * kSlice) return;
+  const DevPart& P = d.part[d.pid[g]];
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  if (P.movable) {
+    const bool can_break = d.inst[P.inst].enable_fracture && P.breakable;
+    const double cnot = P.cnot, snot = P.snot;
+    // one wavefront = one slice: offset and width are wave-uniform
+    const uint64_t s = g >> 6;
+    const uint64_t b0 = d.soff[s], b1 = d.soff[s + 1];
+    const uint32_t blo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b0);
+    const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b0 >> 32));
+    const uint64_t base = ((uint64_t)bhi << 32) | (uint64_t)blo;
+    const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((b1 - b0) >> 6));
+    uint32_t* bp = d.bond + base + (g & 63);
+    const double4 xi = d.X[g];
+    const double4 ui = d.U[g];
+    for (uint32_t k = 0; k < W; k += kBondUnroll) {
+      uint32_t e[kBondUnroll];
+#pragma unroll
+      for (int u = 0; u < kBondUnroll; u++) e[u] = (k + u < W) ? bp[(uint64_t)(k + u) * kSlice] : kPad;
+      double4 xj[kBondUnroll], uj[kBondUnroll];
+#pragma unroll
+      for (int u = 0; u < kBondUnroll; u++) {
+        const uint32_t j = (e[u] & kBroken) ? (uint32_t)g : e[u];
+        xj[u] = d.X[j];
+        uj[u] = d.U[j];
+      }
+#pragma unroll
+      for (int u = 0; u < kBondUnroll; u++) {
+        if (e[u] & kBroken) continue;
+        BondTerm t = bond_term(xi, ui, xj[u], uj[u], cnot);
+        f0 += t.t0;
+        f1 += t.t1;
+        f2 += t.t2;
+        if (can_break && t.s > snot) bp[(uint64_t)(k + u) * kSlice] = e[u] | kBroken;
+      }
+    }
+  }
+  d.F[3 * g] = f0;
+  d.F[3 * g + 1] = f1;
+  d.F[3 * g + 2] = f2;
+}
+
+// ---------------------------------------------------------------------------
+// Contact, wall and external force, added to the PD sum in the reference's
+// order (timeloop.h:544-651): self-contact (q ascending), wall planes,
+// external force, then pair contact in pair order.
+__global__ __launch_bounds__(256) void k_contact(StepArgs a) {
   const Dev& d = a.d;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
@@ -237,39 +339,7 @@ __global__ __launch_bounds__(256) void k_force(StepArgs a) {
   const DevInst& I = d.inst[P.inst];
   const double4 xi = d.X[g];
   const double4 ui = d.U[g];
-  double f[3] = {0.0, 0.0, 0.0};
-
-  // (a) bond-based PD force + critical-stretch break
-  //     get_peridynamic_force (particle2.h:107-193), remove_bonds
-  //     (particle2.h:196-237): the stretch is tested against snot right after
-  //     the term is summed; a broken entry stops contributing from the next
-  //     step on, exactly like the reference's erase after the force pass.
-  if (P.movable) {
-    const bool can_break = I.enable_fracture && P.breakable;
-    const uint64_t s = g >> 6;
-    const uint64_t base = d.soff[s];
-    const uint32_t W = (uint32_t)((d.soff[s + 1] - base) >> 6);
-    uint32_t* bp = d.bond + base + (g & 63);
-    const double cnot = P.cnot, snot = P.snot;
-    for (uint32_t k = 0; k < W; k++) {
-      uint32_t e = bp[(uint64_t)k * kSlice];
-      if (e & kBroken) continue;
-      const double4 xj = d.X[e];
-      const double4 uj = d.U[e];
-      double x0 = xj.x - xi.x, x1 = xj.y - xi.y, x2 = xj.z - xi.z;   // xi (gen_xi)
-      double e0 = (uj.x - ui.x) + x0;
-      double e1 = (uj.y - ui.y) + x1;
-      double e2 = (uj.z - ui.z) + x2;
-      double nn = nrm3(e0, e1, e2);
-      double L = nrm3(x0, x1, x2);                                    // xi_norm
-      double str = (nn - L) / L;
-      double c = (cnot * str) * xj.w;
-      f[0] += c * (e0 / nn);
-      f[1] += c * (e1 / nn);
-      f[2] += c * (e2 / nn);
-      if (can_break && str > snot) bp[(uint64_t)k * kSlice] = e | kBroken;
-    }
-  }
+  double f[3] = {d.F[3 * g], d.F[3 * g + 1], d.F[3 * g + 2]};
 
   const bool selfc = self_active(P, I);
   const bool pairc = (d.pactive[p] & 1u) != 0;
@@ -499,11 +569,18 @@ __global__ void k_clear_bflag(Dev d, uint64_t n) {
 // alive directed bonds (for counters and read-back)
 __global__ __launch_bounds__(256) void k_count_alive(Dev d, uint64_t nent, unsigned long long* out) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // padding entries carry the broken bit, so they are not counted
+  __shared__ uint32_t part[4];
   uint32_t alive = 0;
-  if (q < nent) alive = (d.bond[q] & kBroken) ? 0u : 1u;
-  // wavefront reduction
+  for (uint64_t i = q; i < nent; i += (uint64_t)gridDim.x * blockDim.x)
+    alive += (d.bond[i] & kBroken) ? 0u : 1u;
   for (int off = 32; off > 0; off >>= 1) alive += __shfl_down(alive, off, 64);
-  if ((threadIdx.x & 63) == 0 && alive) atomicAdd(out, (unsigned long long)alive);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = alive;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = (unsigned long long)part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(out, t);
+  }
 }
 
 }  // namespace pd
