@@ -3,7 +3,10 @@
 
 Workload: 125-sphere gravity settling -- spheres of radius 1e-3 m with about
 8k jittered-lattice nodes each (1.0M nodes, 92.6M directed bonds), delta = 3h,
-walls on six sides, gravity, FP64, dt = 1e-8 s.  One "step" is one full
+walls on six sides, gravity, FP64, dt = 1e-8 s.  Each sphere starts with a
+seeded rigid velocity (+-0.5 m/s per axis) plus a seeded per-node velocity
+(+-0.01 m/s): without it every node of a sphere shares one displacement and
+every bond has exactly zero stretch, which is not a representative load.  One "step" is one full
 explicit step of the library (Verlet position half, contact list, bond force
 + break + contact + wall + gravity, Verlet velocity half).
 
@@ -31,11 +34,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+WORKLOAD = dict(sphere_speed=0.5, node_speed=0.01)
+
+
 def cpu_baseline(nspheres, steps, threads):
     """The oracle on the first `nspheres` spheres of the workload."""
     from oracle import oracle
     from peridem_amd import gen
-    s = gen.gravity_settling(nspheres=nspheres)
+    s = gen.gravity_settling(nspheres=nspheres, **WORKLOAD)
     o = oracle.engine(s, ranks=threads)
     o.step(2)
     t0 = time.perf_counter()
@@ -69,7 +75,7 @@ def main():
 
     from peridem_amd import abi, gen
     t0 = time.perf_counter()
-    s = gen.gravity_settling(nspheres=args.spheres)
+    s = gen.gravity_settling(nspheres=args.spheres, **WORKLOAD)
     N, B, Bm = s.nodes, s.bonds, s.movable_bonds()
     log(f"[bench] workload: {args.spheres} spheres, {N} nodes, {B} directed bonds "
         f"(generated in {time.perf_counter() - t0:.1f} s)")
@@ -120,7 +126,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"gravity settling, {args.spheres} spheres R=1e-3 m, ~8k "
-                               f"jittered-lattice nodes each, delta=3h, 6 walls, dt=1e-8",
+                               f"jittered-lattice nodes each, delta=3h, 6 walls, gravity, seeded velocities "
+                               f"(0.5 m/s rigid + 0.01 m/s per node), dt=1e-8",
                    "nodes": N, "directed_bonds": B, "movable_bonds": Bm,
                    "alive_bonds": cnt["alive_bonds"], "parallelism": "single GPU",
                    "step_hbm_fraction": step_bytes / (step_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),

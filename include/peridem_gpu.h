@@ -177,6 +177,18 @@ int pd_get_profile(pd_ctx* ctx, pd_profile* out); /* resets the accumulators */
 const char* pd_last_error(pd_ctx* ctx);
 void pd_destroy(pd_ctx* ctx);
 
+/* The bond kernel evaluates sqrt and division with the backend's own
+ * instruction sequences minus their operand-rescaling wrappers, valid while
+ * coordinates, displacements and bond lengths stay in [2^-600, 2^600] (or
+ * are 0); outside that domain it switches, for good, to the IEEE operators.
+ * Results are bitwise the same either way.  pd_math_mode reports whether the
+ * context has switched (*safe = 1).  pd_selftest_math checks the fast
+ * sequences against IEEE `/` and `sqrt` on n random operands of the domain on
+ * the current device and returns the mismatch counts (expected 0).  No
+ * counterpart in the reference (a CPU code using the C library's sqrt). */
+int pd_math_mode(pd_ctx* ctx, int* safe);
+int pd_selftest_math(uint64_t n, uint64_t seed, uint64_t* mismatch_div, uint64_t* mismatch_sqrt);
+
 /* Library version and build target, for logs. */
 const char* pd_version(void);
 

@@ -127,6 +127,10 @@ def load_gpu_lib(path=GPU_LIB):
         lib.pd_set_profiling.restype = C.c_int
         lib.pd_get_profile.argtypes = [C.c_void_p, C.POINTER(Profile)]
         lib.pd_get_profile.restype = C.c_int
+        lib.pd_math_mode.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        lib.pd_math_mode.restype = C.c_int
+        lib.pd_selftest_math.argtypes = [_u64, _u64, C.POINTER(_u64), C.POINTER(_u64)]
+        lib.pd_selftest_math.restype = C.c_int
         _LIBS[path] = (lib, _bind(lib, "pd_"))
     return _LIBS[path]
 
@@ -220,6 +224,11 @@ class Engine:
         self._check(self.lib.pd_get_profile(self.ctx, C.byref(pr)), "get_profile")
         return pr.as_dict()
 
+    def math_mode_safe(self):
+        v = C.c_int()
+        self._check(self.lib.pd_math_mode(self.ctx, C.byref(v)), "math_mode")
+        return bool(v.value)
+
     def close(self):
         if self.ctx:
             self.f["destroy"](self.ctx)
@@ -230,6 +239,16 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+def selftest_math(n, seed=1):
+    """(division mismatches, sqrt mismatches) of the fast-path arithmetic."""
+    lib, _ = load_gpu_lib()
+    md, ms = _u64(), _u64()
+    rc = lib.pd_selftest_math(n, seed, C.byref(md), C.byref(ms))
+    if rc != PD_OK:
+        raise PDError(f"pd_selftest_math: {ERRORS.get(rc, rc)}")
+    return md.value, ms.value
 
 
 def gpu_engine(system):

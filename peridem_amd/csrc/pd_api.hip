@@ -67,6 +67,7 @@ struct pd_ctx {
   std::vector<uint64_t> movable_bonds;  // per particle directed bonds
   uint64_t steps_since_set = 0;
   bool prof = false;
+  bool safe_math_static = false;
   hipEvent_t pev[PD_NPHASE + 1] = {};
   pd_profile acc{};
 };
@@ -376,6 +377,7 @@ int pd_commit(pd_ctx* c) {
   ALLOC(pairs, pairs.size());
   ALLOC(counters, 8);
   ALLOC(err, 1);
+  ALLOC(flags, 1);
   ALLOC(reaction, 18 * di.size());
   ALLOC(wmask, Np);
   ALLOC(bflag, Np / 256 + 1);
@@ -409,6 +411,26 @@ int pd_commit(pd_ctx* c) {
   HIPCHK(c, hipMemsetAsync(d.pactive, 0, P * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.counters, 0, 8 * sizeof(unsigned long long), st));
   HIPCHK(c, hipMemsetAsync(d.err, 0, sizeof(uint32_t), st));
+  {
+    // fast-path domain of the bond arithmetic: coordinates and reference
+    // bond lengths (displacements are checked by the position kernel)
+    uint32_t fl = 0;
+    for (uint64_t g = 0; g < N && !fl; g++)
+      if (!(in_fast_range(X[g].x) && in_fast_range(X[g].y) && in_fast_range(X[g].z))) fl = kFlagSafeMath;
+    for (uint64_t g = 0; g < N && !fl; g++)
+      for (uint64_t k = rstart[g]; k < rstart[g + 1]; k++) {
+        const double4& a = X[g];
+        const double4& b = X[rows[k]];
+        double x0 = b.x - a.x, x1 = b.y - a.y, x2 = b.z - a.z;
+        double l2 = (x0 * x0 + x1 * x1) + x2 * x2;
+        if (!(l2 >= kFastLo && l2 <= kFastHi)) {
+          fl = kFlagSafeMath;
+          break;
+        }
+      }
+    c->safe_math_static = fl != 0;
+    HIPCHK(c, hipMemcpyAsync(d.flags, &fl, sizeof fl, hipMemcpyHostToDevice, st));
+  }
   HIPCHK(c, hipMemsetAsync(d.reaction, 0, 18 * di.size() * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.wmask, 0, Np, st));
   HIPCHK(c, hipMemsetAsync(d.bflag, 0, Np / 256 + 1, st));
@@ -792,6 +814,36 @@ int pd_get_profile(pd_ctx* c, pd_profile* out) {
   if (!c || !out) return PD_E_ARG;
   *out = c->acc;
   c->acc = pd_profile{};
+  return PD_OK;
+}
+
+int pd_selftest_math(uint64_t n, uint64_t seed, uint64_t* mismatch_div, uint64_t* mismatch_sqrt) {
+  if (!mismatch_div || !mismatch_sqrt) return PD_E_ARG;
+  unsigned long long* dm = nullptr;
+  if (hipMalloc((void**)&dm, 2 * sizeof(unsigned long long)) != hipSuccess) return PD_E_HIP;
+  int rc = PD_OK;
+  unsigned long long hm[2] = {0, 0};
+  if (hipMemset(dm, 0, 2 * sizeof(unsigned long long)) != hipSuccess) rc = PD_E_HIP;
+  if (rc == PD_OK) {
+    hipLaunchKernelGGL(k_selftest_math, dim3(2048), dim3(256), 0, 0, n, seed, dm);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(hm, dm, sizeof hm, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = PD_E_HIP;
+  }
+  hipFree(dm);
+  *mismatch_div = hm[0];
+  *mismatch_sqrt = hm[1];
+  return rc;
+}
+
+int pd_math_mode(pd_ctx* c, int* safe) {
+  if (!c || !safe) return PD_E_ARG;
+  uint32_t fl = 0;
+  if (c->committed) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(&fl, c->d.flags, sizeof fl, hipMemcpyDeviceToHost));
+  }
+  *safe = (fl & kFlagSafeMath) ? 1 : 0;
   return PD_OK;
 }
 

@@ -17,6 +17,8 @@ constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 
 // error bits raised by kernels, read back by the host after pd_step
 constexpr uint32_t kErrCandOverflow = 1u;
+// state bits of Dev::flags
+constexpr uint32_t kFlagSafeMath = 1u;  // an operand left the fast-path range: use IEEE ops
 
 // Per-particle parameters (ParticleN scalars, particle2.h:51-59).
 struct DevPart {
@@ -63,6 +65,7 @@ struct Dev {
   uint32_t* pairs;   // [npairs*2] particle pairs (i<j) of the same instance
   unsigned long long* counters;   // [0] pair computations
   uint32_t* err;
+  uint32_t* flags;   // [1] kFlagSafeMath (sticky)
   // wall reaction (per instance, 6 planes x 3) and helpers
   double* reaction;
   uint8_t* wmask;    // [N] planes in contact this step (reaction steps only)
@@ -91,10 +94,17 @@ struct Dev {
 // callers test sqrt_ok/div_ok and use the plain operators otherwise.  The
 // refined reciprocal depends only on the denominator, so the three divisions
 // e_k / |e| of one bond share it.
-__device__ __forceinline__ bool sqrt_ok(double x) { return (x >= 0x1p-767) & (x <= 0x1p+900); }
-__device__ __forceinline__ bool den_ok(double b) { return (b >= 0x1p-380) & (b <= 0x1p+450); }
-__device__ __forceinline__ bool num_ok(double a) {
-  return (__builtin_fabs(a) >= 0x1p-380) & (__builtin_fabs(a) <= 0x1p+450) | (a == 0.0);
+// Fast-path domain.  Node coordinates, reference bond lengths (checked at
+// commit) and displacements (checked every step by the position kernel) are 0
+// or of magnitude in [2^-600, 2^600]; per bond only |e|^2 is range-checked.
+// Then every operand of the bond arithmetic is a multiple of 2^-652 and
+// bounded by 2^302, far from the exponents where the backend's sqrt and
+// division sequences rescale or fix up their operands (below 2^-767 for
+// sqrt, numerators below 2^-969 or exponent gaps of 768 for division).
+constexpr double kFastLo = 0x1p-600, kFastHi = 0x1p+600;
+__host__ __device__ inline bool in_fast_range(double v) {
+  double a = v < 0 ? -v : v;
+  return (v == 0.0) || ((a >= kFastLo) && (a <= kFastHi));
 }
 
 __device__ __forceinline__ double sqrt_core(double x) {
@@ -123,12 +133,13 @@ __device__ __forceinline__ Rcp rcp_core(double b) {
   double r2 = __builtin_fma(r1, e1, r1);
   return {-b, r2};
 }
-// a / b for b > 0; the sign of a zero quotient is a's, as IEEE division gives
+// a / b for b > 0.  Equal to IEEE division except that a zero numerator may
+// give +0 where IEEE gives -0; callers only use it where that cannot matter
+// (a zero term added to a sum that is never -0).
 __device__ __forceinline__ double div_core(double a, Rcp R) {
   double q = a * R.r;
   double rem = __builtin_fma(R.nb, q, a);
-  double res = __builtin_fma(rem, R.r, q);
-  return __builtin_copysign(res, a);
+  return __builtin_fma(rem, R.r, q);
 }
 
 // Eigen 3.3 fixed-size 1x3 norm/dot order.

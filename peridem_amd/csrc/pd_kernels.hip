@@ -44,6 +44,7 @@ __global__ __launch_bounds__(256) void k_position(Dev d, double dt) {
     u.y = u.y + ((dt * v1) + (h * a1));
     u.z = u.z + ((dt * v2) + (h * a2));
     d.U[g] = u;
+    if (!(in_fast_range(u.x) & in_fast_range(u.y) & in_fast_range(u.z))) atomicOr(d.flags, kFlagSafeMath);
     cp[0] = x.x + u.x;
     cp[1] = x.y + u.y;
     cp[2] = x.z + u.z;
@@ -240,9 +241,15 @@ struct BondTerm {
   double t0, t1, t2;
 };
 
-// one term of the PD sum in the reference's operation order (particle2.h:139-153)
+// One term of the PD sum in the reference's operation order
+// (particle2.h:139-153):
+//   e = (u_j - u_i) + xi,  s = (|e| - |xi|) / |xi|,  t = (cnot*s*V_j) * (e/|e|)
+// In the fast domain the sqrt/division sequences are the backend's own
+// without their range wrappers (bitwise equal); a bond with |e| == |xi|
+// exactly has s = +0 and adds +-0 to a sum that is never -0, so it skips
+// the unit vector.  Outside the domain the plain IEEE operators are used.
 __device__ __forceinline__ BondTerm bond_term(const double4& xi, const double4& ui, const double4& xj,
-                                              const double4& uj, double cnot) {
+                                              const double4& uj, double cnot, bool fast) {
   double x0 = xj.x - xi.x, x1 = xj.y - xi.y, x2 = xj.z - xi.z;   // xi (gen_xi, particle2.h:253)
   double e0 = (uj.x - ui.x) + x0;
   double e1 = (uj.y - ui.y) + x1;
@@ -250,26 +257,26 @@ __device__ __forceinline__ BondTerm bond_term(const double4& xi, const double4& 
   double n2 = (e0 * e0 + e1 * e1) + e2 * e2;
   double l2 = (x0 * x0 + x1 * x1) + x2 * x2;
   BondTerm r;
-  bool fast = sqrt_ok(n2) & sqrt_ok(l2);
-  double nn, L;
-  if (fast) {
-    nn = sqrt_core(n2);
-    L = sqrt_core(l2);
+  if (fast & (n2 >= kFastLo) & (n2 <= kFastHi)) {
+    double nn = sqrt_core(n2);
+    double L = sqrt_core(l2);
     double diff = nn - L;
-    fast = num_ok(diff) & num_ok(e0) & num_ok(e1) & num_ok(e2) & den_ok(nn) & den_ok(L);
-    if (fast) {
-      double str = div_core(diff, rcp_core(L));
-      double c = (cnot * str) * xj.w;
-      Rcp rn = rcp_core(nn);
-      r.s = str;
-      r.t0 = c * div_core(e0, rn);
-      r.t1 = c * div_core(e1, rn);
-      r.t2 = c * div_core(e2, rn);
+    if (diff == 0.0) {
+      r.s = 0.0;
+      r.t0 = r.t1 = r.t2 = 0.0;
       return r;
     }
+    double str = div_core(diff, rcp_core(L));
+    double c = (cnot * str) * xj.w;
+    Rcp rn = rcp_core(nn);
+    r.s = str;
+    r.t0 = c * div_core(e0, rn);
+    r.t1 = c * div_core(e1, rn);
+    r.t2 = c * div_core(e2, rn);
+    return r;
   }
-  nn = sqrt(n2);
-  L = sqrt(l2);
+  double nn = sqrt(n2);
+  double L = sqrt(l2);
   double str = (nn - L) / L;
   double c = (cnot * str) * xj.w;
   r.s = str;
@@ -288,6 +295,7 @@ __global__ __launch_bounds__(256) void k_bond(Dev d) {
   if (P.movable) {
     const bool can_break = d.inst[P.inst].enable_fracture && P.breakable;
     const double cnot = P.cnot, snot = P.snot;
+    const bool fast = (*d.flags & kFlagSafeMath) == 0;
     // one wavefront = one slice: offset and width are wave-uniform
     const uint64_t s = g >> 6;
     const uint64_t b0 = d.soff[s], b1 = d.soff[s + 1];
@@ -312,7 +320,7 @@ __global__ __launch_bounds__(256) void k_bond(Dev d) {
 #pragma unroll
       for (int u = 0; u < kBondUnroll; u++) {
         if (e[u] & kBroken) continue;
-        BondTerm t = bond_term(xi, ui, xj[u], uj[u], cnot);
+        BondTerm t = bond_term(xi, ui, xj[u], uj[u], cnot, fast);
         f0 += t.t0;
         f1 += t.t1;
         f2 += t.t2;
@@ -581,6 +589,47 @@ __global__ __launch_bounds__(256) void k_count_alive(Dev d, uint64_t nent, unsig
     unsigned long long t = (unsigned long long)part[0] + part[1] + part[2] + part[3];
     if (t) atomicAdd(out, t);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Self-test of the fast-path arithmetic: random operands of the fast domain,
+// div_core/sqrt_core against the IEEE operators, bit for bit.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double make_double(uint64_t mant, int e, int neg) {
+  uint64_t b = ((uint64_t)(e + 1023) << 52) | (mant & 0xFFFFFFFFFFFFFull) | ((uint64_t)neg << 63);
+  return __longlong_as_double((long long)b);
+}
+__global__ __launch_bounds__(256) void k_selftest_math(uint64_t n, uint64_t seed,
+                                                       unsigned long long* mism) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bad_div = 0, bad_sqrt = 0;
+  for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t r0 = mix64(seed ^ (i * 4 + 0)), r1 = mix64(seed ^ (i * 4 + 1));
+    uint64_t r2 = mix64(seed ^ (i * 4 + 2)), r3 = mix64(seed ^ (i * 4 + 3));
+    // denominator in [2^-300, 2^301): |e| or |xi|
+    int eb = (int)(r0 % 601) - 300;
+    double b = make_double(r1, eb, 0);
+    // numerator: a unit-vector component (exponent eb-652 .. eb) or a length
+    // difference (exponent -352 .. 301), 1/32 of them zero of either sign
+    int ea = ((r2 >> 40) & 1) ? eb - (int)(r0 >> 32) % 653 : (int)((r0 >> 20) % 654) - 352;
+    double a = make_double(r3, ea, (int)(r2 >> 63));
+    if ((r2 & 31) == 0) a = ((r2 >> 8) & 1) ? -0.0 : 0.0;
+    double q = div_core(a, rcp_core(b));
+    double qi = a / b;
+    bool eq = (a == 0.0) ? (q == 0.0 && qi == 0.0)
+                         : (__double_as_longlong(q) == __double_as_longlong(qi));
+    bad_div += eq ? 0u : 1u;
+    // sqrt argument in [2^-600, 2^601)
+    double x = make_double(r2 >> 3, (int)(r3 % 1201) - 600, 0);
+    bad_sqrt += (__double_as_longlong(sqrt_core(x)) == __double_as_longlong(sqrt(x))) ? 0u : 1u;
+  }
+  if (bad_div) atomicAdd(&mism[0], (unsigned long long)bad_div);
+  if (bad_sqrt) atomicAdd(&mism[1], (unsigned long long)bad_sqrt);
 }
 
 }  // namespace pd

@@ -272,18 +272,30 @@ def _jittered_grid_centres(n, R, box, gap, rng):
 
 
 def gravity_settling(nspheres=125, R=1e-3, nodes_per_sphere=8000, box=1.2e-2, seed=2,
-                     E=72e9, nu=0.25, rho=2440.0, Gc=135.0, g=9.81, **sys_kw):
-    """Spheres at seeded random non-overlapping centres in a walled box under gravity."""
+                     E=72e9, nu=0.25, rho=2440.0, Gc=135.0, g=9.81, sphere_speed=0.0,
+                     node_speed=0.0, **sys_kw):
+    """Spheres at seeded random non-overlapping centres in a walled box under gravity.
+
+    With sphere_speed = node_speed = 0 every node of a sphere keeps the same
+    displacement, so every bond keeps exactly zero stretch.  sphere_speed adds
+    a seeded rigid velocity per sphere (uniform in +-sphere_speed per axis)
+    and node_speed a seeded per-node velocity (uniform in +-node_speed), which
+    makes every bond carry stretch.
+    """
     rng = np.random.default_rng(seed)
     h = R * (4.0 * math.pi / 3.0 / nodes_per_sphere) ** (1.0 / 3.0)
     L = lattice_sphere(R / h)
     Rc = 0.9 * h
     cs = _jittered_grid_centres(nspheres, R, box, 2 * h, rng)
     parts = []
+    vrng = np.random.default_rng(seed + 7919)
     for c in cs:
         p = make_particle(L, h, c, rng, E, nu, rho, Gc)
         p.ext = np.zeros((p.n, 3))
         p.ext[:, 2] = -g * rho
+        if sphere_speed or node_speed:
+            p.vel = (vrng.uniform(-sphere_speed, sphere_speed, size=(1, 3)) +
+                     vrng.uniform(-node_speed, node_speed, size=(p.n, 3)))
         parts.append(p)
     kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(E, nu, Rc),
               friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=0,
