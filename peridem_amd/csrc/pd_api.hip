@@ -499,7 +499,8 @@ static int launch_step(pd_ctx* c, bool want_reaction) {
     hipLaunchKernelGGL(k_clear_bflag, dim3(grid_of(Np / 256 + 1, 256)), dim3(256), 0, st, d,
                        (uint64_t)(Np / 256 + 1));
   if ((rc = mark(2))) return rc;
-  hipLaunchKernelGGL(k_bond, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
+  // grid padded to a multiple of the XCD count (k_bond remaps blocks)
+  hipLaunchKernelGGL(k_bond, dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
   if ((rc = mark(3))) return rc;
   hipLaunchKernelGGL(k_contact, dim3(grid_of(Np, 256)), dim3(256), 0, st, a);
   if ((rc = mark(4))) return rc;

@@ -226,68 +226,34 @@ __device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) 
 
 // ---------------------------------------------------------------------------
 // Bond-based PD force + critical-stretch break (get_peridynamic_force,
-// particle2.h:107-193; remove_bonds, particle2.h:196-237).  One thread per
-// node; the 64 nodes of a wavefront share one bond slice, so the k-th entries
-// of their rows are one coalesced 256-byte load.  Each thread sums its row in
-// row order, exactly like the reference, so no atomics and no reordering.
-// The stretch is tested against snot right after the term is summed; a broken
-// entry stops contributing from the next step on, as the reference's erase
-// after the force pass does.  Four entries are fetched per iteration so each
-// wavefront keeps several gathers in flight.
-constexpr int kBondUnroll = 4;
-
-struct BondTerm {
-  double s;       // stretch
-  double t0, t1, t2;
-};
-
-// One term of the PD sum in the reference's operation order
-// (particle2.h:139-153):
+// particle2.h:107-193; remove_bonds, particle2.h:196-237).
+//
+// One thread per node sums its row in row order, exactly like the reference:
+// no atomics, no reordering.  The 64 nodes of a wavefront share one bond
+// slice (entry k of lane l at soff + k*64 + l), so the k-th entries of their
+// rows are one coalesced 256-byte load.  The stretch is tested against snot
+// right after the term is summed; a broken entry stops contributing from the
+// next step on, as the reference's erase after the force pass does.
+//
+// Per term (particle2.h:139-153):
 //   e = (u_j - u_i) + xi,  s = (|e| - |xi|) / |xi|,  t = (cnot*s*V_j) * (e/|e|)
-// In the fast domain the sqrt/division sequences are the backend's own
-// without their range wrappers (bitwise equal); a bond with |e| == |xi|
-// exactly has s = +0 and adds +-0 to a sum that is never -0, so it skips
-// the unit vector.  Outside the domain the plain IEEE operators are used.
-__device__ __forceinline__ BondTerm bond_term(const double4& xi, const double4& ui, const double4& xj,
-                                              const double4& uj, double cnot, bool fast) {
-  double x0 = xj.x - xi.x, x1 = xj.y - xi.y, x2 = xj.z - xi.z;   // xi (gen_xi, particle2.h:253)
-  double e0 = (uj.x - ui.x) + x0;
-  double e1 = (uj.y - ui.y) + x1;
-  double e2 = (uj.z - ui.z) + x2;
-  double n2 = (e0 * e0 + e1 * e1) + e2 * e2;
-  double l2 = (x0 * x0 + x1 * x1) + x2 * x2;
-  BondTerm r;
-  if (fast & (n2 >= kFastLo) & (n2 <= kFastHi)) {
-    double nn = sqrt_core(n2);
-    double L = sqrt_core(l2);
-    double diff = nn - L;
-    if (diff == 0.0) {
-      r.s = 0.0;
-      r.t0 = r.t1 = r.t2 = 0.0;
-      return r;
-    }
-    double str = div_core(diff, rcp_core(L));
-    double c = (cnot * str) * xj.w;
-    Rcp rn = rcp_core(nn);
-    r.s = str;
-    r.t0 = c * div_core(e0, rn);
-    r.t1 = c * div_core(e1, rn);
-    r.t2 = c * div_core(e2, rn);
-    return r;
-  }
-  double nn = sqrt(n2);
-  double L = sqrt(l2);
-  double str = (nn - L) / L;
-  double c = (cnot * str) * xj.w;
-  r.s = str;
-  r.t0 = c * (e0 / nn);
-  r.t1 = c * (e1 / nn);
-  r.t2 = c * (e2 / nn);
-  return r;
-}
+// Eight entries are processed per iteration as straight-line code, so their
+// dependency chains interleave.  In the fast domain sqrt and division are the
+// backend's own sequences without their range wrappers (bitwise equal);
+// whenever a lane leaves it, the whole batch uses the IEEE operators.  A dead
+// (broken or padding) entry reads the node itself, gets unit lengths and a
+// zero coefficient, and adds +-0 to a sum that is never -0: no effect.
+//
+// Workgroups are mapped so that each XCD (blocks are dealt round-robin over
+// the 8 XCDs) runs one contiguous range of nodes: neighbouring tiles, which
+// gather the same neighbour records, share that XCD's L2.
+constexpr int kBondUnroll = 8;
+constexpr int kXcds = 8;
 
 __global__ __launch_bounds__(256) void k_bond(Dev d) {
-  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nbp = gridDim.x;   // a multiple of kXcds
+  const uint64_t blk = (blockIdx.x % kXcds) * (nbp / kXcds) + blockIdx.x / kXcds;
+  const uint64_t g = blk * blockDim.x + threadIdx.x;
   // padded node range: S slices of 64; whole wavefronts exit together
   if (g >= d.S * kSlice) return;
   const DevPart& P = d.part[d.pid[g]];
@@ -306,25 +272,64 @@ __global__ __launch_bounds__(256) void k_bond(Dev d) {
     uint32_t* bp = d.bond + base + (g & 63);
     const double4 xi = d.X[g];
     const double4 ui = d.U[g];
-    for (uint32_t k = 0; k < W; k += kBondUnroll) {
-      uint32_t e[kBondUnroll];
+    constexpr int U = kBondUnroll;
+    for (uint32_t k = 0; k < W; k += U) {
+      uint32_t e[U];
+      bool alive[U];
 #pragma unroll
-      for (int u = 0; u < kBondUnroll; u++) e[u] = (k + u < W) ? bp[(uint64_t)(k + u) * kSlice] : kPad;
-      double4 xj[kBondUnroll], uj[kBondUnroll];
+      for (int u = 0; u < U; u++) {
+        e[u] = (k + u < W) ? bp[(uint64_t)(k + u) * kSlice] : kPad;
+        alive[u] = !(e[u] & kBroken);
+      }
+      double4 xj[U], uj[U];
 #pragma unroll
-      for (int u = 0; u < kBondUnroll; u++) {
-        const uint32_t j = (e[u] & kBroken) ? (uint32_t)g : e[u];
+      for (int u = 0; u < U; u++) {
+        const uint32_t j = alive[u] ? e[u] : (uint32_t)g;
         xj[u] = d.X[j];
         uj[u] = d.U[j];
       }
+      double ev[U][3], n2[U], l2[U];
+      bool ok = fast;
 #pragma unroll
-      for (int u = 0; u < kBondUnroll; u++) {
-        if (e[u] & kBroken) continue;
-        BondTerm t = bond_term(xi, ui, xj[u], uj[u], cnot, fast);
-        f0 += t.t0;
-        f1 += t.t1;
-        f2 += t.t2;
-        if (can_break && t.s > snot) bp[(uint64_t)(k + u) * kSlice] = e[u] | kBroken;
+      for (int u = 0; u < U; u++) {
+        double x0 = xj[u].x - xi.x, x1 = xj[u].y - xi.y, x2 = xj[u].z - xi.z;  // xi (gen_xi)
+        ev[u][0] = (uj[u].x - ui.x) + x0;
+        ev[u][1] = (uj[u].y - ui.y) + x1;
+        ev[u][2] = (uj[u].z - ui.z) + x2;
+        n2[u] = alive[u] ? (ev[u][0] * ev[u][0] + ev[u][1] * ev[u][1]) + ev[u][2] * ev[u][2] : 1.0;
+        l2[u] = alive[u] ? (x0 * x0 + x1 * x1) + x2 * x2 : 1.0;
+        ok &= (n2[u] >= kFastLo) & (n2[u] <= kFastHi);
+      }
+      double str[U], t[U][3];
+      if (__all(ok)) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          double nn = sqrt_core(n2[u]);
+          double L = sqrt_core(l2[u]);
+          str[u] = div_core(nn - L, rcp_core(L));
+          double c = alive[u] ? (cnot * str[u]) * xj[u].w : 0.0;
+          Rcp rn = rcp_core(nn);
+#pragma unroll
+          for (int q = 0; q < 3; q++) t[u][q] = c * div_core(ev[u][q], rn);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          double nn = sqrt(n2[u]);
+          double L = sqrt(l2[u]);
+          str[u] = (nn - L) / L;
+          double c = alive[u] ? (cnot * str[u]) * xj[u].w : 0.0;
+#pragma unroll
+          for (int q = 0; q < 3; q++) t[u][q] = c * (ev[u][q] / nn);
+        }
+      }
+      // summed in row order
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        f0 += t[u][0];
+        f1 += t[u][1];
+        f2 += t[u][2];
+        if (alive[u] & can_break & (str[u] > snot)) bp[(uint64_t)(k + u) * kSlice] = e[u] | kBroken;
       }
     }
   }
