@@ -37,6 +37,20 @@ def log(msg):
 WORKLOAD = dict(sphere_speed=0.5, node_speed=0.01)
 
 
+def traffic_bytes(nodes, bonds):
+    """HBM bytes per k_bond launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, when it was taken on this same workload."""
+    path = os.path.join(ROOT, "profiles", "k_bond_gravity_settling_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("nodes") != nodes or t.get("movable_bonds") != bonds:
+        return None
+    return t.get("hbm_bytes_per_launch")
+
+
 def cpu_baseline(nspheres, steps, threads):
     """The oracle on the first `nspheres` spheres of the workload."""
     from oracle import oracle
@@ -66,6 +80,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device (a plain copy "
+                         "kernel of known size that calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -73,6 +90,13 @@ def main():
                                    "halo exchange)", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    probe = None
+    if args.copy_probe_mib:
+        # torch's HIP state is set up before the library's own
+        import torch
+        probe = torch.ones(args.copy_probe_mib * 1024 * 1024 // 8, dtype=torch.float64,
+                           device="cuda")
+        torch.cuda.synchronize()
     from peridem_amd import abi, gen
     t0 = time.perf_counter()
     s = gen.gravity_settling(nspheres=args.spheres, **WORKLOAD)
@@ -106,6 +130,13 @@ def main():
     cnt = e.counters()
     e.close()
 
+    if probe is not None:
+        import torch
+        b = torch.empty_like(probe)
+        b.copy_(probe)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {args.copy_probe_mib} MiB device-to-device")
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: {args.cpu_spheres} spheres x {args.cpu_steps} steps, "
@@ -133,7 +164,7 @@ def main():
                    "step_hbm_fraction": step_bytes / (step_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
                    "phase_ms": {k: v["ms"] / max(v["launches"], 1) for k, v in prof.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(N, Bm),
                      "kernel": "k_bond", "bytes_per_launch": force_bytes,
                      "avg_launch_ms": force_ms},
         "cpu_baseline": cpu,

@@ -1,0 +1,88 @@
+"""Summaries of rocprofv3 CSV output for the profiles/ folder.
+
+  python -m peridem_amd.profsum trace <dir>            kernel-trace --stats table
+  python -m peridem_amd.profsum counters <dir> [kernel] per-dispatch PMC averages
+  python -m peridem_amd.profsum traffic <fetch_dir> <write_dir> <copy_mib> <nodes> <bonds>
+        HBM bytes per launch of every kernel, FETCH_SIZE/WRITE_SIZE scaled by
+        what the counters read for the plain device copy of copy_mib MiB
+"""
+import collections
+import csv
+import glob
+import json
+import sys
+
+
+def _rows(d, name):
+    path = glob.glob(f"{d}/**/*{name}", recursive=True)
+    if not path:
+        raise SystemExit(f"no {name} under {d}")
+    with open(path[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def _short(k):
+    return k.split("(")[0]
+
+
+def trace(d):
+    rows = _rows(d, "kernel_stats.csv")
+    print("| kernel | calls | avg us | min us | max us | % of GPU time |")
+    print("|---|---|---|---|---|---|")
+    for r in rows:
+        print(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+              f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+
+
+def _agg(d):
+    out = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in _rows(d, "counter_collection.csv"):
+        out[_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return out
+
+
+def counters(d, kernel=None):
+    a = _agg(d)
+    for k, cs in a.items():
+        if kernel and kernel not in k:
+            continue
+        print(f"`{k}` ({len(next(iter(cs.values())))} dispatches, mean per dispatch)")
+        for c, v in sorted(cs.items()):
+            print(f"  {c:32s} {sum(v) / len(v):18.1f}")
+
+
+def traffic(fd, wd, copy_mib, nodes, bonds):
+    f, w = _agg(fd), _agg(wd)
+    copy_bytes = copy_mib * 1024 * 1024
+    # the plain copy: the largest copyBuffer dispatch
+    cf = max(f["__amd_rocclr_copyBuffer"]["FETCH_SIZE"])
+    cw = max(w["__amd_rocclr_copyBuffer"]["WRITE_SIZE"])
+    fs, ws = copy_bytes / (cf * 1024), copy_bytes / (cw * 1024)
+    print(f"copy probe {copy_mib} MiB: FETCH_SIZE {cf:.0f} KiB -> scale {fs:.4f}; "
+          f"WRITE_SIZE {cw:.0f} KiB -> scale {ws:.4f}")
+    print("| kernel | fetched MB | written MB | HBM MB per launch |")
+    print("|---|---|---|---|")
+    res = {}
+    for k in f:
+        if not k.startswith("pd::"):
+            continue
+        fb = sum(f[k]["FETCH_SIZE"]) / len(f[k]["FETCH_SIZE"]) * 1024 * fs
+        wb = sum(w[k]["WRITE_SIZE"]) / len(w[k]["WRITE_SIZE"]) * 1024 * ws
+        res[k] = fb + wb
+        print(f"| `{k}` | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} |")
+    return {"nodes": nodes, "movable_bonds": bonds, "fetch_scale": fs, "write_scale": ws,
+            "hbm_bytes_per_launch": res.get("pd::k_bond"),
+            "per_kernel_bytes": res}
+
+
+if __name__ == "__main__":
+    cmd = sys.argv[1]
+    if cmd == "trace":
+        trace(sys.argv[2])
+    elif cmd == "counters":
+        counters(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else None)
+    elif cmd == "traffic":
+        t = traffic(sys.argv[2], sys.argv[3], int(sys.argv[4]), int(sys.argv[5]), int(sys.argv[6]))
+        if len(sys.argv) > 7:
+            with open(sys.argv[7], "w") as fh:
+                json.dump(t, fh, indent=1)
