@@ -1,0 +1,122 @@
+"""The C++ host driver: config semantics, setup containers, and (GPU) a full
+run of the drop-in driver checked bitwise against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import assert_bitwise, small_impact
+from peridem_amd import setupio
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "peridem_amd", "driver", "simulate3d")
+
+
+def run_driver(*args, check=True):
+    assert os.path.exists(DRIVER), "driver not built (__graft_entry__.build())"
+    r = subprocess.run([DRIVER, *args], capture_output=True, text=True, timeout=300)
+    if check and r.returncode != 0:
+        raise AssertionError(f"driver failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    return r
+
+
+def test_config_parser_follows_reference_rules(tmp_path):
+    cfg = tmp_path / "main.conf"
+    cfg.write_text("# comment\n timesteps = 1234\nmodulo=7\n dt = 2.5e-8 \n"
+                   "damping_ratio = 0.25\nspeed_wall_z_max = -5\nwall_z_max=0.01\n"
+                   "not_a_key = 3\n#enable_fracture = 1\nself_contact = 0\n")
+    r = run_driver("--print-config", "-c", str(cfg))
+    vals = dict(l.split() for l in r.stdout.splitlines() if len(l.split()) == 2 and not l.startswith("Reading"))
+    assert vals["timesteps"] == "1234" and vals["modulo"] == "7"
+    assert float(vals["dt"]) == 2.5e-8 and float(vals["damping_ratio"]) == 0.25
+    assert float(vals["speed_wall_z_max"]) == -5 and float(vals["wall_z_max"]) == 0.01
+    assert vals["enable_fracture"] == "0"          # commented out: default kept
+    assert vals["self_contact"] == "0" and vals["allow_damping"] == "1"
+    assert "Wrong config name: not_a_key" in r.stderr
+
+
+def test_config_missing_value_is_an_error(tmp_path):
+    cfg = tmp_path / "bad.conf"
+    cfg.write_text("timesteps =\n")
+    r = run_driver("--print-config", "-c", str(cfg), check=False)
+    assert r.returncode != 0 and "Missing value for option: timesteps" in r.stdout
+
+
+def test_setup_container_roundtrip(tmp_path):
+    s = small_impact(seed=3)
+    a = tmp_path / "setup.pdbin"
+    setupio.write_setup(s, a)
+    ref = setupio.read_pdbin(a)
+    if "hdf5 yes" in run_driver("--features").stdout:
+        h = tmp_path / "setup.h5"
+        run_driver("--convert", str(a), str(h))
+        b = tmp_path / "back.pdbin"
+        run_driver("--convert", str(h), str(b))
+        got = setupio.read_pdbin(b)
+    else:
+        got = setupio.read_pdbin(a)
+    assert set(got) == set(ref)
+    for k in ref:
+        assert got[k].dtype == ref[k].dtype and np.array_equal(got[k], ref[k]), k
+
+
+def _oracle_run(oracle_lib, s, T, modulo, zero_speed_at):
+    o = oracle_lib.engine(s)
+    snaps = {}
+    for t in range(1, T + 1):
+        if t == zero_speed_at:
+            o.set_wall_speed([0] * 6)
+        o.step(1)
+        if t % modulo == 0:
+            snaps[t] = ([o.state(p) for p in range(len(s.particles))],
+                        [o.connectivity(p) for p in range(len(s.particles))], o.wall())
+    o.close()
+    return snaps
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("container", ["pdbin", "h5"])
+def test_driver_run_matches_oracle(tmp_path, oracle_lib, container):
+    s = small_impact(seed=13)
+    s.self_contact = 1
+    s.enable_fracture = 1
+    setup = tmp_path / "setup.pdbin"
+    setupio.write_setup(s, setup)
+    has_h5 = "hdf5 yes" in run_driver("--features").stdout
+    if container == "h5":
+        if not has_h5:
+            pytest.skip("driver built without HDF5")
+        h = tmp_path / "setup.h5"
+        run_driver("--convert", str(setup), str(h))
+        setup = h
+    T, modulo, zero_at = 30, 10, 15
+    cfg = tmp_path / "main.conf"
+    setupio.write_config(cfg, timesteps=T, modulo=modulo, dt=s.dt, enable_fracture=1, self_contact=1,
+                         allow_damping=1, allow_friction=1, compute_wall_reaction=1,
+                         set_zero_wall_speed_timestep=zero_at,
+                         speed_wall_x_min=s.wall_speed[0], speed_wall_y_min=s.wall_speed[1],
+                         speed_wall_z_min=s.wall_speed[2], speed_wall_x_max=s.wall_speed[3],
+                         speed_wall_y_max=s.wall_speed[4], speed_wall_z_max=s.wall_speed[5])
+    out = tmp_path / "out"
+    print(f"[driver] container={container} T={T}", flush=True)
+    r = run_driver("-c", str(cfg), "-i", str(setup), "-o", str(out))
+    assert "Starting time loop" in r.stdout
+    snaps = _oracle_run(oracle_lib, s, T, modulo, zero_at)
+    for c, t in enumerate(range(modulo, T + 1, modulo), start=1):
+        tc = out / f"tc_{c:05d}.h5"
+        wl = out / f"wall_{c:05d}.h5"
+        if container == "h5":
+            run_driver("--convert", str(tc), str(tmp_path / "tc.pdbin"))
+            run_driver("--convert", str(wl), str(tmp_path / "wl.pdbin"))
+            tc, wl = tmp_path / "tc.pdbin", tmp_path / "wl.pdbin"
+        got, gw = setupio.read_pdbin(tc), setupio.read_pdbin(wl)
+        states, conns, (lrtb, reac) = snaps[t]
+        for p in range(len(s.particles)):
+            g = f"P_{p:05d}/"
+            for k, key in (("CurrPos", "currpos"), ("vel", "vel"), ("acc", "acc"), ("force", "force")):
+                assert_bitwise(got[g + k], states[p][key], f"step {t} {g}{k}")
+            assert np.array_equal(got[g + "Connectivity"], conns[p]), f"step {t} {g}Connectivity"
+        assert_bitwise(gw["wall_info"].ravel(), lrtb, f"step {t} wall_info")
+        assert_bitwise(gw["reaction"], reac, f"step {t} reaction")
+    assert (out / "plotinfo.h5").exists() and (out / "run_time.h5").exists()
