@@ -153,10 +153,54 @@ int pd_get_connectivity(pd_ctx* ctx, uint32_t p, uint32_t* pairs,
 /* Wall planes (lrtb order) and, when compute_wall_reaction is on, the
  * reaction of the last step as get_reaction() orders it (contact.h:255-272):
  * [x_min, y_min, z_min, x_max, y_max, z_max] x 3 components. */
-int pd_get_wall(pd_ctx* ctx, uint32_t instance, double lrtb[6], double reaction[18]);
+int pd_get_wall(pd_ctx* ctx, uint32_t job, double lrtb[6], double reaction[18]);
 int pd_get_counters(pd_ctx* ctx, pd_counters* out);
 /* Time of the last pd_step call on the device (HIP events), milliseconds. */
 int pd_last_step_ms(pd_ctx* ctx, double* ms);
+
+/* ---------------------------------------------------------------------------
+ * Job pool: many independent systems advanced by the same launches.
+ *
+ * No counterpart in the reference, which runs one system per process
+ * (run/simulate3d.cpp:142).  A job is what one reference run holds -- its
+ * particles (pd_particle_desc, the `instance` field ignored), its
+ * pd_run_desc -- plus a step budget.  Each pd_step advances every job that
+ * has budget left by min(nsteps, remaining) steps, each job with its own dt,
+ * clock t, walls and switches; a job whose budget is used up stays frozen
+ * (state unchanged) until it is read back and removed.  A job's results are
+ * bitwise those of the same system run alone.  pd_commit is the plain case:
+ * one job per instance value, unlimited budget, plain particle indices
+ * 0..n-1 in the order of pd_add_particle.
+ *
+ * Capacities are fixed by pd_reserve (one allocation of device memory);
+ * pd_add_job fails with PD_E_NOMEM when a job does not fit in the free
+ * space.  pd_job_footprint gives what a job needs, for admission control. */
+typedef struct pd_pool_desc {
+  uint64_t max_nodes;         /* node slots (rounded up to 64) */
+  uint64_t max_bond_entries;  /* sliced bond entries (see pd_job_footprint) */
+  uint64_t max_particles;
+  uint64_t max_jobs;
+} pd_pool_desc;
+int pd_reserve(pd_ctx* ctx, const pd_pool_desc* pool);
+/* node_slots, bond_entries: pool units the job occupies; bytes: device
+ * memory they stand for.  Any output pointer may be NULL. */
+int pd_job_footprint(const pd_particle_desc* parts, uint32_t nparts, uint64_t* node_slots,
+                     uint64_t* bond_entries, uint64_t* bytes);
+/* nsteps = UINT64_MAX: no budget limit.  *job receives the job slot. */
+int pd_add_job(pd_ctx* ctx, const pd_run_desc* run, const pd_particle_desc* parts,
+               uint32_t nparts, uint64_t nsteps, uint32_t* job);
+int pd_remove_job(pd_ctx* ctx, uint32_t job);
+int pd_job_remaining(pd_ctx* ctx, uint32_t job, uint64_t* remaining);
+/* pd_get_state / pd_get_connectivity / pd_set_flag for particle p of one
+ * job; pd_get_wall takes the job slot as its instance argument. */
+int pd_job_get_state(pd_ctx* ctx, uint32_t job, uint32_t p, double* disp, double* currpos,
+                     double* vel, double* acc, double* force);
+int pd_job_get_connectivity(pd_ctx* ctx, uint32_t job, uint32_t p, uint32_t* pairs,
+                            uint64_t cap, uint64_t* m);
+int pd_job_set_flag(pd_ctx* ctx, uint32_t job, int flag, int value);
+/* Runs of 8 steps replay one captured HIP graph (default on); off: every
+ * step is launched kernel by kernel.  Results are the same either way. */
+int pd_set_graphs(pd_ctx* ctx, int on);
 
 /* Per-phase device time, measured with HIP events on the context's stream
  * while profiling is on (each step then ends with a host wait, so profiled

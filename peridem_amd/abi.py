@@ -55,6 +55,11 @@ class ParticleDesc(C.Structure):
                 ("stoppable", _i32), ("torque_axis", _u32), ("torque_val", _dbl)]
 
 
+class PoolDesc(C.Structure):
+    _fields_ = [("max_nodes", _u64), ("max_bond_entries", _u64), ("max_particles", _u64),
+                ("max_jobs", _u64)]
+
+
 class Counters(C.Structure):
     _fields_ = [(n, _u64) for n in ("steps", "bond_updates", "pair_computations",
                                     "alive_bonds", "total_bonds", "nodes", "particles",
@@ -131,6 +136,21 @@ def load_gpu_lib(path=GPU_LIB):
         lib.pd_math_mode.restype = C.c_int
         lib.pd_selftest_math.argtypes = [_u64, _u64, C.POINTER(_u64), C.POINTER(_u64)]
         lib.pd_selftest_math.restype = C.c_int
+        P = C.c_void_p
+        for name, res, args in (
+                ("pd_reserve", C.c_int, [P, C.POINTER(PoolDesc)]),
+                ("pd_job_footprint", C.c_int, [C.POINTER(ParticleDesc), _u32, C.POINTER(_u64),
+                                               C.POINTER(_u64), C.POINTER(_u64)]),
+                ("pd_add_job", C.c_int, [P, C.POINTER(RunDesc), C.POINTER(ParticleDesc), _u32, _u64,
+                                         _pu]),
+                ("pd_remove_job", C.c_int, [P, _u32]),
+                ("pd_job_remaining", C.c_int, [P, _u32, C.POINTER(_u64)]),
+                ("pd_job_get_state", C.c_int, [P, _u32, _u32, _pd, _pd, _pd, _pd, _pd]),
+                ("pd_job_get_connectivity", C.c_int, [P, _u32, _u32, _pu, _u64, C.POINTER(_u64)]),
+                ("pd_job_set_flag", C.c_int, [P, _u32, C.c_int, C.c_int]),
+                ("pd_set_graphs", C.c_int, [P, C.c_int])):
+            f = getattr(lib, name)
+            f.restype, f.argtypes = res, args
         _LIBS[path] = (lib, _bind(lib, "pd_"))
     return _LIBS[path]
 
@@ -224,10 +244,140 @@ class Engine:
         self._check(self.lib.pd_get_profile(self.ctx, C.byref(pr)), "get_profile")
         return pr.as_dict()
 
+    def set_graphs(self, on=True):
+        self._check(self.lib.pd_set_graphs(self.ctx, int(on)), "set_graphs")
+
     def math_mode_safe(self):
         v = C.c_int()
         self._check(self.lib.pd_math_mode(self.ctx, C.byref(v)), "math_mode")
         return bool(v.value)
+
+    def close(self):
+        if self.ctx:
+            self.f["destroy"](self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+UNLIMITED = (1 << 64) - 1
+
+
+def _descs(system, keep):
+    arr = (ParticleDesc * len(system.particles))()
+    for i, p in enumerate(system.particles):
+        arr[i] = p.desc(keep)
+    return arr
+
+
+def footprint(system):
+    """(node slots, bond entries, device bytes) a system takes in a job pool."""
+    lib, _ = load_gpu_lib()
+    keep = []
+    arr = _descs(system, keep)
+    ns, ne, nb = _u64(), _u64(), _u64()
+    rc = lib.pd_job_footprint(arr, len(system.particles), C.byref(ns), C.byref(ne), C.byref(nb))
+    if rc != PD_OK:
+        raise PDError(f"pd_job_footprint: {ERRORS.get(rc, rc)}")
+    return ns.value, ne.value, nb.value
+
+
+class Pool:
+    """A job pool of the HIP library: independent systems advanced together."""
+
+    def __init__(self, max_nodes, max_bond_entries, max_particles, max_jobs):
+        self.lib, self.f = load_gpu_lib()
+        self.ctx = C.c_void_p()
+        self.systems = {}
+        rd = RunDesc()
+        rd.dt = 1.0
+        self._check(self.f["create"](C.byref(self.ctx), C.byref(rd)), "create")
+        pool = PoolDesc(max_nodes, max_bond_entries, max_particles, max_jobs)
+        self._check(self.lib.pd_reserve(self.ctx, C.byref(pool)), "reserve")
+
+    def _check(self, rc, what):
+        if rc != PD_OK:
+            msg = self.f["last_error"](self.ctx) if self.ctx else b""
+            raise PDError(f"pd_{what}: {ERRORS.get(rc, rc)}: "
+                          f"{(msg or b'').decode(errors='replace')}")
+
+    def add_job(self, system, nsteps=UNLIMITED):
+        """Admit a system with a step budget; returns its job slot."""
+        keep = []
+        arr = _descs(system, keep)
+        rd = system.run_desc()
+        job = _u32()
+        self._check(self.lib.pd_add_job(self.ctx, C.byref(rd), arr, len(system.particles),
+                                        nsteps, C.byref(job)), "add_job")
+        self.systems[job.value] = system
+        return job.value
+
+    def try_add_job(self, system, nsteps=UNLIMITED):
+        """add_job, or None when the pool has no room (PD_E_NOMEM)."""
+        keep = []
+        arr = _descs(system, keep)
+        rd = system.run_desc()
+        job = _u32()
+        rc = self.lib.pd_add_job(self.ctx, C.byref(rd), arr, len(system.particles), nsteps,
+                                 C.byref(job))
+        if rc == -6:
+            return None
+        self._check(rc, "add_job")
+        self.systems[job.value] = system
+        return job.value
+
+    def remove_job(self, job):
+        self._check(self.lib.pd_remove_job(self.ctx, job), "remove_job")
+        self.systems.pop(job, None)
+
+    def remaining(self, job):
+        r = _u64()
+        self._check(self.lib.pd_job_remaining(self.ctx, job, C.byref(r)), "job_remaining")
+        return r.value
+
+    def step(self, n=1):
+        self._check(self.f["step"](self.ctx, n), "step")
+
+    def state(self, job, p):
+        n = self.systems[job].particles[p].n
+        out = {k: np.empty((n, 3)) for k in ("disp", "currpos", "vel", "acc", "force")}
+        self._check(self.lib.pd_job_get_state(self.ctx, job, p, *[
+            _ptr(out[k]) for k in ("disp", "currpos", "vel", "acc", "force")]), "job_get_state")
+        return out
+
+    def connectivity(self, job, p):
+        m = _u64()
+        self._check(self.lib.pd_job_get_connectivity(self.ctx, job, p, None, 0, C.byref(m)),
+                    "job_get_connectivity")
+        out = np.empty((m.value, 2), dtype=np.uint32)
+        self._check(self.lib.pd_job_get_connectivity(self.ctx, job, p, _ptr(out, C.c_uint32),
+                                                     m.value, C.byref(m)), "job_get_connectivity")
+        return out
+
+    def wall(self, job):
+        lrtb, reac = np.empty(6), np.empty(18)
+        self._check(self.f["get_wall"](self.ctx, job, _ptr(lrtb), _ptr(reac)), "get_wall")
+        return lrtb, reac.reshape(6, 3)
+
+    def set_flag(self, job, flag, value):
+        self._check(self.lib.pd_job_set_flag(self.ctx, job, flag, int(value)), "job_set_flag")
+
+    def set_graphs(self, on=True):
+        self._check(self.lib.pd_set_graphs(self.ctx, int(on)), "set_graphs")
+
+    def counters(self):
+        c = Counters()
+        self._check(self.f["get_counters"](self.ctx, C.byref(c)), "get_counters")
+        return c.as_dict()
+
+    def last_step_ms(self):
+        ms = C.c_double()
+        self._check(self.lib.pd_last_step_ms(self.ctx, C.byref(ms)), "last_step_ms")
+        return ms.value
 
     def close(self):
         if self.ctx:

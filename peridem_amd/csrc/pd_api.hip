@@ -1,8 +1,17 @@
 // pd_api.hip -- host side of the C-ABI declared in include/peridem_gpu.h.
 //
-// Owns the device state of one context, builds the neighbour rows in the
-// reference's conn2NArr order (load_hdf5.h:27-37), lays them out as 64-node
-// bond slices and drives the per-step launch sequence on one HIP stream.
+// Device state is a pool of job slots.  A job is one independent system (the
+// particles of one reference run, its contact and wall parameters, its own
+// step counter and step budget).  A job occupies whole 64-node bond slices
+// and one contiguous range of bond entries; every step launch covers all
+// resident jobs, and a job that has used its budget stays frozen in place
+// until it is read back and removed.  The plain reference interface
+// (pd_add_particle + pd_commit) is one job per instance value with an
+// unlimited budget.
+//
+// Rows are built in the reference's conn2NArr order (load_hdf5.h:27-37) and
+// laid out as 64-node slices.  The per-step launch sequence runs on one HIP
+// stream; runs of steps replay a captured HIP graph of kGraphSteps steps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -10,6 +19,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -21,55 +31,113 @@ using namespace pd;
 
 namespace {
 
+constexpr uint64_t kUnlimited = ~0ull;
+constexpr int kGraphSteps = 8;  // steps per captured graph
+// device bytes per node slot (X, U, V, A, F, E, pid, nflag, deg, wmask, ckey,
+// csorted, ~2 hash buckets) and per bond entry
+constexpr uint64_t kBytesPerNode = 32 + 32 + 4 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16;
+constexpr uint64_t kBytesPerEntry = 4;
+
 struct HostPart {
-  uint32_t n = 0, inst = 0;
-  std::vector<double> pos, vol, disp, vel, acc, ext;
-  std::vector<uint32_t> conn, clamped, bdry;
-  double delta = 0, rho = 0, cnot = 0, snot = 0;
+  uint32_t n = 0;
+  std::vector<double> pos;
+  std::vector<uint32_t> clamped;
+  double snot = 0;
   int movable = 1, breakable = 1, stoppable = 1;
-  uint32_t taxis = 0;
-  double tval = 0;
-  uint64_t off = 0;      // first global node
-  uint64_t nbonds = 0;   // directed entries
+  uint64_t off = 0;         // first node slot
+  uint32_t slot = 0;        // particle slot
+  uint64_t nbonds = 0;      // directed entries
   std::vector<double> cl0;  // clamped nodes: disp, vel, acc when the state was set
 };
 
-struct HostInst {
-  pd_run_desc run;
-  double wall[6], speed[6];
+struct Job {
+  bool used = false;
+  pd_run_desc run{};
+  std::vector<HostPart> parts;
+  uint64_t node0 = 0, nslices = 0;   // node slots [node0, node0 + 64*nslices)
+  uint64_t bond0 = 0, nentries = 0;  // bond entries [bond0, bond0 + nentries)
+  uint64_t remaining = 0, steps_done = 0, steps_since_set = 0;
+  uint64_t movable_bonds = 0, total_bonds = 0, nodes = 0;
+  bool reaction_valid = false;
 };
 
-template <typename T>
-struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
+// first-fit allocator over [0, cap)
+struct Ranges {
+  std::map<uint64_t, uint64_t> fr;  // start -> length
+  void init(uint64_t cap) {
+    fr.clear();
+    if (cap) fr[0] = cap;
+  }
+  bool alloc(uint64_t n, uint64_t* at) {
+    if (n == 0) {
+      *at = 0;
+      return true;
+    }
+    for (auto it = fr.begin(); it != fr.end(); ++it)
+      if (it->second >= n) {
+        *at = it->first;
+        uint64_t rest = it->second - n, s = it->first + n;
+        fr.erase(it);
+        if (rest) fr[s] = rest;
+        return true;
+      }
+    return false;
+  }
+  void release(uint64_t at, uint64_t n) {
+    if (!n) return;
+    auto it = fr.emplace(at, n).first;
+    auto nx = std::next(it);
+    if (nx != fr.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      fr.erase(nx);
+    }
+    if (it != fr.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        fr.erase(it);
+      }
+    }
+  }
+};
+
+// rows of one job in conn2NArr order, node indices relative to the job
+struct JobRows {
+  uint64_t nodes = 0, nslices = 0, entries = 0;
+  std::vector<uint32_t> deg;
+  std::vector<uint64_t> rstart;
+  std::vector<uint32_t> rows;
+  std::vector<uint32_t> width;  // per slice
+  std::vector<uint64_t> poff;   // per particle: first node (job-relative)
 };
 
 }  // namespace
 
 struct pd_ctx {
-  std::vector<HostPart> parts;
-  std::vector<HostInst> insts;
-  bool committed = false;
+  pd_run_desc run0{};
+  // pd_add_particle copies, owned here until pd_commit
+  std::vector<pd_particle_desc> pending;
+  std::vector<std::vector<double>> keep_d;
+  std::vector<std::vector<uint32_t>> keep_u;
+  std::vector<std::pair<uint32_t, uint32_t>> plain;  // plain particle -> (job, local)
+  bool committed = false, reserved = false;
   std::string err;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int device = 0;
   Dev d{};
   std::vector<void*> allocs;
-  uint64_t* inst_nodes = nullptr;   // device [ninst+1]
-  std::vector<uint64_t> inst_nodes_h;
-  uint64_t nbond_entries = 0;       // slice entries incl. padding
-  uint32_t t = 1;                   // next reference step index
+  uint64_t cap_slices = 0, cap_entries = 0, cap_parts = 0, cap_jobs = 0, cap_pairs = 0;
+  std::vector<Job> jobs;
+  Ranges node_free, bond_free;
+  std::vector<uint32_t> part_free;
   pd_counters cnt{};
   double last_ms = 0;
-  bool reaction_valid = false;
-  std::vector<uint64_t> movable_bonds;  // per particle directed bonds
-  uint64_t steps_since_set = 0;
   bool prof = false;
-  bool safe_math_static = false;
   hipEvent_t pev[PD_NPHASE + 1] = {};
   pd_profile acc{};
+  bool use_graph = true, graph_dirty = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
 };
 
 static int fail(pd_ctx* c, int code, const char* fmt, ...) {
@@ -82,12 +150,12 @@ static int fail(pd_ctx* c, int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(c, call)                                                              \
-  do {                                                                               \
-    hipError_t e_ = (call);                                                          \
-    if (e_ != hipSuccess)                                                            \
+#define HIPCHK(c, call)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
       return fail((c), PD_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                  __FILE__, __LINE__);                                               \
+                  __FILE__, __LINE__);                                                   \
   } while (0)
 
 template <typename T>
@@ -103,9 +171,166 @@ static int dalloc(pd_ctx* c, T** p, size_t n) {
 
 static inline unsigned grid_of(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+static void void_part(DevPart& q, uint32_t void_inst) {
+  memset(&q, 0, sizeof q);
+  q.inst = void_inst;
+  q.rho = q.rinv = 1.0;
+}
+
+static void void_inst(DevInst& q) {
+  memset(&q, 0, sizeof q);
+  q.dt = 1.0;
+}
+
+static void fill_inst(const pd_run_desc& r, DevInst& D) {
+  D.Rc = r.contact_rad;
+  D.Kn = r.normal_stiffness;
+  D.mu = r.friction_coefficient;
+  D.zeta = r.damping_ratio;
+  // cell edge slightly above R_c: the reference's pair distance
+  // (u_j-u_i)+(X_j-X_i) differs from CurrPos_j-CurrPos_i by a few ulp
+  D.cell = r.contact_rad * (1.0 + 1e-9) + 1e-300;
+  D.dt = r.dt;
+  D.allow_wall = r.allow_wall;
+  D.allow_damping = r.allow_damping;
+  D.allow_friction = r.allow_friction;
+  D.self_contact = r.self_contact;
+  D.enable_fracture = r.enable_fracture;
+  D.gradient_extforce = r.gradient_extforce;
+  D.compute_reaction = r.compute_wall_reaction;
+  D.extforce_maxstep = r.extforce_maxstep;
+  D.pad0 = 0;
+}
+
+static int check_run(pd_ctx* c, const pd_run_desc& r) {
+  if (r.nl_bdry_only) return fail(c, PD_E_UNSUPPORTED, "nl_bdry_only is not built yet");
+  if (r.enable_torque) return fail(c, PD_E_UNSUPPORTED, "enable_torque is not built yet");
+  if (!(r.dt > 0)) return fail(c, PD_E_ARG, "dt must be positive");
+  return PD_OK;
+}
+
+static int check_particles(pd_ctx* c, const pd_particle_desc* P, uint32_t n) {
+  for (uint32_t i = 0; i < n; i++) {
+    const pd_particle_desc& d = P[i];
+    if (d.nnodes == 0 || !d.pos || !d.vol) return fail(c, PD_E_ARG, "particle %u needs nnodes, pos, vol", i);
+    if (d.nconn && !d.conn) return fail(c, PD_E_ARG, "particle %u: nconn > 0 but conn is NULL", i);
+    for (uint64_t l = 0; l < 2 * d.nconn; l++)
+      if (d.conn[l] >= d.nnodes)
+        return fail(c, PD_E_ARG, "particle %u: Connectivity entry %llu out of range", i,
+                    (unsigned long long)l);
+    if (d.nclamped && !d.clamped) return fail(c, PD_E_ARG, "particle %u: clamped is NULL", i);
+    for (uint32_t q = 0; q < d.nclamped; q++)
+      if (d.clamped[q] >= d.nnodes) return fail(c, PD_E_ARG, "particle %u: clamped node out of range", i);
+    if (d.nbdry && !d.bdry) return fail(c, PD_E_ARG, "particle %u: bdry is NULL", i);
+    for (uint32_t q = 0; q < d.nbdry; q++)
+      if (d.bdry[q] >= d.nnodes) return fail(c, PD_E_ARG, "particle %u: boundary node out of range", i);
+  }
+  return PD_OK;
+}
+
+// conn2NArr rows of a job (load_hdf5.h:27-37) and its slice widths
+static void build_rows(const pd_particle_desc* P, uint32_t np, JobRows& R) {
+  R.poff.resize(np);
+  uint64_t nodes = 0;
+  for (uint32_t i = 0; i < np; i++) {
+    R.poff[i] = nodes;
+    nodes += P[i].nnodes;
+  }
+  R.nodes = nodes;
+  R.nslices = (nodes + kSlice - 1) / kSlice;
+  const uint64_t Np = R.nslices * kSlice;
+  R.deg.assign(Np, 0);
+  for (uint32_t i = 0; i < np; i++)
+    for (uint64_t l = 0; l < 2 * P[i].nconn; l++) R.deg[R.poff[i] + P[i].conn[l]]++;
+  R.rstart.assign(Np + 1, 0);
+  for (uint64_t g = 0; g < Np; g++) R.rstart[g + 1] = R.rstart[g] + R.deg[g];
+  R.rows.resize(R.rstart[Np]);
+  std::vector<uint64_t> fp(R.rstart.begin(), R.rstart.end() - 1);
+  for (uint32_t i = 0; i < np; i++)
+    for (uint64_t l = 0; l < P[i].nconn; l++) {
+      uint64_t a = R.poff[i] + P[i].conn[2 * l], b = R.poff[i] + P[i].conn[2 * l + 1];
+      R.rows[fp[a]++] = (uint32_t)b;
+      R.rows[fp[b]++] = (uint32_t)a;
+    }
+  R.width.assign(R.nslices, 0);
+  R.entries = 0;
+  for (uint64_t s = 0; s < R.nslices; s++) {
+    uint32_t W = 0;
+    for (int l = 0; l < kSlice; l++) W = std::max(W, R.deg[s * kSlice + l]);
+    R.width[s] = W;
+    R.entries += (uint64_t)W * kSlice;
+  }
+}
+
+static bool rows_sorted(const JobRows& R, uint64_t g0, uint64_t g1) {
+  for (uint64_t g = g0; g < g1; g++)
+    for (uint64_t k = R.rstart[g] + 1; k < R.rstart[g + 1]; k++)
+      if (R.rows[k] < R.rows[k - 1]) return false;
+  return true;
+}
+
+static bool any_contact_possible(pd_ctx* c) {
+  if (c->d.npairs) return true;
+  for (auto& J : c->jobs)
+    if (J.used && J.run.self_contact && J.run.enable_fracture) return true;
+  return false;
+}
+
+static bool any_reaction(pd_ctx* c) {
+  for (auto& J : c->jobs)
+    if (J.used && J.run.compute_wall_reaction && J.run.allow_wall) return true;
+  return false;
+}
+
+// particle pairs (i<j) of the same job, in the reference's pair order
+// (timeloop.h:243-255); upload is synchronous
+static int upload_pairs(pd_ctx* c) {
+  std::vector<uint32_t> pairs;
+  for (auto& J : c->jobs) {
+    if (!J.used) continue;
+    for (size_t a = 0; a < J.parts.size(); a++)
+      for (size_t b = a + 1; b < J.parts.size(); b++) {
+        pairs.push_back(J.parts[a].slot);
+        pairs.push_back(J.parts[b].slot);
+      }
+  }
+  const uint64_t np = pairs.size() / 2;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (np > c->cap_pairs) {
+    uint64_t cap = std::max<uint64_t>(np, 2 * c->cap_pairs);
+    uint32_t* nb = nullptr;
+    int rc = dalloc(c, &nb, 2 * cap);
+    if (rc) return rc;
+    if (c->d.pairs) {
+      auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->d.pairs);
+      if (it != c->allocs.end()) c->allocs.erase(it);
+      HIPCHK(c, hipFree(c->d.pairs));
+    }
+    c->d.pairs = nb;
+    c->cap_pairs = cap;
+  }
+  if (np) HIPCHK(c, hipMemcpy(c->d.pairs, pairs.data(), pairs.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  c->d.npairs = np;
+  c->graph_dirty = true;
+  return PD_OK;
+}
+
+static void recount(pd_ctx* c) {
+  uint64_t nodes = 0, parts = 0, bonds = 0;
+  for (auto& J : c->jobs)
+    if (J.used) {
+      nodes += J.nodes;
+      parts += J.parts.size();
+      bonds += J.total_bonds;
+    }
+  c->cnt.nodes = nodes;
+  c->cnt.particles = parts;
+  c->cnt.total_bonds = bonds;
+}
+
 extern "C" {
 
-const char* pd_version(void) { return "peridem_gpu 0.1 (gfx950, FP64, bitwise reference order)"; }
+const char* pd_version(void) { return "peridem_gpu 0.2 (gfx950, FP64, bitwise reference order, job pool)"; }
 
 int pd_create(pd_ctx** out, const pd_run_desc* run) {
   if (!out || !run) return PD_E_ARG;
@@ -113,13 +338,7 @@ int pd_create(pd_ctx** out, const pd_run_desc* run) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PD_E_HIP;
   pd_ctx* c = new pd_ctx();
-  hipGetDevice(&c->device);
-  HostInst I;
-  I.run = *run;
-  memcpy(I.wall, run->wall_lrtb, sizeof I.wall);
-  memcpy(I.speed, run->wall_speed, sizeof I.speed);
-  c->insts.push_back(I);
-  c->t = run->first_step ? run->first_step : 1;
+  c->run0 = *run;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
@@ -132,341 +351,422 @@ int pd_create(pd_ctx** out, const pd_run_desc* run) {
 int pd_add_particle(pd_ctx* c, const pd_particle_desc* d, uint32_t* index) {
   if (!c || !d) return PD_E_ARG;
   if (c->committed) return fail(c, PD_E_STATE, "pd_add_particle after pd_commit");
-  if (d->nnodes == 0 || !d->pos || !d->vol) return fail(c, PD_E_ARG, "particle needs nnodes, pos, vol");
-  if (d->instance >= c->insts.size()) return fail(c, PD_E_ARG, "unknown instance %u", d->instance);
-  if (d->nconn && !d->conn) return fail(c, PD_E_ARG, "nconn > 0 but conn is NULL");
-  HostPart p;
-  uint32_t n = p.n = d->nnodes;
-  p.inst = d->instance;
-  auto cp3 = [n](const double* s, std::vector<double>& v, int k) {
-    v.assign((size_t)n * k, 0.0);
-    if (s) memcpy(v.data(), s, (size_t)n * k * sizeof(double));
+  int rc = check_particles(c, d, 1);
+  if (rc) return rc;
+  if (!c->pending.empty() && d->instance < c->pending.back().instance)
+    return fail(c, PD_E_ARG, "particles must be added in instance order");
+  // the caller keeps ownership of its arrays: copy them
+  pd_particle_desc q = *d;
+  const uint64_t n = d->nnodes;
+  auto cpd = [&](const double* s, size_t k) -> const double* {
+    if (!s) return nullptr;
+    c->keep_d.emplace_back(s, s + k);
+    return c->keep_d.back().data();
   };
-  cp3(d->pos, p.pos, 3);
-  cp3(d->vol, p.vol, 1);
-  cp3(d->disp, p.disp, 3);
-  cp3(d->vel, p.vel, 3);
-  cp3(d->acc, p.acc, 3);
-  cp3(d->extforce, p.ext, 3);
-  p.conn.assign(d->conn, d->conn + 2 * d->nconn);
-  for (uint64_t l = 0; l < 2 * d->nconn; l++)
-    if (p.conn[l] >= n) return fail(c, PD_E_ARG, "Connectivity entry %llu out of range", (unsigned long long)l);
-  if (d->nclamped) p.clamped.assign(d->clamped, d->clamped + d->nclamped);
-  for (auto q : p.clamped)
-    if (q >= n) return fail(c, PD_E_ARG, "clamped node out of range");
-  if (d->nbdry) p.bdry.assign(d->bdry, d->bdry + d->nbdry);
-  for (auto q : p.bdry)
-    if (q >= n) return fail(c, PD_E_ARG, "boundary node out of range");
-  p.delta = d->delta;
-  p.rho = d->rho;
-  p.cnot = d->cnot;
-  p.snot = d->snot;
-  p.movable = d->movable;
-  p.breakable = d->breakable;
-  p.stoppable = d->stoppable;
-  p.taxis = d->torque_axis;
-  p.tval = d->torque_val;
-  if (index) *index = (uint32_t)c->parts.size();
-  c->parts.push_back(std::move(p));
+  auto cpu = [&](const uint32_t* s, size_t k) -> const uint32_t* {
+    if (!s || !k) return nullptr;
+    c->keep_u.emplace_back(s, s + k);
+    return c->keep_u.back().data();
+  };
+  q.pos = cpd(d->pos, 3 * n);
+  q.vol = cpd(d->vol, n);
+  q.disp = cpd(d->disp, 3 * n);
+  q.vel = cpd(d->vel, 3 * n);
+  q.acc = cpd(d->acc, 3 * n);
+  q.extforce = cpd(d->extforce, 3 * n);
+  q.conn = cpu(d->conn, 2 * d->nconn);
+  q.clamped = cpu(d->clamped, d->nclamped);
+  q.bdry = cpu(d->bdry, d->nbdry);
+  if (index) *index = (uint32_t)c->pending.size();
+  c->pending.push_back(q);
   return PD_OK;
 }
 
-static int check_supported(pd_ctx* c) {
-  for (auto& I : c->insts) {
-    if (I.run.nl_bdry_only) return fail(c, PD_E_UNSUPPORTED, "nl_bdry_only is not built yet");
-    if (I.run.enable_torque) return fail(c, PD_E_UNSUPPORTED, "enable_torque is not built yet");
+int pd_job_footprint(const pd_particle_desc* parts, uint32_t nparts, uint64_t* node_slots,
+                     uint64_t* bond_entries, uint64_t* bytes) {
+  if (!parts || !nparts) return PD_E_ARG;
+  if (check_particles(nullptr, parts, nparts)) return PD_E_ARG;
+  JobRows R;
+  build_rows(parts, nparts, R);
+  if (node_slots) *node_slots = R.nslices * kSlice;
+  if (bond_entries) *bond_entries = R.entries;
+  if (bytes) *bytes = R.nslices * kSlice * kBytesPerNode + R.entries * kBytesPerEntry;
+  return PD_OK;
+}
+
+int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
+  if (!c || !pool) return PD_E_ARG;
+  if (c->reserved) return fail(c, PD_E_STATE, "pool already reserved");
+  if (!pool->max_nodes || !pool->max_jobs || !pool->max_particles)
+    return fail(c, PD_E_ARG, "pool needs nodes, particles and jobs");
+  const uint64_t S = (pool->max_nodes + kSlice - 1) / kSlice, N = S * kSlice;
+  if (N >= 0x7FFFFFFFull) return fail(c, PD_E_OVERFLOW, "more than 2^31-1 node slots");
+  if (pool->max_particles >= 0x7FFFFFFFull || pool->max_jobs >= 0x7FFFFFFFull)
+    return fail(c, PD_E_OVERFLOW, "too many particle or job slots");
+  c->cap_slices = S;
+  c->cap_entries = pool->max_bond_entries;
+  c->cap_parts = pool->max_particles;
+  c->cap_jobs = pool->max_jobs;
+  const uint64_t P = c->cap_parts + 1, I = c->cap_jobs + 1;  // + the void slots
+  uint32_t T = 1024;
+  while (T < 2 * N && T < (1u << 31)) T <<= 1;
+  Dev& d = c->d;
+  d.N = N;
+  d.S = S;
+  d.P = P;
+  d.ninst = I;
+  d.T = T;
+  d.npairs = 0;
+  int rc;
+#define ALLOC(f, n) \
+  if ((rc = dalloc(c, &d.f, (n)))) return rc;
+  ALLOC(X, N);
+  ALLOC(U, N);
+  ALLOC(V, 3 * N);
+  ALLOC(A, 3 * N);
+  ALLOC(F, 3 * N);
+  ALLOC(E, 3 * N);
+  ALLOC(pid, N);
+  ALLOC(nflag, N);
+  ALLOC(deg, N);
+  ALLOC(soff, S);
+  ALLOC(swid, S);
+  ALLOC(bond, c->cap_entries);
+  ALLOC(part, P);
+  ALLOC(inst, I);
+  ALLOC(lrtb, 6 * P);
+  ALLOC(lrtb_next, 6 * P);
+  ALLOC(pactive, P);
+  ALLOC(counters, 8);
+  ALLOC(err, 1);
+  ALLOC(flags, 1);
+  ALLOC(ctl, 1);
+  ALLOC(reaction, 18 * I);
+  ALLOC(wmask, N);
+  ALLOC(bflag, N / 256 + 1);
+  ALLOC(ckey, N);
+  ALLOC(cstart, (size_t)T + 1);
+  ALLOC(ccur, T);
+  ALLOC(csorted, N);
+  ALLOC(scan_tmp, T / kScanBlock + 2);
+#undef ALLOC
+  hipStream_t st = c->stream;
+  // every slot starts void: node -> void particle, void particle -> void job
+  const uint32_t vp = (uint32_t)(P - 1), vi = (uint32_t)(I - 1);
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(N, 256)), dim3(256), 0, st, d.pid, N, vp);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemsetAsync(d.X, 0, N * sizeof(double4), st));
+  HIPCHK(c, hipMemsetAsync(d.U, 0, N * sizeof(double4), st));
+  HIPCHK(c, hipMemsetAsync(d.V, 0, 3 * N * sizeof(double), st));
+  HIPCHK(c, hipMemsetAsync(d.A, 0, 3 * N * sizeof(double), st));
+  HIPCHK(c, hipMemsetAsync(d.F, 0, 3 * N * sizeof(double), st));
+  HIPCHK(c, hipMemsetAsync(d.E, 0, 3 * N * sizeof(double), st));
+  HIPCHK(c, hipMemsetAsync(d.nflag, 0, N, st));
+  HIPCHK(c, hipMemsetAsync(d.deg, 0, N * sizeof(uint32_t), st));
+  HIPCHK(c, hipMemsetAsync(d.soff, 0, S * sizeof(uint64_t), st));
+  HIPCHK(c, hipMemsetAsync(d.swid, 0, S * sizeof(uint32_t), st));
+  if (c->cap_entries) HIPCHK(c, hipMemsetAsync(d.bond, 0xFF, c->cap_entries * sizeof(uint32_t), st));
+  HIPCHK(c, hipMemsetAsync(d.lrtb, 0, 6 * P * sizeof(double), st));
+  HIPCHK(c, hipMemsetAsync(d.pactive, 0, P * sizeof(uint32_t), st));
+  HIPCHK(c, hipMemsetAsync(d.counters, 0, 8 * sizeof(unsigned long long), st));
+  HIPCHK(c, hipMemsetAsync(d.err, 0, sizeof(uint32_t), st));
+  HIPCHK(c, hipMemsetAsync(d.flags, 0, sizeof(uint32_t), st));
+  HIPCHK(c, hipMemsetAsync(d.ctl, 0, sizeof(DevCtl), st));
+  HIPCHK(c, hipMemsetAsync(d.reaction, 0, 18 * I * sizeof(double), st));
+  HIPCHK(c, hipMemsetAsync(d.wmask, 0, N, st));
+  HIPCHK(c, hipMemsetAsync(d.bflag, 0, N / 256 + 1, st));
+  std::vector<DevPart> vparts(P);
+  for (auto& q : vparts) void_part(q, vi);
+  std::vector<DevInst> vinst(I);
+  for (auto& q : vinst) void_inst(q);
+  std::vector<unsigned long long> lnext(6 * P);
+  for (uint64_t q = 0; q < P; q++)
+    for (int k = 0; k < 3; k++) {
+      lnext[6 * q + k] = ~0ull;
+      lnext[6 * q + 3 + k] = 0ull;
+    }
+  HIPCHK(c, hipStreamSynchronize(st));
+  HIPCHK(c, hipMemcpy(d.part, vparts.data(), P * sizeof(DevPart), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d.inst, vinst.data(), I * sizeof(DevInst), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d.lrtb_next, lnext.data(), lnext.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+  c->node_free.init(S);
+  c->bond_free.init(c->cap_entries);
+  c->part_free.clear();
+  for (uint64_t q = c->cap_parts; q-- > 0;) c->part_free.push_back((uint32_t)q);
+  c->jobs.assign(c->cap_jobs, Job());
+  c->reserved = true;
+  c->graph_dirty = true;
+  return PD_OK;
+}
+
+int pd_add_job(pd_ctx* c, const pd_run_desc* run, const pd_particle_desc* P, uint32_t np,
+               uint64_t nsteps, uint32_t* job) {
+  if (!c || !run || !P || !np) return PD_E_ARG;
+  if (!c->reserved) return fail(c, PD_E_STATE, "pd_add_job before pd_reserve");
+  int rc;
+  if ((rc = check_run(c, *run)) || (rc = check_particles(c, P, np))) return rc;
+  uint32_t jid = 0;
+  while (jid < c->cap_jobs && c->jobs[jid].used) jid++;
+  if (jid == c->cap_jobs)
+    return fail(c, PD_E_NOMEM, "all %llu job slots are in use", (unsigned long long)c->cap_jobs);
+  if (c->part_free.size() < np) return fail(c, PD_E_NOMEM, "not enough particle slots");
+  JobRows R;
+  build_rows(P, np, R);
+  uint64_t s0 = 0, b0 = 0;
+  if (!c->node_free.alloc(R.nslices, &s0))
+    return fail(c, PD_E_NOMEM, "no free range of %llu node slices", (unsigned long long)R.nslices);
+  if (!c->bond_free.alloc(R.entries, &b0)) {
+    c->node_free.release(s0, R.nslices);
+    return fail(c, PD_E_NOMEM, "no free range of %llu bond entries", (unsigned long long)R.entries);
   }
+  Job& J = c->jobs[jid];
+  J = Job();
+  J.used = true;
+  J.run = *run;
+  J.node0 = s0 * kSlice;
+  J.nslices = R.nslices;
+  J.bond0 = b0;
+  J.nentries = R.entries;
+  J.remaining = nsteps;
+  J.nodes = R.nodes;
+  const uint64_t g0 = J.node0, Np = R.nslices * kSlice;
+  const uint32_t vpart = (uint32_t)(c->d.P - 1);
+  // node arrays of the job; the tail of the last slice stays void
+  std::vector<double4> X(Np, make_double4(0, 0, 0, 0)), U(Np, make_double4(0, 0, 0, 0));
+  std::vector<double> V(3 * Np, 0.0), A(3 * Np, 0.0), E(3 * Np, 0.0);
+  std::vector<uint32_t> pid(Np, vpart);
+  std::vector<uint8_t> nflag(Np, 0);
+  std::vector<DevPart> dparts(np);
+  std::vector<double> lrtb(6 * np);
+  uint32_t fl = 0;
+  for (uint32_t i = 0; i < np; i++) {
+    const pd_particle_desc& d = P[i];
+    HostPart H;
+    H.n = d.nnodes;
+    H.pos.assign(d.pos, d.pos + 3ull * d.nnodes);
+    if (d.nclamped) H.clamped.assign(d.clamped, d.clamped + d.nclamped);
+    H.snot = d.snot;
+    H.movable = d.movable;
+    H.breakable = d.breakable;
+    H.stoppable = d.stoppable;
+    H.off = g0 + R.poff[i];
+    H.slot = c->part_free.back();
+    c->part_free.pop_back();
+    H.nbonds = R.rstart[R.poff[i] + d.nnodes] - R.rstart[R.poff[i]];
+    H.cl0.resize(9 * H.clamped.size());
+    for (size_t q = 0; q < H.clamped.size(); q++)
+      for (int k = 0; k < 3; k++) {
+        uint32_t cn = H.clamped[q];
+        H.cl0[9 * q + k] = d.disp ? d.disp[3 * cn + k] : 0.0;
+        H.cl0[9 * q + 3 + k] = d.vel ? d.vel[3 * cn + k] : 0.0;
+        H.cl0[9 * q + 6 + k] = d.acc ? d.acc[3 * cn + k] : 0.0;
+      }
+    J.total_bonds += H.nbonds;
+    if (H.movable) J.movable_bonds += H.nbonds;
+    DevPart& D = dparts[i];
+    memset(&D, 0, sizeof D);
+    D.node_off = H.off;
+    D.nnodes = d.nnodes;
+    D.inst = jid;
+    D.rho = d.rho;
+    D.rinv = 1 / d.rho;
+    D.cnot = d.cnot;
+    D.snot = d.snot;
+    D.movable = d.movable;
+    D.breakable = d.breakable;
+    D.stoppable = d.stoppable;
+    D.rows_sorted = rows_sorted(R, R.poff[i], R.poff[i] + d.nnodes);
+    double o[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t v = 0; v < d.nnodes; v++) {
+      const uint64_t r = R.poff[i] + v;
+      const double u0 = d.disp ? d.disp[3 * v] : 0.0, u1 = d.disp ? d.disp[3 * v + 1] : 0.0,
+                   u2 = d.disp ? d.disp[3 * v + 2] : 0.0;
+      X[r] = make_double4(d.pos[3 * v], d.pos[3 * v + 1], d.pos[3 * v + 2], d.vol[v]);
+      U[r] = make_double4(u0, u1, u2, 0.0);
+      for (int k = 0; k < 3; k++) {
+        V[3 * r + k] = d.vel ? d.vel[3 * v + k] : 0.0;
+        A[3 * r + k] = d.acc ? d.acc[3 * v + k] : 0.0;
+        E[3 * r + k] = d.extforce ? d.extforce[3 * v + k] : 0.0;
+      }
+      pid[r] = H.slot;
+      if (!(in_fast_range(X[r].x) && in_fast_range(X[r].y) && in_fast_range(X[r].z))) fl = kFlagSafeMath;
+      // initial lagged box: min/max of pos + disp (timeloop.h:389-393)
+      const double cp[3] = {d.pos[3 * v] + u0, d.pos[3 * v + 1] + u1, d.pos[3 * v + 2] + u2};
+      for (int k = 0; k < 3; k++) {
+        if (v == 0) o[k] = o[3 + k] = cp[k];
+        o[k] = (cp[k] < o[k]) ? cp[k] : o[k];
+        o[3 + k] = (o[3 + k] < cp[k]) ? cp[k] : o[3 + k];
+      }
+    }
+    for (auto q : H.clamped) nflag[R.poff[i] + q] |= 1;
+    for (int k = 0; k < 6; k++) lrtb[6 * i + k] = o[k];
+    J.parts.push_back(std::move(H));
+  }
+  // fast-path domain of the bond arithmetic: reference bond lengths
+  // (coordinates above; displacements are checked by the position kernel)
+  for (uint64_t g = 0; g < R.nodes && !fl; g++)
+    for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) {
+      const double4& a = X[g];
+      const double4& b = X[R.rows[k]];
+      double x0 = b.x - a.x, x1 = b.y - a.y, x2 = b.z - a.z;
+      double l2 = (x0 * x0 + x1 * x1) + x2 * x2;
+      if (!(l2 >= kFastLo && l2 <= kFastHi)) {
+        fl = kFlagSafeMath;
+        break;
+      }
+    }
+  // bond slices of the job, entry k of lane l at soff[s] + 64k + l
+  std::vector<uint64_t> soff(R.nslices);
+  std::vector<uint32_t> bond(R.entries, kPad);
+  {
+    uint64_t at = b0;
+    for (uint64_t s = 0; s < R.nslices; s++) {
+      soff[s] = at;
+      at += (uint64_t)R.width[s] * kSlice;
+    }
+    for (uint64_t g = 0; g < R.nodes; g++) {
+      const uint64_t s = g / kSlice, l = g % kSlice;
+      for (uint32_t k = 0; k < R.deg[g]; k++)
+        bond[soff[s] - b0 + (uint64_t)k * kSlice + l] = (uint32_t)(g0 + R.rows[R.rstart[g] + k]);
+    }
+  }
+  DevInst I;
+  memset(&I, 0, sizeof I);
+  fill_inst(*run, I);
+  memcpy(I.wall, run->wall_lrtb, sizeof I.wall);
+  memcpy(I.speed, run->wall_speed, sizeof I.speed);
+  I.remaining = nsteps;
+  I.node0 = g0;
+  I.node1 = g0 + R.nodes;
+  I.t = run->first_step ? run->first_step : 1;
+  const double zero18[18] = {0};
+  Dev& d = c->d;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+#define UP(dst, src, n) HIPCHK(c, hipMemcpy((dst), (src), (n) * sizeof(*(dst)), hipMemcpyHostToDevice))
+  UP(d.X + g0, X.data(), Np);
+  UP(d.U + g0, U.data(), Np);
+  UP(d.V + 3 * g0, V.data(), 3 * Np);
+  UP(d.A + 3 * g0, A.data(), 3 * Np);
+  UP(d.E + 3 * g0, E.data(), 3 * Np);
+  {
+    const std::vector<double> Z(3 * Np, 0.0);
+    UP(d.F + 3 * g0, Z.data(), 3 * Np);
+  }
+  UP(d.pid + g0, pid.data(), Np);
+  UP(d.nflag + g0, nflag.data(), Np);
+  UP(d.deg + g0, R.deg.data(), Np);
+  UP(d.soff + s0, soff.data(), R.nslices);
+  UP(d.swid + s0, R.width.data(), R.nslices);
+  if (R.entries) UP(d.bond + b0, bond.data(), R.entries);
+  for (uint32_t i = 0; i < np; i++) {
+    const uint32_t sl = J.parts[i].slot;
+    UP(d.part + sl, &dparts[i], 1);
+    UP(d.lrtb + 6 * sl, &lrtb[6 * i], 6);
+  }
+  UP(d.inst + jid, &I, 1);
+  UP(d.reaction + 18 * jid, zero18, 18);
+#undef UP
+  if (fl) {
+    uint32_t cur = 0;
+    HIPCHK(c, hipMemcpy(&cur, d.flags, sizeof cur, hipMemcpyDeviceToHost));
+    cur |= fl;
+    HIPCHK(c, hipMemcpy(d.flags, &cur, sizeof cur, hipMemcpyHostToDevice));
+  }
+  if ((rc = upload_pairs(c))) return rc;
+  recount(c);
+  c->graph_dirty = true;
+  if (job) *job = jid;
   return PD_OK;
 }
 
-static void fill_inst(const HostInst& H, DevInst& D) {
-  const pd_run_desc& r = H.run;
-  D.Rc = r.contact_rad;
-  D.Kn = r.normal_stiffness;
-  D.mu = r.friction_coefficient;
-  D.zeta = r.damping_ratio;
-  // cell edge slightly above R_c: the reference's pair distance
-  // (u_j-u_i)+(X_j-X_i) differs from CurrPos_j-CurrPos_i by a few ulp
-  D.cell = r.contact_rad * (1.0 + 1e-9) + 1e-300;
-  memcpy(D.wall, H.wall, sizeof D.wall);
-  memcpy(D.speed, H.speed, sizeof D.speed);
-  D.allow_wall = r.allow_wall;
-  D.allow_damping = r.allow_damping;
-  D.allow_friction = r.allow_friction;
-  D.self_contact = r.self_contact;
-  D.enable_fracture = r.enable_fracture;
-  D.gradient_extforce = r.gradient_extforce;
-  D.extforce_maxstep = r.extforce_maxstep;
-  D.pad0 = 0;
-}
-
-static void fill_part(const HostPart& H, DevPart& D, int rows_sorted) {
-  D.node_off = H.off;
-  D.nnodes = H.n;
-  D.inst = H.inst;
-  D.rho = H.rho;
-  D.rinv = 1 / H.rho;
-  D.cnot = H.cnot;
-  D.snot = H.snot;
-  D.movable = H.movable;
-  D.breakable = H.breakable;
-  D.stoppable = H.stoppable;
-  D.rows_sorted = rows_sorted;
+int pd_remove_job(pd_ctx* c, uint32_t job) {
+  if (!c) return PD_E_ARG;
+  if (job >= c->jobs.size() || !c->jobs[job].used) return fail(c, PD_E_ARG, "no job %u", job);
+  Job& J = c->jobs[job];
+  Dev& d = c->d;
+  hipStream_t st = c->stream;
+  HIPCHK(c, hipStreamSynchronize(st));
+  const uint64_t Np = J.nslices * kSlice;
+  if (Np) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(Np, 256)), dim3(256), 0, st, d.pid + J.node0, Np,
+                       (uint32_t)(d.P - 1));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemsetAsync(d.deg + J.node0, 0, Np * sizeof(uint32_t), st));
+    HIPCHK(c, hipMemsetAsync(d.nflag + J.node0, 0, Np, st));
+    HIPCHK(c, hipMemsetAsync(d.swid + J.node0 / kSlice, 0, J.nslices * sizeof(uint32_t), st));
+  }
+  if (J.nentries) HIPCHK(c, hipMemsetAsync(d.bond + J.bond0, 0xFF, J.nentries * sizeof(uint32_t), st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  DevInst vi;
+  void_inst(vi);
+  HIPCHK(c, hipMemcpy(d.inst + job, &vi, sizeof vi, hipMemcpyHostToDevice));
+  std::vector<DevPart> vps(J.parts.size());
+  for (size_t i = 0; i < J.parts.size(); i++) {
+    void_part(vps[i], (uint32_t)(d.ninst - 1));
+    HIPCHK(c, hipMemcpy(d.part + J.parts[i].slot, &vps[i], sizeof(DevPart), hipMemcpyHostToDevice));
+    c->part_free.push_back(J.parts[i].slot);
+  }
+  c->node_free.release(J.node0 / kSlice, J.nslices);
+  c->bond_free.release(J.bond0, J.nentries);
+  J = Job();
+  int rc = upload_pairs(c);
+  if (rc) return rc;
+  recount(c);
+  c->graph_dirty = true;
+  return PD_OK;
 }
 
 int pd_commit(pd_ctx* c) {
   if (!c) return PD_E_ARG;
   if (c->committed) return fail(c, PD_E_STATE, "already committed");
-  if (c->parts.empty()) return fail(c, PD_E_STATE, "no particles");
-  int rc = check_supported(c);
+  if (c->pending.empty()) return fail(c, PD_E_STATE, "no particles");
+  int rc = check_run(c, c->run0);
   if (rc) return rc;
-  // particles of one instance must be contiguous (pair order within instance)
-  for (size_t i = 1; i < c->parts.size(); i++)
-    if (c->parts[i].inst < c->parts[i - 1].inst)
-      return fail(c, PD_E_ARG, "particles must be added in instance order");
-  uint64_t N = 0;
-  for (auto& p : c->parts) {
-    p.off = N;
-    N += p.n;
+  // one job per instance value, particles in the order they were added
+  std::vector<std::pair<size_t, size_t>> groups;
+  for (size_t i = 0; i < c->pending.size(); i++)
+    if (groups.empty() || c->pending[i].instance != c->pending[groups.back().first].instance)
+      groups.push_back({i, i + 1});
+    else
+      groups.back().second = i + 1;
+  uint64_t slots = 0, entries = 0;
+  for (auto& gr : groups) {
+    uint64_t ns = 0, ne = 0;
+    if (pd_job_footprint(&c->pending[gr.first], (uint32_t)(gr.second - gr.first), &ns, &ne, nullptr))
+      return fail(c, PD_E_ARG, "bad particle description");
+    slots += ns;
+    entries += ne;
   }
-  if (N >= 0x7FFFFFFFull) return fail(c, PD_E_OVERFLOW, "more than 2^31-1 nodes");
-  const uint64_t S = (N + kSlice - 1) / kSlice;
-  const uint64_t Np = S * kSlice;
-
-  // conn2NArr: rows in Connectivity file order (load_hdf5.h:27-37)
-  std::vector<uint32_t> deg(Np, 0);
-  for (auto& p : c->parts)
-    for (size_t l = 0; l < p.conn.size(); l++) deg[p.off + p.conn[l]]++;
-  std::vector<uint64_t> rstart(Np + 1, 0);
-  for (uint64_t g = 0; g < Np; g++) rstart[g + 1] = rstart[g] + deg[g];
-  std::vector<uint32_t> rows(rstart[Np]);
-  {
-    std::vector<uint64_t> fillp(rstart.begin(), rstart.end() - 1);
-    for (auto& p : c->parts)
-      for (size_t l = 0; l + 1 < p.conn.size(); l += 2) {
-        uint64_t i = p.off + p.conn[l], j = p.off + p.conn[l + 1];
-        rows[fillp[i]++] = (uint32_t)j;
-        rows[fillp[j]++] = (uint32_t)i;
-      }
+  if (slots >= 0x7FFFFFFFull) return fail(c, PD_E_OVERFLOW, "more than 2^31-1 nodes");
+  if (!c->reserved) {
+    pd_pool_desc pool{};
+    pool.max_nodes = slots;
+    pool.max_bond_entries = entries;
+    pool.max_particles = c->pending.size();
+    pool.max_jobs = groups.size();
+    if ((rc = pd_reserve(c, &pool))) return rc;
   }
-  // slices of 64 rows, entries interleaved (entry k of lane l at k*64+l)
-  std::vector<uint64_t> soff(S + 1, 0);
-  for (uint64_t s = 0; s < S; s++) {
-    uint32_t W = 0;
-    for (int l = 0; l < kSlice; l++) W = std::max(W, deg[s * kSlice + l]);
-    soff[s + 1] = soff[s] + (uint64_t)W * kSlice;
+  for (auto& gr : groups) {
+    uint32_t job = 0;
+    if ((rc = pd_add_job(c, &c->run0, &c->pending[gr.first], (uint32_t)(gr.second - gr.first), kUnlimited,
+                         &job)))
+      return rc;
+    for (size_t i = gr.first; i < gr.second; i++) c->plain.push_back({job, (uint32_t)(i - gr.first)});
   }
-  c->nbond_entries = soff[S];
-  std::vector<uint32_t> bond(soff[S], kPad);
-  for (uint64_t g = 0; g < N; g++) {
-    uint64_t s = g / kSlice, l = g % kSlice;
-    for (uint32_t k = 0; k < deg[g]; k++) bond[soff[s] + (uint64_t)k * kSlice + l] = rows[rstart[g] + k];
-  }
-  // per particle: rows ascending? (enables binary search for bonded pairs)
-  std::vector<DevPart> dp(c->parts.size());
-  c->movable_bonds.assign(c->parts.size(), 0);
-  for (size_t pi = 0; pi < c->parts.size(); pi++) {
-    auto& p = c->parts[pi];
-    int sorted = 1;
-    uint64_t nb = 0;
-    for (uint64_t g = p.off; g < p.off + p.n; g++) {
-      nb += deg[g];
-      for (uint64_t k = rstart[g] + 1; k < rstart[g + 1] && sorted; k++)
-        if (rows[k] < rows[k - 1]) sorted = 0;
-    }
-    p.nbonds = nb;
-    c->movable_bonds[pi] = nb;
-    c->cnt.total_bonds += nb;
-    fill_part(p, dp[pi], sorted);
-  }
-  std::vector<DevInst> di(c->insts.size());
-  for (size_t i = 0; i < c->insts.size(); i++) fill_inst(c->insts[i], di[i]);
-
-  // node arrays
-  std::vector<double4> X(Np), U(Np);
-  std::vector<double> V(3 * Np, 0.0), A(3 * Np, 0.0), E(3 * Np, 0.0);
-  std::vector<uint32_t> pid(Np, 0);
-  std::vector<uint8_t> nflag(Np, 0);
-  for (size_t pi = 0; pi < c->parts.size(); pi++) {
-    auto& p = c->parts[pi];
-    for (uint32_t i = 0; i < p.n; i++) {
-      uint64_t g = p.off + i;
-      X[g] = make_double4(p.pos[3 * i], p.pos[3 * i + 1], p.pos[3 * i + 2], p.vol[i]);
-      U[g] = make_double4(p.disp[3 * i], p.disp[3 * i + 1], p.disp[3 * i + 2], 0.0);
-      for (int k = 0; k < 3; k++) {
-        V[3 * g + k] = p.vel[3 * i + k];
-        A[3 * g + k] = p.acc[3 * i + k];
-        E[3 * g + k] = p.ext[3 * i + k];
-      }
-      pid[g] = (uint32_t)pi;
-    }
-    for (auto q : p.clamped) nflag[p.off + q] |= 1;
-  }
-  for (uint64_t g = N; g < Np; g++) {
-    X[g] = make_double4(0, 0, 0, 0);
-    U[g] = make_double4(0, 0, 0, 0);
-    pid[g] = 0;
-  }
-  // initial lagged boxes: minmax(pos + disp) of every particle (timeloop.h:389-393)
-  const size_t P = c->parts.size();
-  std::vector<double> lrtb(6 * P);
-  for (size_t pi = 0; pi < P; pi++) {
-    auto& p = c->parts[pi];
-    double o[6];
-    for (int k = 0; k < 3; k++) o[k] = o[3 + k] = p.pos[k] + p.disp[k];
-    for (uint32_t i = 0; i < p.n; i++)
-      for (int k = 0; k < 3; k++) {
-        double v = p.pos[3 * i + k] + p.disp[3 * i + k];
-        o[k] = (v < o[k]) ? v : o[k];
-        o[3 + k] = (o[3 + k] < v) ? v : o[3 + k];
-      }
-    for (int k = 0; k < 6; k++) lrtb[6 * pi + k] = o[k];
-  }
-  std::vector<unsigned long long> lnext(6 * P);
-  for (size_t pi = 0; pi < P; pi++)
-    for (int k = 0; k < 3; k++) {
-      lnext[6 * pi + k] = ~0ull;
-      lnext[6 * pi + 3 + k] = 0ull;
-    }
-  // particle pairs (i<j) within each instance, in the reference's pair order
-  std::vector<uint32_t> pairs;
-  c->inst_nodes_h.assign(c->insts.size() + 1, 0);
-  {
-    size_t i0 = 0;
-    for (size_t i = 0; i <= P; i++) {
-      if (i == P || (i > i0 && c->parts[i].inst != c->parts[i0].inst)) {
-        for (size_t a = i0; a < i; a++)
-          for (size_t b = a + 1; b < i; b++) {
-            pairs.push_back((uint32_t)a);
-            pairs.push_back((uint32_t)b);
-          }
-        i0 = i;
-      }
-    }
-    for (size_t i = 0; i < P; i++) c->inst_nodes_h[c->parts[i].inst + 1] += c->parts[i].n;
-    for (size_t k = 0; k < c->insts.size(); k++) c->inst_nodes_h[k + 1] += c->inst_nodes_h[k];
-  }
-  uint32_t T = 1024;
-  while (T < 2 * N && T < (1u << 31)) T <<= 1;
-
-  Dev& d = c->d;
-  d.N = N;
-  d.S = S;
-  d.P = P;
-  d.npairs = pairs.size() / 2;
-  d.ninst = c->insts.size();
-  d.T = T;
-#define ALLOC(f, n) \
-  if ((rc = dalloc(c, &d.f, (n)))) return rc;
-  ALLOC(X, Np);
-  ALLOC(U, Np);
-  ALLOC(V, 3 * Np);
-  ALLOC(A, 3 * Np);
-  ALLOC(F, 3 * Np);
-  ALLOC(E, 3 * Np);
-  ALLOC(pid, Np);
-  ALLOC(nflag, Np);
-  ALLOC(deg, Np);
-  ALLOC(soff, S + 1);
-  ALLOC(bond, soff[S]);
-  ALLOC(part, P);
-  ALLOC(inst, di.size());
-  ALLOC(lrtb, 6 * P);
-  ALLOC(lrtb_next, 6 * P);
-  ALLOC(pactive, P);
-  ALLOC(pairs, pairs.size());
-  ALLOC(counters, 8);
-  ALLOC(err, 1);
-  ALLOC(flags, 1);
-  ALLOC(reaction, 18 * di.size());
-  ALLOC(wmask, Np);
-  ALLOC(bflag, Np / 256 + 1);
-  ALLOC(ckey, Np);
-  ALLOC(cstart, (size_t)T + 1);
-  ALLOC(ccur, T);
-  ALLOC(csorted, Np);
-  ALLOC(scan_tmp, T / kScanBlock + 2);
-#undef ALLOC
-  if ((rc = dalloc(c, &c->inst_nodes, c->inst_nodes_h.size()))) return rc;
-  hipStream_t st = c->stream;
-#define UP(dst, src, n) HIPCHK(c, hipMemcpyAsync((dst), (src), (n) * sizeof(*(dst)), hipMemcpyHostToDevice, st))
-  UP(d.X, X.data(), Np);
-  UP(d.U, U.data(), Np);
-  UP(d.V, V.data(), 3 * Np);
-  UP(d.A, A.data(), 3 * Np);
-  UP(d.E, E.data(), 3 * Np);
-  UP(d.pid, pid.data(), Np);
-  UP(d.nflag, nflag.data(), Np);
-  UP(d.deg, deg.data(), Np);
-  UP(d.soff, soff.data(), S + 1);
-  if (soff[S]) UP(d.bond, bond.data(), soff[S]);
-  UP(d.part, dp.data(), P);
-  UP(d.inst, di.data(), di.size());
-  UP(d.lrtb, lrtb.data(), 6 * P);
-  UP(d.lrtb_next, lnext.data(), 6 * P);
-  if (!pairs.empty()) UP(d.pairs, pairs.data(), pairs.size());
-  UP(c->inst_nodes, c->inst_nodes_h.data(), c->inst_nodes_h.size());
-#undef UP
-  HIPCHK(c, hipMemsetAsync(d.F, 0, 3 * Np * sizeof(double), st));
-  HIPCHK(c, hipMemsetAsync(d.pactive, 0, P * sizeof(uint32_t), st));
-  HIPCHK(c, hipMemsetAsync(d.counters, 0, 8 * sizeof(unsigned long long), st));
-  HIPCHK(c, hipMemsetAsync(d.err, 0, sizeof(uint32_t), st));
-  {
-    // fast-path domain of the bond arithmetic: coordinates and reference
-    // bond lengths (displacements are checked by the position kernel)
-    uint32_t fl = 0;
-    for (uint64_t g = 0; g < N && !fl; g++)
-      if (!(in_fast_range(X[g].x) && in_fast_range(X[g].y) && in_fast_range(X[g].z))) fl = kFlagSafeMath;
-    for (uint64_t g = 0; g < N && !fl; g++)
-      for (uint64_t k = rstart[g]; k < rstart[g + 1]; k++) {
-        const double4& a = X[g];
-        const double4& b = X[rows[k]];
-        double x0 = b.x - a.x, x1 = b.y - a.y, x2 = b.z - a.z;
-        double l2 = (x0 * x0 + x1 * x1) + x2 * x2;
-        if (!(l2 >= kFastLo && l2 <= kFastHi)) {
-          fl = kFlagSafeMath;
-          break;
-        }
-      }
-    c->safe_math_static = fl != 0;
-    HIPCHK(c, hipMemcpyAsync(d.flags, &fl, sizeof fl, hipMemcpyHostToDevice, st));
-  }
-  HIPCHK(c, hipMemsetAsync(d.reaction, 0, 18 * di.size() * sizeof(double), st));
-  HIPCHK(c, hipMemsetAsync(d.wmask, 0, Np, st));
-  HIPCHK(c, hipMemsetAsync(d.bflag, 0, Np / 256 + 1, st));
-  HIPCHK(c, hipStreamSynchronize(st));
-  // host copies no longer needed
-  for (auto& p : c->parts) {
-    p.cl0.resize(9 * p.clamped.size());
-    for (size_t q = 0; q < p.clamped.size(); q++)
-      for (int k = 0; k < 3; k++) {
-        p.cl0[9 * q + k] = p.disp[3 * p.clamped[q] + k];
-        p.cl0[9 * q + 3 + k] = p.vel[3 * p.clamped[q] + k];
-        p.cl0[9 * q + 6 + k] = p.acc[3 * p.clamped[q] + k];
-      }
-    std::vector<uint32_t>().swap(p.conn);
-    std::vector<double>().swap(p.disp);
-    std::vector<double>().swap(p.vel);
-    std::vector<double>().swap(p.acc);
-    std::vector<double>().swap(p.ext);
-  }
-  c->cnt.nodes = N;
-  c->cnt.particles = P;
+  c->pending.clear();
+  c->keep_d.clear();
+  c->keep_u.clear();
   c->committed = true;
   return PD_OK;
 }
 
-static bool any_contact_possible(pd_ctx* c) {
-  if (c->d.npairs) return true;
-  for (auto& I : c->insts)
-    if (I.run.self_contact && I.run.enable_fracture) return true;
-  return false;
-}
+}  // extern "C"
 
-static int launch_step(pd_ctx* c, bool want_reaction) {
+// One explicit step of every resident job (timeloop.h:395-797).  Contains no
+// memset and no host synchronisation unless profiling, so it can be captured.
+static int launch_step(pd_ctx* c) {
   Dev& d = c->d;
   hipStream_t st = c->stream;
-  const double dt = c->insts[0].run.dt;
   const uint64_t Np = d.S * kSlice;
   auto mark = [&](int k) -> int {
     if (c->prof) HIPCHK(c, hipEventRecord(c->pev[k], st));
@@ -474,41 +774,28 @@ static int launch_step(pd_ctx* c, bool want_reaction) {
   };
   int rc;
   if ((rc = mark(0))) return rc;
-  hipLaunchKernelGGL(k_position, dim3(grid_of(Np, 256)), dim3(256), 0, st, d, dt);
+  hipLaunchKernelGGL(k_position, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if ((rc = mark(1))) return rc;
   if (d.npairs) hipLaunchKernelGGL(k_pairs, dim3(grid_of(d.npairs, 256)), dim3(256), 0, st, d);
   if (any_contact_possible(c)) {
-    HIPCHK(c, hipMemsetAsync(d.cstart, 0, ((size_t)d.T + 1) * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_of((uint64_t)d.T + 1, 256)), dim3(256), 0, st, d.cstart,
+                       (uint64_t)d.T + 1, 0u);
     hipLaunchKernelGGL(k_cellkey, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
-    uint32_t nb = grid_of(d.T, kScanBlock);
+    const uint32_t nb = grid_of(d.T, kScanBlock);
     hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(kScanBlock), 0, st, d.cstart, d.T, d.scan_tmp);
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kScanBlock), 0, st, d.scan_tmp, nb, d.cstart + d.T);
     hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(kScanBlock), 0, st, d.cstart, d.T, d.scan_tmp, d.ccur);
     hipLaunchKernelGGL(k_scatter, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
   }
-  StepArgs a;
-  a.d = d;
-  a.dt = dt;
-  const pd_run_desc& r0 = c->insts[0].run;
-  double sc = (double)c->t / (double)r0.extforce_maxstep;
-  if (sc > 1) sc = 1;
-  a.ext_scale = sc;
-  a.t = c->t;
-  a.want_reaction = want_reaction ? 1u : 0u;
-  if (want_reaction)
-    hipLaunchKernelGGL(k_clear_bflag, dim3(grid_of(Np / 256 + 1, 256)), dim3(256), 0, st, d,
-                       (uint64_t)(Np / 256 + 1));
   if ((rc = mark(2))) return rc;
   // grid padded to a multiple of the XCD count (k_bond remaps blocks)
   hipLaunchKernelGGL(k_bond, dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
   if ((rc = mark(3))) return rc;
-  hipLaunchKernelGGL(k_contact, dim3(grid_of(Np, 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_contact, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if ((rc = mark(4))) return rc;
-  if (want_reaction)
-    hipLaunchKernelGGL(k_reaction, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d, dt, c->inst_nodes);
-  hipLaunchKernelGGL(k_update, dim3(grid_of(Np, 256)), dim3(256), 0, st, d, dt);
-  hipLaunchKernelGGL(k_finalize, dim3(grid_of(std::max<uint64_t>(d.P, d.ninst), 256)), dim3(256), 0,
-                     st, d, dt);
+  if (any_reaction(c)) hipLaunchKernelGGL(k_reaction, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
+  hipLaunchKernelGGL(k_update, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_finalize, dim3(grid_of(std::max<uint64_t>(d.P, d.ninst), 256)), dim3(256), 0, st, d);
   if ((rc = mark(5))) return rc;
   HIPCHK(c, hipGetLastError());
   if (c->prof) {
@@ -520,34 +807,80 @@ static int launch_step(pd_ctx* c, bool want_reaction) {
       c->acc.launches[k] += 1;
     }
   }
-  // host mirror of the wall planes (timeloop.h:784-797)
-  for (auto& I : c->insts)
-    for (int k = 0; k < 6; k++) I.wall[k] += I.speed[k] * dt;
-  for (size_t pi = 0; pi < c->parts.size(); pi++)
-    if (c->parts[pi].movable) c->cnt.bond_updates += c->movable_bonds[pi];
-  c->t++;
-  c->cnt.steps++;
-  c->steps_since_set++;
   return PD_OK;
 }
 
+static int drop_graph(pd_ctx* c) {
+  if (c->gexec) {
+    HIPCHK(c, hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  if (c->graph) {
+    HIPCHK(c, hipGraphDestroy(c->graph));
+    c->graph = nullptr;
+  }
+  return PD_OK;
+}
+
+// Capture kGraphSteps steps into a graph; captured again whenever the launch
+// sequence or its arguments change (jobs added or removed, switches).
+static int ensure_graph(pd_ctx* c) {
+  if (c->gexec && !c->graph_dirty) return PD_OK;
+  int rc = drop_graph(c);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int s = 0; s < kGraphSteps; s++) {
+    rc = launch_step(c);
+    if (rc) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+  }
+  HIPCHK(c, hipStreamEndCapture(c->stream, &c->graph));
+  HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  c->graph_dirty = false;
+  return PD_OK;
+}
+
+extern "C" {
+
 int pd_step(pd_ctx* c, uint32_t nsteps) {
   if (!c) return PD_E_ARG;
-  if (!c->committed) return fail(c, PD_E_STATE, "pd_step before pd_commit");
+  if (!c->reserved) return fail(c, PD_E_STATE, "pd_step before pd_commit / pd_reserve");
   if (nsteps == 0) return PD_OK;
-  int rc = check_supported(c);
-  if (rc) return rc;
-  bool want_reac = false;
-  for (auto& I : c->insts) want_reac |= I.run.compute_wall_reaction && I.run.allow_wall;
+  bool any = false;
+  for (auto& J : c->jobs) any |= J.used && J.remaining > 0;
+  if (!any) return PD_OK;
+  int rc;
+  // steps_left drives reaction_step(): the reaction is formed on the last
+  // step of this call (and on each job's own last step)
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  for (uint32_t s = 0; s < nsteps; s++)
-    if ((rc = launch_step(c, want_reac && s + 1 == nsteps))) return rc;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, &c->d.ctl->steps_left, (uint64_t)1, nsteps);
+  HIPCHK(c, hipGetLastError());
+  uint32_t done = 0;
+  if (c->use_graph && !c->prof && nsteps >= (uint32_t)kGraphSteps) {
+    if ((rc = ensure_graph(c))) return rc;
+    for (; done + kGraphSteps <= nsteps; done += kGraphSteps) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  }
+  for (; done < nsteps; done++)
+    if ((rc = launch_step(c))) return rc;
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
-  c->reaction_valid = want_reac;
+  for (auto& J : c->jobs) {
+    if (!J.used || !J.remaining) continue;
+    const uint64_t k = std::min<uint64_t>(J.remaining, nsteps);
+    if (J.remaining != kUnlimited) J.remaining -= k;
+    J.steps_done += k;
+    J.steps_since_set += k;
+    J.reaction_valid = J.run.compute_wall_reaction && J.run.allow_wall;
+    c->cnt.bond_updates += J.movable_bonds * k;
+  }
+  c->cnt.steps += nsteps;
   uint32_t err = 0;
   HIPCHK(c, hipMemcpy(&err, c->d.err, sizeof err, hipMemcpyDeviceToHost));
   if (err & kErrCandOverflow)
@@ -555,122 +888,212 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
   return PD_OK;
 }
 
-int pd_set_flag(pd_ctx* c, int flag, int value) {
+int pd_set_graphs(pd_ctx* c, int on) {
   if (!c) return PD_E_ARG;
-  for (size_t i = 0; i < c->insts.size(); i++) {
-    pd_run_desc& r = c->insts[i].run;
-    switch (flag) {
-      case PD_FLAG_ENABLE_FRACTURE: r.enable_fracture = value; break;
-      case PD_FLAG_SELF_CONTACT: r.self_contact = value; break;
-      case PD_FLAG_ALLOW_DAMPING: r.allow_damping = value; break;
-      case PD_FLAG_ALLOW_FRICTION: r.allow_friction = value; break;
-      case PD_FLAG_ENABLE_TORQUE: r.enable_torque = value; break;
-      case PD_FLAG_COMPUTE_WALL_REACTION: r.compute_wall_reaction = value; break;
-      default: return fail(c, PD_E_ARG, "unknown flag %d", flag);
-    }
-    if (c->committed) {
-      DevInst D;
-      fill_inst(c->insts[i], D);
-      // wall planes live on the device; keep them
-      HIPCHK(c, hipMemcpyAsync(&D.wall, &c->d.inst[i].wall, sizeof D.wall, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      HIPCHK(c, hipMemcpy(&c->d.inst[i], &D, sizeof D, hipMemcpyHostToDevice));
-    }
-  }
+  c->use_graph = on != 0;
   return PD_OK;
 }
 
-static int push_inst(pd_ctx* c, size_t i) {
-  if (!c->committed) return PD_OK;
-  DevInst D;
-  fill_inst(c->insts[i], D);
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(&c->d.inst[i], &D, sizeof D, hipMemcpyHostToDevice));
+int pd_job_remaining(pd_ctx* c, uint32_t job, uint64_t* remaining) {
+  if (!c || !remaining) return PD_E_ARG;
+  if (job >= c->jobs.size() || !c->jobs[job].used) return fail(c, PD_E_ARG, "no job %u", job);
+  *remaining = c->jobs[job].remaining;
   return PD_OK;
+}
+
+}  // extern "C"
+
+// read-modify-write of one job's device parameters
+template <typename F>
+static int edit_inst(pd_ctx* c, uint32_t job, F f) {
+  DevInst D;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(&D, c->d.inst + job, sizeof D, hipMemcpyDeviceToHost));
+  f(D);
+  HIPCHK(c, hipMemcpy(c->d.inst + job, &D, sizeof D, hipMemcpyHostToDevice));
+  return PD_OK;
+}
+
+static int locate(pd_ctx* c, uint32_t job, uint32_t p, HostPart** H) {
+  if (job >= c->jobs.size() || !c->jobs[job].used || p >= c->jobs[job].parts.size())
+    return fail(c, PD_E_ARG, "no particle %u in job %u", p, job);
+  *H = &c->jobs[job].parts[p];
+  return PD_OK;
+}
+
+// plain particle index -> (job, particle of the job)
+static int plain(pd_ctx* c, uint32_t pi, uint32_t* job, uint32_t* p) {
+  if (!c->committed) return fail(c, PD_E_STATE, "before pd_commit");
+  if (pi >= c->plain.size()) return fail(c, PD_E_ARG, "no particle %u", pi);
+  *job = c->plain[pi].first;
+  *p = c->plain[pi].second;
+  return PD_OK;
+}
+
+static void apply_flag(pd_run_desc& r, int flag, int value) {
+  switch (flag) {
+    case PD_FLAG_ENABLE_FRACTURE: r.enable_fracture = value; break;
+    case PD_FLAG_SELF_CONTACT: r.self_contact = value; break;
+    case PD_FLAG_ALLOW_DAMPING: r.allow_damping = value; break;
+    case PD_FLAG_ALLOW_FRICTION: r.allow_friction = value; break;
+    case PD_FLAG_ENABLE_TORQUE: r.enable_torque = value; break;
+    case PD_FLAG_COMPUTE_WALL_REACTION: r.compute_wall_reaction = value; break;
+  }
+}
+
+static int job_set_flag(pd_ctx* c, uint32_t j, int flag, int value) {
+  Job& J = c->jobs[j];
+  pd_run_desc r = J.run;
+  apply_flag(r, flag, value);
+  int rc = check_run(c, r);
+  if (rc) return rc;
+  J.run = r;
+  rc = edit_inst(c, j, [&](DevInst& D) {
+    D.enable_fracture = r.enable_fracture;
+    D.self_contact = r.self_contact;
+    D.allow_damping = r.allow_damping;
+    D.allow_friction = r.allow_friction;
+    D.compute_reaction = r.compute_wall_reaction;
+  });
+  c->graph_dirty = true;
+  return rc;
+}
+
+static int edit_part(pd_ctx* c, uint32_t job, uint32_t p, int which, int value, double snot, bool set_snot) {
+  HostPart* H;
+  int rc = locate(c, job, p, &H);
+  if (rc) return rc;
+  Job& J = c->jobs[job];
+  if (!set_snot) {
+    const bool was = H->movable != 0;
+    switch (which) {
+      case PD_PFLAG_MOVABLE: H->movable = value; break;
+      case PD_PFLAG_STOPPABLE: H->stoppable = value; break;
+      case PD_PFLAG_BREAKABLE: H->breakable = value; break;
+      default: return fail(c, PD_E_ARG, "unknown particle flag %d", which);
+    }
+    const bool now = H->movable != 0;
+    if (was && !now) J.movable_bonds -= H->nbonds;
+    if (!was && now) J.movable_bonds += H->nbonds;
+  } else {
+    H->snot = snot;
+  }
+  DevPart D;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(&D, c->d.part + H->slot, sizeof D, hipMemcpyDeviceToHost));
+  D.movable = H->movable;
+  D.stoppable = H->stoppable;
+  D.breakable = H->breakable;
+  D.snot = H->snot;
+  HIPCHK(c, hipMemcpy(c->d.part + H->slot, &D, sizeof D, hipMemcpyHostToDevice));
+  return PD_OK;
+}
+
+extern "C" {
+
+int pd_set_flag(pd_ctx* c, int flag, int value) {
+  if (!c) return PD_E_ARG;
+  if (flag < PD_FLAG_ENABLE_FRACTURE || flag > PD_FLAG_COMPUTE_WALL_REACTION)
+    return fail(c, PD_E_ARG, "unknown flag %d", flag);
+  for (uint32_t j = 0; j < c->jobs.size(); j++)
+    if (c->jobs[j].used) {
+      int rc = job_set_flag(c, j, flag, value);
+      if (rc) return rc;
+    }
+  apply_flag(c->run0, flag, value);
+  return PD_OK;
+}
+
+int pd_job_set_flag(pd_ctx* c, uint32_t job, int flag, int value) {
+  if (!c) return PD_E_ARG;
+  if (flag < PD_FLAG_ENABLE_FRACTURE || flag > PD_FLAG_COMPUTE_WALL_REACTION)
+    return fail(c, PD_E_ARG, "unknown flag %d", flag);
+  if (job >= c->jobs.size() || !c->jobs[job].used) return fail(c, PD_E_ARG, "no job %u", job);
+  return job_set_flag(c, job, flag, value);
 }
 
 int pd_set_wall_speed(pd_ctx* c, const double speed[6]) {
   if (!c || !speed) return PD_E_ARG;
-  for (size_t i = 0; i < c->insts.size(); i++) {
-    memcpy(c->insts[i].speed, speed, sizeof c->insts[i].speed);
-    int rc = push_inst(c, i);
-    if (rc) return rc;
-  }
+  for (uint32_t j = 0; j < c->jobs.size(); j++)
+    if (c->jobs[j].used) {
+      int rc = edit_inst(c, j, [&](DevInst& D) { memcpy(D.speed, speed, sizeof D.speed); });
+      if (rc) return rc;
+      memcpy(c->jobs[j].run.wall_speed, speed, 6 * sizeof(double));
+    }
+  memcpy(c->run0.wall_speed, speed, 6 * sizeof(double));
   return PD_OK;
 }
 
 int pd_set_wall(pd_ctx* c, const double lrtb[6]) {
   if (!c || !lrtb) return PD_E_ARG;
-  for (size_t i = 0; i < c->insts.size(); i++) {
-    memcpy(c->insts[i].wall, lrtb, sizeof c->insts[i].wall);
-    int rc = push_inst(c, i);
-    if (rc) return rc;
-  }
+  for (uint32_t j = 0; j < c->jobs.size(); j++)
+    if (c->jobs[j].used) {
+      int rc = edit_inst(c, j, [&](DevInst& D) { memcpy(D.wall, lrtb, sizeof D.wall); });
+      if (rc) return rc;
+    }
+  memcpy(c->run0.wall_lrtb, lrtb, 6 * sizeof(double));
   return PD_OK;
 }
 
-int pd_set_particle_flag(pd_ctx* c, uint32_t p, int which, int value) {
-  if (!c || p >= c->parts.size()) return PD_E_ARG;
-  auto& H = c->parts[p];
-  switch (which) {
-    case PD_PFLAG_MOVABLE: H.movable = value; break;
-    case PD_PFLAG_STOPPABLE: H.stoppable = value; break;
-    case PD_PFLAG_BREAKABLE: H.breakable = value; break;
-    default: return fail(c, PD_E_ARG, "unknown particle flag %d", which);
+int pd_set_particle_flag(pd_ctx* c, uint32_t pi, int which, int value) {
+  if (!c) return PD_E_ARG;
+  if (!c->committed) {
+    if (pi >= c->pending.size()) return fail(c, PD_E_ARG, "no particle %u", pi);
+    pd_particle_desc& q = c->pending[pi];
+    switch (which) {
+      case PD_PFLAG_MOVABLE: q.movable = value; break;
+      case PD_PFLAG_STOPPABLE: q.stoppable = value; break;
+      case PD_PFLAG_BREAKABLE: q.breakable = value; break;
+      default: return fail(c, PD_E_ARG, "unknown particle flag %d", which);
+    }
+    return PD_OK;
   }
-  if (c->committed) {
-    DevPart D;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(&D, &c->d.part[p], sizeof D, hipMemcpyDeviceToHost));
-    D.movable = H.movable;
-    D.stoppable = H.stoppable;
-    D.breakable = H.breakable;
-    HIPCHK(c, hipMemcpy(&c->d.part[p], &D, sizeof D, hipMemcpyHostToDevice));
-  }
-  return PD_OK;
+  uint32_t j, p;
+  int rc = plain(c, pi, &j, &p);
+  return rc ? rc : edit_part(c, j, p, which, value, 0, false);
 }
 
-int pd_set_snot(pd_ctx* c, uint32_t p, double snot) {
-  if (!c || p >= c->parts.size()) return PD_E_ARG;
-  c->parts[p].snot = snot;
-  if (c->committed) {
-    DevPart D;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(&D, &c->d.part[p], sizeof D, hipMemcpyDeviceToHost));
-    D.snot = snot;
-    HIPCHK(c, hipMemcpy(&c->d.part[p], &D, sizeof D, hipMemcpyHostToDevice));
+int pd_set_snot(pd_ctx* c, uint32_t pi, double snot) {
+  if (!c) return PD_E_ARG;
+  if (!c->committed) {
+    if (pi >= c->pending.size()) return fail(c, PD_E_ARG, "no particle %u", pi);
+    c->pending[pi].snot = snot;
+    return PD_OK;
   }
-  return PD_OK;
+  uint32_t j, p;
+  int rc = plain(c, pi, &j, &p);
+  return rc ? rc : edit_part(c, j, p, 0, 0, snot, true);
 }
 
-int pd_get_state(pd_ctx* c, uint32_t pi, double* disp, double* currpos, double* vel, double* acc,
-                 double* force) {
-  if (!c || pi >= c->parts.size()) return PD_E_ARG;
-  if (!c->committed) return fail(c, PD_E_STATE, "pd_get_state before pd_commit");
-  auto& p = c->parts[pi];
+int pd_job_get_state(pd_ctx* c, uint32_t job, uint32_t pi, double* disp, double* currpos, double* vel,
+                     double* acc, double* force) {
+  if (!c) return PD_E_ARG;
+  HostPart* Hp;
+  int rc = locate(c, job, pi, &Hp);
+  if (rc) return rc;
+  const HostPart& p = *Hp;
+  const Job& J = c->jobs[job];
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (disp || currpos) {
     std::vector<double4> U(p.n), X(p.n);
     HIPCHK(c, hipMemcpy(U.data(), c->d.U + p.off, p.n * sizeof(double4), hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(X.data(), c->d.X + p.off, p.n * sizeof(double4), hipMemcpyDeviceToHost));
     for (uint32_t i = 0; i < p.n; i++) {
-      double u[3] = {U[i].x, U[i].y, U[i].z};
-      double x[3] = {X[i].x, X[i].y, X[i].z};
+      const double u[3] = {U[i].x, U[i].y, U[i].z};
+      const double x[3] = {X[i].x, X[i].y, X[i].z};
       for (int k = 0; k < 3; k++) {
         if (disp) disp[3 * i + k] = u[k];
-        // CurrPos is pos+disp of the position half; clamped nodes keep it
-        // although their disp was reset (timeloop.h:539,767-774)
         if (currpos) currpos[3 * i + k] = x[k] + u[k];
       }
     }
-    if (currpos && c->steps_since_set == 1) {
+    if (currpos && J.steps_since_set == 1) {
       // CurrPos of a clamped node is pos + the disp of the position half,
       // formed before the clamp reset disp (timeloop.h:539,767-774).  From
       // the second step on that disp is 0 + (dt*0 + h*0); only the first step
       // after the state was set sees the initial disp/vel/acc.
-      const double dt = c->insts[p.inst].run.dt, h = (dt * dt) * 0.5;
+      const double dt = J.run.dt, h = (dt * dt) * 0.5;
       for (size_t q = 0; q < p.clamped.size(); q++) {
-        uint32_t i = p.clamped[q];
+        const uint32_t i = p.clamped[q];
         for (int k = 0; k < 3; k++) {
           double u = p.cl0[9 * q + k] + ((dt * p.cl0[9 * q + 3 + k]) + (h * p.cl0[9 * q + 6 + k]));
           currpos[3 * i + k] = p.pos[3 * i + k] + u;
@@ -683,17 +1106,26 @@ int pd_get_state(pd_ctx* c, uint32_t pi, double* disp, double* currpos, double* 
     HIPCHK(c, hipMemcpy(dst, src + 3 * p.off, 3ull * p.n * sizeof(double), hipMemcpyDeviceToHost));
     return PD_OK;
   };
-  int rc;
   if ((rc = down3(vel, c->d.V))) return rc;
   if ((rc = down3(acc, c->d.A))) return rc;
   if ((rc = down3(force, c->d.F))) return rc;
   return PD_OK;
 }
 
+int pd_get_state(pd_ctx* c, uint32_t pi, double* disp, double* currpos, double* vel, double* acc,
+                 double* force) {
+  if (!c) return PD_E_ARG;
+  uint32_t j, p;
+  int rc = plain(c, pi, &j, &p);
+  return rc ? rc : pd_job_get_state(c, j, p, disp, currpos, vel, acc, force);
+}
+
 int pd_set_state(pd_ctx* c, uint32_t pi, const double* disp, const double* vel, const double* acc) {
-  if (!c || pi >= c->parts.size()) return PD_E_ARG;
-  if (!c->committed) return fail(c, PD_E_STATE, "pd_set_state before pd_commit");
-  auto& p = c->parts[pi];
+  if (!c) return PD_E_ARG;
+  uint32_t job, lp;
+  int rc = plain(c, pi, &job, &lp);
+  if (rc) return rc;
+  HostPart& p = c->jobs[job].parts[lp];
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (disp) {
     std::vector<double4> U(p.n);
@@ -704,78 +1136,80 @@ int pd_set_state(pd_ctx* c, uint32_t pi, const double* disp, const double* vel, 
   if (acc) HIPCHK(c, hipMemcpy(c->d.A + 3 * p.off, acc, 3ull * p.n * sizeof(double), hipMemcpyHostToDevice));
   for (size_t q = 0; q < p.clamped.size(); q++)
     for (int k = 0; k < 3; k++) {
-      uint32_t i = p.clamped[q];
+      const uint32_t i = p.clamped[q];
       if (disp) p.cl0[9 * q + k] = disp[3 * i + k];
       if (vel) p.cl0[9 * q + 3 + k] = vel[3 * i + k];
       if (acc) p.cl0[9 * q + 6 + k] = acc[3 * i + k];
     }
-  c->steps_since_set = 0;
+  c->jobs[job].steps_since_set = 0;
   return PD_OK;
 }
 
-// Download the bond rows of particle p (local indices, broken bit kept).
-static int particle_rows(pd_ctx* c, uint32_t pi, std::vector<uint32_t>& deg,
-                         std::vector<std::vector<uint32_t>>& rows) {
-  auto& p = c->parts[pi];
-  uint64_t s0 = p.off / kSlice, s1 = (p.off + p.n + kSlice - 1) / kSlice;
-  std::vector<uint64_t> soff(s1 - s0 + 1);
+int pd_job_get_connectivity(pd_ctx* c, uint32_t job, uint32_t pi, uint32_t* pairs, uint64_t cap, uint64_t* m) {
+  if (!c || !m) return PD_E_ARG;
+  HostPart* Hp;
+  int rc = locate(c, job, pi, &Hp);
+  if (rc) return rc;
+  const HostPart& p = *Hp;
+  // the particle's slices and their bond entries
+  const uint64_t s0 = p.off / kSlice, s1 = (p.off + p.n + kSlice - 1) / kSlice;
+  std::vector<uint64_t> soff(s1 - s0);
+  std::vector<uint32_t> swid(s1 - s0), deg(p.n);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(soff.data(), c->d.soff + s0, soff.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
-  std::vector<uint32_t> ent(soff.back() - soff.front());
-  if (!ent.empty())
-    HIPCHK(c, hipMemcpy(ent.data(), c->d.bond + soff.front(), ent.size() * sizeof(uint32_t),
-                        hipMemcpyDeviceToHost));
-  deg.resize(p.n);
+  HIPCHK(c, hipMemcpy(swid.data(), c->d.swid + s0, swid.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(deg.data(), c->d.deg + p.off, p.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  rows.assign(p.n, {});
+  const uint64_t lo = soff.front(), hi = soff.back() + (uint64_t)swid.back() * kSlice;
+  std::vector<uint32_t> ent(hi - lo);
+  if (!ent.empty())
+    HIPCHK(c, hipMemcpy(ent.data(), c->d.bond + lo, ent.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  // NbdArr2conn (timeloop.h:20-44): pairs (i, q) with i < q, in row order
+  std::vector<uint32_t> out;
   for (uint32_t i = 0; i < p.n; i++) {
-    uint64_t g = p.off + i, s = g / kSlice, l = g % kSlice;
-    uint64_t base = soff[s - s0] - soff.front();
-    rows[i].resize(deg[i]);
+    const uint64_t g = p.off + i, s = g / kSlice, l = g % kSlice;
     for (uint32_t k = 0; k < deg[i]; k++) {
-      uint32_t e = ent[base + (uint64_t)k * kSlice + l];
-      rows[i][k] = ((e & kIdxMask) - (uint32_t)p.off) | (e & kBroken);
+      const uint32_t e = ent[soff[s - s0] - lo + (uint64_t)k * kSlice + l];
+      if (e & kBroken) continue;
+      const uint32_t q = e - (uint32_t)p.off;
+      if (i < q) {
+        out.push_back(i);
+        out.push_back(q);
+      }
     }
   }
+  *m = out.size() / 2;
+  if (!pairs) return PD_OK;
+  if (cap < *m)
+    return fail(c, PD_E_ARG, "connectivity buffer holds %llu pairs, %llu needed", (unsigned long long)cap,
+                (unsigned long long)*m);
+  if (!out.empty()) memcpy(pairs, out.data(), out.size() * sizeof(uint32_t));
   return PD_OK;
 }
 
 int pd_get_connectivity(pd_ctx* c, uint32_t pi, uint32_t* pairs, uint64_t cap, uint64_t* m) {
-  if (!c || pi >= c->parts.size() || !m) return PD_E_ARG;
-  if (!c->committed) return fail(c, PD_E_STATE, "pd_get_connectivity before pd_commit");
-  std::vector<uint32_t> deg;
-  std::vector<std::vector<uint32_t>> rows;
-  int rc = particle_rows(c, pi, deg, rows);
-  if (rc) return rc;
-  uint64_t n = 0;
-  for (uint32_t i = 0; i < rows.size(); i++)
-    for (uint32_t e : rows[i])
-      if (!(e & kBroken) && i < e) n++;
-  *m = n;
-  if (!pairs) return PD_OK;
-  if (cap < n) return fail(c, PD_E_ARG, "connectivity buffer holds %llu pairs, %llu needed",
-                           (unsigned long long)cap, (unsigned long long)n);
-  uint64_t w = 0;
-  for (uint32_t i = 0; i < rows.size(); i++)
-    for (uint32_t e : rows[i])
-      if (!(e & kBroken) && i < e) {
-        pairs[2 * w] = i;
-        pairs[2 * w + 1] = e;
-        w++;
-      }
-  return PD_OK;
+  if (!c) return PD_E_ARG;
+  uint32_t j, p;
+  int rc = plain(c, pi, &j, &p);
+  return rc ? rc : pd_job_get_connectivity(c, j, p, pairs, cap, m);
 }
 
-int pd_get_wall(pd_ctx* c, uint32_t instance, double lrtb[6], double reaction[18]) {
-  if (!c || instance >= c->insts.size()) return PD_E_ARG;
-  if (lrtb) memcpy(lrtb, c->insts[instance].wall, 6 * sizeof(double));
+int pd_get_wall(pd_ctx* c, uint32_t job, double lrtb[6], double reaction[18]) {
+  if (!c) return PD_E_ARG;
+  if (!c->committed && !c->reserved) {
+    if (lrtb) memcpy(lrtb, c->run0.wall_lrtb, 6 * sizeof(double));
+    if (reaction) memset(reaction, 0, 18 * sizeof(double));
+    return PD_OK;
+  }
+  if (job >= c->jobs.size() || !c->jobs[job].used) return fail(c, PD_E_ARG, "no job %u", job);
+  DevInst D;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(&D, c->d.inst + job, sizeof D, hipMemcpyDeviceToHost));
+  if (lrtb) memcpy(lrtb, D.wall, 6 * sizeof(double));
   if (reaction) {
-    if (c->committed && c->reaction_valid) {
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      HIPCHK(c, hipMemcpy(reaction, c->d.reaction + 18 * instance, 18 * sizeof(double), hipMemcpyDeviceToHost));
-    } else {
-      for (int k = 0; k < 18; k++) reaction[k] = 0.0;
-    }
+    if (c->jobs[job].reaction_valid)
+      HIPCHK(c, hipMemcpy(reaction, c->d.reaction + 18 * job, 18 * sizeof(double), hipMemcpyDeviceToHost));
+    else
+      memset(reaction, 0, 18 * sizeof(double));
   }
   return PD_OK;
 }
@@ -783,17 +1217,18 @@ int pd_get_wall(pd_ctx* c, uint32_t instance, double lrtb[6], double reaction[18
 int pd_get_counters(pd_ctx* c, pd_counters* out) {
   if (!c || !out) return PD_E_ARG;
   *out = c->cnt;
-  if (c->committed) {
+  if (c->reserved) {
     unsigned long long pw = 0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(&pw, c->d.counters, sizeof pw, hipMemcpyDeviceToHost));
     out->pair_computations = pw;
     unsigned long long* dcount = c->d.counters + 1;
     HIPCHK(c, hipMemsetAsync(dcount, 0, sizeof(unsigned long long), c->stream));
-    if (c->nbond_entries)
-      hipLaunchKernelGGL(k_count_alive, dim3(std::min<uint64_t>(grid_of(c->nbond_entries, 256), 4096)),
-                         dim3(256), 0, c->stream,
-                         c->d, c->nbond_entries, dcount);
+    if (c->cap_entries) {
+      hipLaunchKernelGGL(k_count_alive, dim3(std::min<uint64_t>(grid_of(c->cap_entries, 256), 4096)), dim3(256), 0,
+                         c->stream, c->d, c->cap_entries, dcount);
+      HIPCHK(c, hipGetLastError());
+    }
     unsigned long long alive = 0;
     HIPCHK(c, hipMemcpyAsync(&alive, dcount, sizeof alive, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -831,7 +1266,7 @@ int pd_selftest_math(uint64_t n, uint64_t seed, uint64_t* mismatch_div, uint64_t
         hipMemcpy(hm, dm, sizeof hm, hipMemcpyDeviceToHost) != hipSuccess)
       rc = PD_E_HIP;
   }
-  hipFree(dm);
+  (void)hipFree(dm);
   *mismatch_div = hm[0];
   *mismatch_sqrt = hm[1];
   return rc;
@@ -840,7 +1275,7 @@ int pd_selftest_math(uint64_t n, uint64_t seed, uint64_t* mismatch_div, uint64_t
 int pd_math_mode(pd_ctx* c, int* safe) {
   if (!c || !safe) return PD_E_ARG;
   uint32_t fl = 0;
-  if (c->committed) {
+  if (c->reserved) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(&fl, c->d.flags, sizeof fl, hipMemcpyDeviceToHost));
   }
@@ -858,13 +1293,14 @@ const char* pd_last_error(pd_ctx* c) { return c ? c->err.c_str() : "null context
 
 void pd_destroy(pd_ctx* c) {
   if (!c) return;
-  if (c->stream) hipStreamSynchronize(c->stream);
-  for (void* p : c->allocs) hipFree(p);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)drop_graph(c);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (int k = 0; k <= PD_NPHASE; k++)
-    if (c->pev[k]) hipEventDestroy(c->pev[k]);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (c->pev[k]) (void)hipEventDestroy(c->pev[k]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 

@@ -29,17 +29,29 @@ struct DevPart {
   int32_t movable, breakable, stoppable, rows_sorted;
 };
 
-// Per-instance (independent system) parameters: Contact + RectWall
-// (contact.h:104-183, 185-376) and the run switches of run_timeloop.
+// Per-job (independent system) parameters: Contact + RectWall
+// (contact.h:104-183, 185-376), the run switches of run_timeloop, and the
+// job's own clock.  A job with remaining == 0 is frozen (finished, or a free
+// slot): every kernel leaves its nodes untouched.
 struct DevInst {
   double Rc, Kn, mu, zeta;
   double cell;              // contact cell edge (>= Rc)
+  double dt;
   double wall[6];           // x_min, y_min, z_min, x_max, y_max, z_max
   double speed[6];
+  uint64_t remaining;       // steps still to run
+  uint64_t node0, node1;    // node range [node0, node1)
+  uint32_t t;               // reference step index of the next step (timeloop.h:395)
   int32_t allow_wall, allow_damping, allow_friction, self_contact;
-  int32_t enable_fracture, gradient_extforce;
+  int32_t enable_fracture, gradient_extforce, compute_reaction;
   uint32_t extforce_maxstep;
   uint32_t pad0;
+};
+
+// Per-launch control words shared by the kernels of a step.
+struct DevCtl {
+  uint32_t steps_left;      // steps left in the current pd_step call
+  uint32_t pad;
 };
 
 struct Dev {
@@ -53,8 +65,10 @@ struct Dev {
   uint32_t* pid;     // particle of node
   uint8_t* nflag;    // bit0 clamped
   uint32_t* deg;     // row length at commit
-  // bonds: sliced rows, 64 nodes per slice, entry k of lane l at soff[s] + k*64 + l
-  uint64_t* soff;    // [S+1]
+  // bonds: sliced rows, 64 nodes per slice, entry k of lane l at
+  // soff[s] + k*64 + l for k < swid[s]
+  uint64_t* soff;    // [S]
+  uint32_t* swid;    // [S]
   uint32_t* bond;
   // particles / instances
   DevPart* part;
@@ -66,6 +80,7 @@ struct Dev {
   unsigned long long* counters;   // [0] pair computations
   uint32_t* err;
   uint32_t* flags;   // [1] kFlagSafeMath (sticky)
+  DevCtl* ctl;
   // wall reaction (per instance, 6 planes x 3) and helpers
   double* reaction;
   uint8_t* wmask;    // [N] planes in contact this step (reaction steps only)
@@ -76,7 +91,7 @@ struct Dev {
   uint32_t* ccur;    // [T]
   uint32_t* csorted; // [N]
   uint32_t* scan_tmp;
-  uint64_t N, S, P, npairs, ninst;
+  uint64_t N, S, P, npairs, ninst;   // N = S*64 node slots; P, ninst incl. the void slot
   uint32_t T;        // hash table size (power of two)
   uint32_t pad;
 };

@@ -14,27 +14,31 @@
 
 namespace pd {
 
-struct StepArgs {
-  Dev d;
-  double dt;
-  double ext_scale;      // extforce ramp min(t/extforce_maxstep, 1) (timeloop.h:582-587)
-  uint32_t t;            // reference step index of this step
-  uint32_t want_reaction;
-};
+__device__ __forceinline__ bool live(const DevInst& I) { return I.remaining != 0; }
+
+// Does job I accumulate its wall reaction in this step?  On its last step and
+// on the last step of a pd_step call, so the value read back afterwards is the
+// reaction of the job's latest step, as Wall.reaction_* is (timeloop.h:397).
+__device__ __forceinline__ bool reaction_step(const Dev& d, const DevInst& I) {
+  return I.compute_reaction && I.allow_wall && (d.ctl->steps_left == 1 || I.remaining == 1);
+}
 
 // ---------------------------------------------------------------------------
 // Verlet position half (timeloop.h:537-540; overloads.h:51-70,149-159):
 //   disp += dt*vel + (dt*dt*0.5)*acc ; CurrPos = pos + disp
 // plus the min/max of CurrPos that becomes the particle's lagged box at the
 // end of the step (timeloop.h:779).
-__global__ __launch_bounds__(256) void k_position(Dev d, double dt) {
+__global__ __launch_bounds__(256) void k_position(Dev d) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ unsigned long long smin[3][256], smax[3][256];
   __shared__ uint32_t sp[256];
+  // the block's wall-contact flag of this step starts clear
+  if (threadIdx.x == 0 && g < d.N) d.bflag[g >> 8] = 0;
   double cp[3] = {0, 0, 0};
   uint32_t p = 0xFFFFFFFFu;
-  if (g < d.N) {
+  if (g < d.N && live(d.inst[d.part[d.pid[g]].inst])) {
     p = d.pid[g];
+    const double dt = d.inst[d.part[p].inst].dt;
     double4 u = d.U[g];
     const double4 x = d.X[g];
     const double h = (dt * dt) * 0.5;
@@ -96,8 +100,8 @@ __global__ __launch_bounds__(256) void k_pairs(Dev d) {
   uint32_t i = d.pairs[2 * q], j = d.pairs[2 * q + 1];
   const DevPart& Pi = d.part[i];
   const DevPart& Pj = d.part[j];
-  if (!(Pi.movable || Pj.movable)) return;
   const DevInst& I = d.inst[Pi.inst];
+  if (!live(I) || !(Pi.movable || Pj.movable)) return;
   if (!intersects_box_fext(d.lrtb + 6 * i, d.lrtb + 6 * j, I.Rc)) return;
   atomicAdd(&d.counters[0], 1ull);
   // bit 1: the particle's nodes are contact candidates this step;
@@ -124,7 +128,7 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
   // a node is a candidate for others when its particle has an active pair
   // (pactive marks only stoppable ones, so test the pair flag bit separately)
   uint32_t key = kNoKey;
-  if (d.pactive[p] & 2u || self_active(P, I)) {
+  if (live(I) && (d.pactive[p] & 2u || self_active(P, I))) {
     double4 x = d.X[g];
     double4 u = d.U[g];
     double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
@@ -257,18 +261,20 @@ __global__ __launch_bounds__(256) void k_bond(Dev d) {
   // padded node range: S slices of 64; whole wavefronts exit together
   if (g >= d.S * kSlice) return;
   const DevPart& P = d.part[d.pid[g]];
+  const DevInst& I = d.inst[P.inst];
+  if (!live(I)) return;            // frozen job or free slot: F keeps its value
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   if (P.movable) {
-    const bool can_break = d.inst[P.inst].enable_fracture && P.breakable;
+    const bool can_break = I.enable_fracture && P.breakable;
     const double cnot = P.cnot, snot = P.snot;
     const bool fast = (*d.flags & kFlagSafeMath) == 0;
     // one wavefront = one slice: offset and width are wave-uniform
     const uint64_t s = g >> 6;
-    const uint64_t b0 = d.soff[s], b1 = d.soff[s + 1];
+    const uint64_t b0 = d.soff[s];
     const uint32_t blo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b0);
     const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b0 >> 32));
     const uint64_t base = ((uint64_t)bhi << 32) | (uint64_t)blo;
-    const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((b1 - b0) >> 6));
+    const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)d.swid[s]);
     uint32_t* bp = d.bond + base + (g & 63);
     const double4 xi = d.X[g];
     const double4 ui = d.U[g];
@@ -342,14 +348,14 @@ __global__ __launch_bounds__(256) void k_bond(Dev d) {
 // Contact, wall and external force, added to the PD sum in the reference's
 // order (timeloop.h:544-651): self-contact (q ascending), wall planes,
 // external force, then pair contact in pair order.
-__global__ __launch_bounds__(256) void k_contact(StepArgs a) {
-  const Dev& d = a.d;
+__global__ __launch_bounds__(256) void k_contact(Dev d) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
-  const double dt = a.dt;
   const uint32_t p = d.pid[g];
   const DevPart P = d.part[p];
   const DevInst& I = d.inst[P.inst];
+  if (!live(I)) return;
+  const double dt = I.dt;
   const double4 xi = d.X[g];
   const double4 ui = d.U[g];
   double f[3] = {d.F[3 * g], d.F[3 * g + 1], d.F[3 * g + 2]};
@@ -449,7 +455,7 @@ __global__ __launch_bounds__(256) void k_contact(StepArgs a) {
   if (P.movable && P.stoppable && I.allow_wall &&
       !within_interior_fext(d.lrtb + 6 * p, I.wall, I.Rc)) {
     uint32_t m = wall_node(I, dt, cpi, vi, xi.w, P.rho, f, nullptr);
-    if (a.want_reaction) {
+    if (reaction_step(d, I)) {
       d.wmask[g] = (uint8_t)m;
       if (m) d.bflag[g >> 8] = 1;
     }
@@ -458,7 +464,9 @@ __global__ __launch_bounds__(256) void k_contact(StepArgs a) {
   // (d) external force density (timeloop.h:581-597)
   if (P.movable) {
     if (I.gradient_extforce) {
-      double sc = a.ext_scale;
+      // ramp min(t / extforce_maxstep, 1) (timeloop.h:582-587)
+      double sc = (double)I.t / (double)I.extforce_maxstep;
+      if (sc > 1) sc = 1;
       f[0] += sc * d.E[3 * g];
       f[1] += sc * d.E[3 * g + 1];
       f[2] += sc * d.E[3 * g + 2];
@@ -503,12 +511,13 @@ __global__ __launch_bounds__(256) void k_contact(StepArgs a) {
 
 // ---------------------------------------------------------------------------
 // Verlet velocity half (timeloop.h:662-682) and clamped nodes (:767-774).
-__global__ __launch_bounds__(256) void k_update(Dev d, double dt) {
+__global__ __launch_bounds__(256) void k_update(Dev d) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
   const DevPart& P = d.part[d.pid[g]];
-  if (!P.movable) return;
-  const double hdt = 0.5 * dt;
+  const DevInst& I = d.inst[P.inst];
+  if (!live(I) || !P.movable) return;
+  const double hdt = 0.5 * I.dt;
   if (d.nflag[g] & 1u) {
     d.U[g] = make_double4(0.0, 0.0, 0.0, 0.0);
     for (int k = 0; k < 3; k++) d.V[3 * g + k] = d.A[3 * g + k] = 0.0;
@@ -524,10 +533,10 @@ __global__ __launch_bounds__(256) void k_update(Dev d, double dt) {
 
 // Lagged boxes of movable particles (timeloop.h:779), reset of the per-step
 // flags, wall motion (timeloop.h:784-797).
-__global__ __launch_bounds__(256) void k_finalize(Dev d, double dt) {
+__global__ __launch_bounds__(256) void k_finalize(Dev d) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < d.P) {
-    if (d.part[q].movable)
+    if (live(d.inst[d.part[q].inst]) && d.part[q].movable)
       for (int k = 0; k < 6; k++) d.lrtb[6 * q + k] = okey_inv(d.lrtb_next[6 * q + k]);
     for (int k = 0; k < 3; k++) {
       d.lrtb_next[6 * q + k] = ~0ull;
@@ -537,21 +546,27 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d, double dt) {
   }
   if (q < d.ninst) {
     DevInst& I = d.inst[q];
-    for (int k = 0; k < 6; k++) I.wall[k] += I.speed[k] * dt;
+    if (live(I)) {
+      for (int k = 0; k < 6; k++) I.wall[k] += I.speed[k] * I.dt;
+      I.t += 1;
+      I.remaining -= 1;
+    }
   }
+  if (q == 0 && d.ctl->steps_left) d.ctl->steps_left -= 1;
 }
 
 // Wall reaction of one step in the reference order (particle, node, plane,
 // term), recomputed from the same device function as the force.  One thread
 // per instance walks its node range; blocks without a wall contact are
 // skipped through bflag.
-__global__ void k_reaction(Dev d, double dt, const uint64_t* inst_nodes /* [ninst+1] */) {
+__global__ void k_reaction(Dev d) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.ninst) return;
-  DevInst I = d.inst[q];
+  const DevInst I = d.inst[q];
+  if (!live(I) || !reaction_step(d, I)) return;
   double reac[18];
   for (int k = 0; k < 18; k++) reac[k] = 0.0;
-  uint64_t g0 = inst_nodes[q], g1 = inst_nodes[q + 1];
+  const uint64_t g0 = I.node0, g1 = I.node1;
   for (uint64_t g = g0; g < g1;) {
     uint64_t blk = g >> 8;
     if (!d.bflag[blk]) {
@@ -567,17 +582,19 @@ __global__ void k_reaction(Dev d, double dt, const uint64_t* inst_nodes /* [nins
       double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
       double v[3] = {d.V[3 * g], d.V[3 * g + 1], d.V[3 * g + 2]};
       double fdummy[3] = {0, 0, 0};
-      wall_node(I, dt, cp, v, x.w, P.rho, fdummy, reac);
+      wall_node(I, I.dt, cp, v, x.w, P.rho, fdummy, reac);
       d.wmask[g] = 0;
     }
   }
   for (int k = 0; k < 18; k++) d.reaction[18 * q + k] = reac[k];
 }
 
-__global__ void k_clear_bflag(Dev d, uint64_t n) {
-  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) d.bflag[q] = 0;
+// fill (graph-safe replacement of a memset)
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
+
 
 // alive directed bonds (for counters and read-back)
 __global__ __launch_bounds__(256) void k_count_alive(Dev d, uint64_t nent, unsigned long long* out) {

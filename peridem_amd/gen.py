@@ -339,3 +339,41 @@ def crushing_aggregate(nparticles=512, nodes_range=(2000, 12000), seed=3, E=72e9
               wall_speed=(0, 0, 0, 0, 0, top_speed), name="crushing_aggregate")
     kw.update(sys_kw)
     return System(particles=parts, **kw)
+
+
+def impact_pair(nA, nB, v, E, seed=1, h=1e-4, nu=0.25, rho=2440.0, Gc=135.0, gap_h=1.1,
+                **sys_kw):
+    """Two free spheres of about nA and nB lattice nodes (spacing h); A moves at
+    v m/s along +x toward B.  Material constants, contact stiffness and s0
+    follow from E (the per-instance parameters of a batch)."""
+    rng = np.random.default_rng(seed)
+    rA = (3.0 * nA / (4.0 * math.pi)) ** (1.0 / 3.0)
+    rB = (3.0 * nB / (4.0 * math.pi)) ** (1.0 / 3.0)
+    LA, LB = lattice_sphere(rA), lattice_sphere(rB)
+    A = make_particle(LA, h, np.array([0.0, 0.0, 0.0]), rng, E, nu, rho, Gc)
+    B = make_particle(LB, h, np.array([(rA + rB + gap_h) * h, 0.0, 0.0]), rng, E, nu, rho, Gc)
+    A.vel = np.zeros((A.n, 3))
+    A.vel[:, 0] = v
+    Rc = 0.9 * h
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(E, nu, Rc),
+              friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=1,
+              name="impact_pair")
+    kw.update(sys_kw)
+    s = System(particles=[A, B], **kw)
+    s.meta.update(nA=A.n, nB=B.n, v=v, E=E, seed=seed)
+    return s
+
+
+def impact_batch(n=256, seed=4, nodes_range=(500, 20000), speed_range=(5.0, 50.0),
+                 E_range=(20e9, 100e9), **kw):
+    """n independent two-sphere impacts: node counts per particle log-uniform
+    on nodes_range, impact speed and Young's modulus uniform per instance."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(nodes_range[0]), np.log(nodes_range[1])
+    out = []
+    for k in range(n):
+        nA, nB = np.exp(rng.uniform(lo, hi, size=2))
+        v = float(rng.uniform(*speed_range))
+        E = float(rng.uniform(*E_range))
+        out.append(impact_pair(float(nA), float(nB), v, E, seed=int(seed * 100003 + k), **kw))
+    return out

@@ -38,7 +38,8 @@ def test_struct_layout_matches_header(tmp_path):
     offset and the struct sizes with what the C compiler lays out."""
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "peridem_gpu.h"', "int main(){"]
     for cname, cls in (("pd_run_desc", abi.RunDesc), ("pd_particle_desc", abi.ParticleDesc),
-                       ("pd_counters", abi.Counters)):
+                       ("pd_counters", abi.Counters), ("pd_pool_desc", abi.PoolDesc),
+                       ("pd_profile", abi.Profile)):
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for f, _ in cls._fields_:
             lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
@@ -51,7 +52,8 @@ def test_struct_layout_matches_header(tmp_path):
     for line in filter(None, got):
         cname, f, v = line.split()
         cls = {"pd_run_desc": abi.RunDesc, "pd_particle_desc": abi.ParticleDesc,
-               "pd_counters": abi.Counters}[cname]
+               "pd_counters": abi.Counters, "pd_pool_desc": abi.PoolDesc,
+               "pd_profile": abi.Profile}[cname]
         want = C.sizeof(cls) if f == "size" else getattr(cls, f).offset
         assert int(v) == want, (cname, f, v, want)
 
