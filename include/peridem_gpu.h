@@ -202,6 +202,22 @@ int pd_job_set_flag(pd_ctx* ctx, uint32_t job, int flag, int value);
  * step is launched kernel by kernel.  Results are the same either way. */
 int pd_set_graphs(pd_ctx* ctx, int on);
 
+/* Bond-kernel shapes and their autotuner.  No counterpart in the reference.
+ * The library carries several instantiations of the bond kernel (entries
+ * per batch, software pipelining of the index and neighbour loads, loads
+ * switched off for dead entries); each
+ * sums every row in row order, so all give the same bits.  By default the
+ * first pd_step times every shape on the resident state (scratch copies of
+ * the force and bond arrays, so nothing changes) and keeps the fastest.
+ * pd_autotune re-runs that with reps timed launches per shape, returning the
+ * choice and (when ms != NULL) the mean time of each shape [count];
+ * pd_set_bond_shape forces one (-1: tune at the next step). */
+int pd_bond_shape_count(void);
+int pd_bond_shape_info(int shape, int* unroll, int* pipelined, int* masked);
+int pd_set_bond_shape(pd_ctx* ctx, int shape);
+int pd_get_bond_shape(pd_ctx* ctx, int* shape);
+int pd_autotune(pd_ctx* ctx, uint32_t reps, int* best, double* ms);
+
 /* Per-phase device time, measured with HIP events on the context's stream
  * while profiling is on (each step then ends with a host wait, so profiled
  * runs are for kernel durations, not for throughput).  No counterpart in

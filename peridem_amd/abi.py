@@ -148,7 +148,13 @@ def load_gpu_lib(path=GPU_LIB):
                 ("pd_job_get_state", C.c_int, [P, _u32, _u32, _pd, _pd, _pd, _pd, _pd]),
                 ("pd_job_get_connectivity", C.c_int, [P, _u32, _u32, _pu, _u64, C.POINTER(_u64)]),
                 ("pd_job_set_flag", C.c_int, [P, _u32, C.c_int, C.c_int]),
-                ("pd_set_graphs", C.c_int, [P, C.c_int])):
+                ("pd_set_graphs", C.c_int, [P, C.c_int]),
+                ("pd_bond_shape_count", C.c_int, []),
+                ("pd_bond_shape_info", C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                 C.POINTER(C.c_int)]),
+                ("pd_set_bond_shape", C.c_int, [P, C.c_int]),
+                ("pd_get_bond_shape", C.c_int, [P, C.POINTER(C.c_int)]),
+                ("pd_autotune", C.c_int, [P, _u32, C.POINTER(C.c_int), _pd])):
             f = getattr(lib, name)
             f.restype, f.argtypes = res, args
         _LIBS[path] = (lib, _bind(lib, "pd_"))
@@ -246,6 +252,23 @@ class Engine:
 
     def set_graphs(self, on=True):
         self._check(self.lib.pd_set_graphs(self.ctx, int(on)), "set_graphs")
+
+    def set_bond_shape(self, shape):
+        """Force a bond-kernel shape (see bond_shapes()); -1 = autotune."""
+        self._check(self.lib.pd_set_bond_shape(self.ctx, int(shape)), "set_bond_shape")
+
+    def bond_shape(self):
+        v = C.c_int()
+        self._check(self.lib.pd_get_bond_shape(self.ctx, C.byref(v)), "get_bond_shape")
+        return v.value
+
+    def autotune(self, reps=3):
+        """Time every bond-kernel shape on the resident state; returns (best, ms per shape)."""
+        n = self.lib.pd_bond_shape_count()
+        ms = np.zeros(n)
+        best = C.c_int()
+        self._check(self.lib.pd_autotune(self.ctx, reps, C.byref(best), _ptr(ms)), "autotune")
+        return best.value, ms
 
     def math_mode_safe(self):
         v = C.c_int()
@@ -369,6 +392,23 @@ class Pool:
     def set_graphs(self, on=True):
         self._check(self.lib.pd_set_graphs(self.ctx, int(on)), "set_graphs")
 
+    def set_bond_shape(self, shape):
+        """Force a bond-kernel shape (see bond_shapes()); -1 = autotune."""
+        self._check(self.lib.pd_set_bond_shape(self.ctx, int(shape)), "set_bond_shape")
+
+    def bond_shape(self):
+        v = C.c_int()
+        self._check(self.lib.pd_get_bond_shape(self.ctx, C.byref(v)), "get_bond_shape")
+        return v.value
+
+    def autotune(self, reps=3):
+        """Time every bond-kernel shape on the resident state; returns (best, ms per shape)."""
+        n = self.lib.pd_bond_shape_count()
+        ms = np.zeros(n)
+        best = C.c_int()
+        self._check(self.lib.pd_autotune(self.ctx, reps, C.byref(best), _ptr(ms)), "autotune")
+        return best.value, ms
+
     def counters(self):
         c = Counters()
         self._check(self.f["get_counters"](self.ctx, C.byref(c)), "get_counters")
@@ -389,6 +429,17 @@ class Pool:
             self.close()
         except Exception:
             pass
+
+
+def bond_shapes():
+    """[(unroll, pipelined, masked)] of the bond-kernel shapes the library was built with."""
+    lib, _ = load_gpu_lib()
+    out = []
+    for k in range(lib.pd_bond_shape_count()):
+        u, p, m = C.c_int(), C.c_int(), C.c_int()
+        lib.pd_bond_shape_info(k, C.byref(u), C.byref(p), C.byref(m))
+        out.append((u.value, bool(p.value), bool(m.value)))
+    return out
 
 
 def selftest_math(n, seed=1):

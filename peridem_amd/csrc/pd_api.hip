@@ -104,11 +104,13 @@ struct Ranges {
 // rows of one job in conn2NArr order, node indices relative to the job
 struct JobRows {
   uint64_t nodes = 0, nslices = 0, entries = 0;
-  std::vector<uint32_t> deg;
+  std::vector<uint32_t> deg;     // row length
   std::vector<uint64_t> rstart;
   std::vector<uint32_t> rows;
-  std::vector<uint32_t> width;  // per slice
-  std::vector<uint64_t> poff;   // per particle: first node (job-relative)
+  std::vector<uint32_t> slot;    // per row entry: its position k in the node's sliced row
+  std::vector<uint32_t> sdeg;    // per node: sliced row length (last slot + 1)
+  std::vector<uint32_t> width;   // per slice
+  std::vector<uint64_t> poff;    // per particle: first node (job-relative)
 };
 
 }  // namespace
@@ -136,6 +138,9 @@ struct pd_ctx {
   hipEvent_t pev[PD_NPHASE + 1] = {};
   pd_profile acc{};
   bool use_graph = true, graph_dirty = true;
+  int bond_shape = -1;          // index into bond_shapes(); -1: tune at the next step
+  uint64_t tuned_nodes = 0;     // resident nodes when the shape was tuned
+  double tune_ms[kBondShapes] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 };
@@ -252,14 +257,19 @@ static void build_rows(const pd_particle_desc* P, uint32_t np, JobRows& R) {
       R.rows[fp[a]++] = (uint32_t)b;
       R.rows[fp[b]++] = (uint32_t)a;
     }
+  // dense rows: entry k of a row in slot k
+  R.slot.resize(R.rows.size());
+  for (uint64_t g = 0; g < Np; g++)
+    for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) R.slot[k] = (uint32_t)(k - R.rstart[g]);
+  R.sdeg = R.deg;
   R.width.assign(R.nslices, 0);
-  R.entries = 0;
   for (uint64_t s = 0; s < R.nslices; s++) {
     uint32_t W = 0;
     for (int l = 0; l < kSlice; l++) W = std::max(W, R.deg[s * kSlice + l]);
     R.width[s] = W;
-    R.entries += (uint64_t)W * kSlice;
   }
+  R.entries = 0;
+  for (uint64_t s = 0; s < R.nslices; s++) R.entries += (uint64_t)R.width[s] * kSlice;
 }
 
 static bool rows_sorted(const JobRows& R, uint64_t g0, uint64_t g1) {
@@ -626,8 +636,17 @@ int pd_add_job(pd_ctx* c, const pd_run_desc* run, const pd_particle_desc* P, uin
     }
     for (uint64_t g = 0; g < R.nodes; g++) {
       const uint64_t s = g / kSlice, l = g % kSlice;
-      for (uint32_t k = 0; k < R.deg[g]; k++)
-        bond[soff[s] - b0 + (uint64_t)k * kSlice + l] = (uint32_t)(g0 + R.rows[R.rstart[g] + k]);
+      uint32_t* col = bond.data() + (soff[s] - b0) + l;
+      for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++)
+        col[(uint64_t)R.slot[k] * kSlice] = (uint32_t)(g0 + R.rows[k]);
+      // skipped slots: dead entries carrying the next neighbour's index, so
+      // that a sorted row stays sorted for the bonded-pair search
+      uint32_t next = kPad;
+      for (uint32_t k = R.sdeg[g]; k-- > 0;) {
+        uint32_t& e = col[(uint64_t)k * kSlice];
+        if (e == kPad) e = next;
+        else next = e | kBroken;
+      }
     }
   }
   DevInst I;
@@ -654,7 +673,7 @@ int pd_add_job(pd_ctx* c, const pd_run_desc* run, const pd_particle_desc* P, uin
   }
   UP(d.pid + g0, pid.data(), Np);
   UP(d.nflag + g0, nflag.data(), Np);
-  UP(d.deg + g0, R.deg.data(), Np);
+  UP(d.deg + g0, R.sdeg.data(), Np);
   UP(d.soff + s0, soff.data(), R.nslices);
   UP(d.swid + s0, R.width.data(), R.nslices);
   if (R.entries) UP(d.bond + b0, bond.data(), R.entries);
@@ -788,8 +807,9 @@ static int launch_step(pd_ctx* c) {
     hipLaunchKernelGGL(k_scatter, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
   }
   if ((rc = mark(2))) return rc;
-  // grid padded to a multiple of the XCD count (k_bond remaps blocks)
-  hipLaunchKernelGGL(k_bond, dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
+  // grid padded to a multiple of the XCD count (the bond kernel remaps blocks)
+  hipLaunchKernelGGL(bond_shapes()[c->bond_shape < 0 ? 0 : c->bond_shape].fn,
+                     dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
   if ((rc = mark(3))) return rc;
   hipLaunchKernelGGL(k_contact, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if ((rc = mark(4))) return rc;
@@ -808,6 +828,61 @@ static int launch_step(pd_ctx* c) {
     }
   }
   return PD_OK;
+}
+
+// Time every bond-kernel shape on the resident pool and keep the fastest.
+// The timed launches write their force sums and breaks into scratch copies,
+// so the state is untouched.
+static int autotune(pd_ctx* c, uint32_t reps) {
+  Dev t = c->d;
+  const uint64_t Np = t.S * kSlice;
+  int rc = PD_OK;
+  double* F = nullptr;
+  uint32_t* B = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if ((rc = dalloc(c, &F, 3 * Np)) || (rc = dalloc(c, &B, std::max<uint64_t>(c->cap_entries, 1)))) return rc;
+  auto release = [&]() {
+    for (void* q : {(void*)F, (void*)B}) {
+      auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
+      if (it != c->allocs.end()) c->allocs.erase(it);
+      (void)hipFree(q);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  auto run = [&]() -> int {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->cap_entries)
+      HIPCHK(c, hipMemcpyAsync(B, c->d.bond, c->cap_entries * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    t.F = F;
+    t.bond = B;
+    const dim3 grid((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds);
+    int best = 0;
+    for (int v = 0; v < kBondShapes; v++) {
+      hipLaunchKernelGGL(bond_shapes()[v].fn, grid, dim3(256), 0, c->stream, t);   // warm-up
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+      for (uint32_t r = 0; r < reps; r++) hipLaunchKernelGGL(bond_shapes()[v].fn, grid, dim3(256), 0, c->stream, t);
+      HIPCHK(c, hipEventRecord(e1, c->stream));
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+      c->tune_ms[v] = ms / reps;
+      if (c->tune_ms[v] < c->tune_ms[best]) best = v;
+    }
+    c->bond_shape = best;
+    return PD_OK;
+  };
+  rc = run();
+  release();
+  uint64_t nodes = 0;
+  for (auto& J : c->jobs)
+    if (J.used) nodes += J.nodes;
+  c->tuned_nodes = nodes;
+  c->graph_dirty = true;
+  return rc;
 }
 
 static int drop_graph(pd_ctx* c) {
@@ -854,6 +929,7 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
   for (auto& J : c->jobs) any |= J.used && J.remaining > 0;
   if (!any) return PD_OK;
   int rc;
+  if (c->bond_shape < 0 && (rc = autotune(c, 3))) return rc;
   // steps_left drives reaction_step(): the reaction is formed on the last
   // step of this call (and on each job's own last step)
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -885,6 +961,41 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
   HIPCHK(c, hipMemcpy(&err, c->d.err, sizeof err, hipMemcpyDeviceToHost));
   if (err & kErrCandOverflow)
     return fail(c, PD_E_OVERFLOW, "more than %d contact partners within R_c for a node", kMaxCand);
+  return PD_OK;
+}
+
+int pd_bond_shape_count(void) { return kBondShapes; }
+
+int pd_bond_shape_info(int shape, int* unroll, int* pipelined, int* masked) {
+  if (shape < 0 || shape >= kBondShapes) return PD_E_ARG;
+  if (unroll) *unroll = bond_shapes()[shape].unroll;
+  if (pipelined) *pipelined = bond_shapes()[shape].pipe;
+  if (masked) *masked = bond_shapes()[shape].mask;
+  return PD_OK;
+}
+
+int pd_set_bond_shape(pd_ctx* c, int shape) {
+  if (!c) return PD_E_ARG;
+  if (shape < -1 || shape >= kBondShapes) return fail(c, PD_E_ARG, "no bond shape %d", shape);
+  c->bond_shape = shape;
+  c->graph_dirty = true;
+  return PD_OK;
+}
+
+int pd_autotune(pd_ctx* c, uint32_t reps, int* best, double* ms) {
+  if (!c) return PD_E_ARG;
+  if (!c->reserved) return fail(c, PD_E_STATE, "pd_autotune before pd_commit / pd_reserve");
+  int rc = autotune(c, reps ? reps : 3);
+  if (rc) return rc;
+  if (best) *best = c->bond_shape;
+  if (ms)
+    for (int v = 0; v < kBondShapes; v++) ms[v] = c->tune_ms[v];
+  return PD_OK;
+}
+
+int pd_get_bond_shape(pd_ctx* c, int* shape) {
+  if (!c || !shape) return PD_E_ARG;
+  *shape = c->bond_shape;
   return PD_OK;
 }
 

@@ -251,10 +251,97 @@ __device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) 
 // Workgroups are mapped so that each XCD (blocks are dealt round-robin over
 // the 8 XCDs) runs one contiguous range of nodes: neighbouring tiles, which
 // gather the same neighbour records, share that XCD's L2.
-constexpr int kBondUnroll = 8;
 constexpr int kXcds = 8;
 
-__global__ __launch_bounds__(256) void k_bond(Dev d) {
+// The k-th entries of the U rows-batches starting at k (kPad past the width).
+template <int U>
+__device__ __forceinline__ void bond_idx(const uint32_t* bp, uint32_t k, uint32_t W, uint32_t (&e)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; u++) e[u] = (k + u < W) ? bp[(uint64_t)(k + u) * kSlice] : kPad;
+}
+
+// Neighbour records of a batch.  A dead entry reads the node itself
+// (MASK = false, branch-free) or issues no load at all (MASK = true: its
+// lanes are switched off for the loads and take the node's own record).
+template <int U, bool MASK>
+__device__ __forceinline__ void bond_gather(const Dev& d, uint64_t g, const uint32_t (&e)[U], const double4& xi,
+                                            const double4& ui, double4 (&xj)[U], double4 (&uj)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    if (MASK) {
+      if (!(e[u] & kBroken)) {
+        xj[u] = d.X[e[u]];
+        uj[u] = d.U[e[u]];
+      } else {
+        xj[u] = xi;
+        uj[u] = ui;
+      }
+    } else {
+      const uint32_t j = (e[u] & kBroken) ? (uint32_t)g : e[u];
+      xj[u] = d.X[j];
+      uj[u] = d.U[j];
+    }
+  }
+}
+
+// Terms of one batch, added to (f0, f1, f2) in row order; breaks recorded.
+template <int U>
+__device__ __forceinline__ void bond_terms(uint32_t* bp, uint32_t k, const uint32_t (&e)[U], const double4 (&xj)[U],
+                                           const double4 (&uj)[U], const double4& xi, const double4& ui, bool fast,
+                                           bool can_break, double cnot, double snot, double& f0, double& f1,
+                                           double& f2) {
+  bool alive[U];
+  double ev[U][3], n2[U], l2[U];
+  bool ok = fast;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    alive[u] = !(e[u] & kBroken);
+    double x0 = xj[u].x - xi.x, x1 = xj[u].y - xi.y, x2 = xj[u].z - xi.z;  // xi (gen_xi)
+    ev[u][0] = (uj[u].x - ui.x) + x0;
+    ev[u][1] = (uj[u].y - ui.y) + x1;
+    ev[u][2] = (uj[u].z - ui.z) + x2;
+    n2[u] = alive[u] ? (ev[u][0] * ev[u][0] + ev[u][1] * ev[u][1]) + ev[u][2] * ev[u][2] : 1.0;
+    l2[u] = alive[u] ? (x0 * x0 + x1 * x1) + x2 * x2 : 1.0;
+    ok &= (n2[u] >= kFastLo) & (n2[u] <= kFastHi);
+  }
+  double str[U], t[U][3];
+  if (__all(ok)) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      double nn = sqrt_core(n2[u]);
+      double L = sqrt_core(l2[u]);
+      str[u] = div_core(nn - L, rcp_core(L));
+      double c = alive[u] ? (cnot * str[u]) * xj[u].w : 0.0;
+      Rcp rn = rcp_core(nn);
+#pragma unroll
+      for (int q = 0; q < 3; q++) t[u][q] = c * div_core(ev[u][q], rn);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      double nn = sqrt(n2[u]);
+      double L = sqrt(l2[u]);
+      str[u] = (nn - L) / L;
+      double c = alive[u] ? (cnot * str[u]) * xj[u].w : 0.0;
+#pragma unroll
+      for (int q = 0; q < 3; q++) t[u][q] = c * (ev[u][q] / nn);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    f0 += t[u][0];
+    f1 += t[u][1];
+    f2 += t[u][2];
+    if (alive[u] & can_break & (str[u] > snot)) bp[(uint64_t)(k + u) * kSlice] = e[u] | kBroken;
+  }
+}
+
+// U: entries per batch.  MASK: see bond_gather.  PIPE: software pipelining -- the indices of batch
+// k+2 and the neighbour records of batch k+1 are in flight while batch k is
+// evaluated, so the index stream (HBM) and the gathers (L2) overlap the FP64
+// arithmetic instead of stalling every batch.
+template <int U, bool PIPE, bool MASK>
+__global__ __launch_bounds__(256) void k_bond_t(Dev d) {
   const uint64_t nbp = gridDim.x;   // a multiple of kXcds
   const uint64_t blk = (blockIdx.x % kXcds) * (nbp / kXcds) + blockIdx.x / kXcds;
   const uint64_t g = blk * blockDim.x + threadIdx.x;
@@ -278,70 +365,57 @@ __global__ __launch_bounds__(256) void k_bond(Dev d) {
     uint32_t* bp = d.bond + base + (g & 63);
     const double4 xi = d.X[g];
     const double4 ui = d.U[g];
-    constexpr int U = kBondUnroll;
-    for (uint32_t k = 0; k < W; k += U) {
-      uint32_t e[U];
-      bool alive[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        e[u] = (k + u < W) ? bp[(uint64_t)(k + u) * kSlice] : kPad;
-        alive[u] = !(e[u] & kBroken);
+    if (!PIPE) {
+      for (uint32_t k = 0; k < W; k += U) {
+        uint32_t e[U];
+        double4 xj[U], uj[U];
+        bond_idx<U>(bp, k, W, e);
+        bond_gather<U, MASK>(d, g, e, xi, ui, xj, uj);
+        bond_terms<U>(bp, k, e, xj, uj, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
       }
-      double4 xj[U], uj[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const uint32_t j = alive[u] ? e[u] : (uint32_t)g;
-        xj[u] = d.X[j];
-        uj[u] = d.U[j];
-      }
-      double ev[U][3], n2[U], l2[U];
-      bool ok = fast;
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        double x0 = xj[u].x - xi.x, x1 = xj[u].y - xi.y, x2 = xj[u].z - xi.z;  // xi (gen_xi)
-        ev[u][0] = (uj[u].x - ui.x) + x0;
-        ev[u][1] = (uj[u].y - ui.y) + x1;
-        ev[u][2] = (uj[u].z - ui.z) + x2;
-        n2[u] = alive[u] ? (ev[u][0] * ev[u][0] + ev[u][1] * ev[u][1]) + ev[u][2] * ev[u][2] : 1.0;
-        l2[u] = alive[u] ? (x0 * x0 + x1 * x1) + x2 * x2 : 1.0;
-        ok &= (n2[u] >= kFastLo) & (n2[u] <= kFastHi);
-      }
-      double str[U], t[U][3];
-      if (__all(ok)) {
+    } else if (W) {
+      uint32_t e0[U], e1[U];
+      double4 xa[U], ua[U];
+      bond_idx<U>(bp, 0, W, e0);
+      bond_idx<U>(bp, U, W, e1);
+      bond_gather<U, MASK>(d, g, e0, xi, ui, xa, ua);
+      for (uint32_t k = 0; k < W; k += U) {
+        uint32_t e2[U];
+        double4 xb[U], ub[U];
+        bond_idx<U>(bp, k + 2 * U, W, e2);
+        bond_gather<U, MASK>(d, g, e1, xi, ui, xb, ub);
+        bond_terms<U>(bp, k, e0, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
 #pragma unroll
         for (int u = 0; u < U; u++) {
-          double nn = sqrt_core(n2[u]);
-          double L = sqrt_core(l2[u]);
-          str[u] = div_core(nn - L, rcp_core(L));
-          double c = alive[u] ? (cnot * str[u]) * xj[u].w : 0.0;
-          Rcp rn = rcp_core(nn);
-#pragma unroll
-          for (int q = 0; q < 3; q++) t[u][q] = c * div_core(ev[u][q], rn);
+          e0[u] = e1[u];
+          e1[u] = e2[u];
+          xa[u] = xb[u];
+          ua[u] = ub[u];
         }
-      } else {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          double nn = sqrt(n2[u]);
-          double L = sqrt(l2[u]);
-          str[u] = (nn - L) / L;
-          double c = alive[u] ? (cnot * str[u]) * xj[u].w : 0.0;
-#pragma unroll
-          for (int q = 0; q < 3; q++) t[u][q] = c * (ev[u][q] / nn);
-        }
-      }
-      // summed in row order
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        f0 += t[u][0];
-        f1 += t[u][1];
-        f2 += t[u][2];
-        if (alive[u] & can_break & (str[u] > snot)) bp[(uint64_t)(k + u) * kSlice] = e[u] | kBroken;
       }
     }
   }
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
+}
+
+// Instantiated shapes of the bond kernel; the autotuner times them on the
+// resident pool and keeps the fastest.  Every shape sums each row in row order,
+// so all give the same bits.
+struct BondShape {
+  void (*fn)(Dev);
+  int unroll, pipe, mask;
+};
+constexpr int kBondShapes = 9;
+__host__ inline const BondShape* bond_shapes() {
+  static const BondShape t[kBondShapes] = {
+      {k_bond_t<8, false, false>, 8, 0, 0}, {k_bond_t<4, false, false>, 4, 0, 0},
+      {k_bond_t<6, false, false>, 6, 0, 0}, {k_bond_t<3, true, false>, 3, 1, 0},
+      {k_bond_t<4, true, false>, 4, 1, 0},  {k_bond_t<5, true, false>, 5, 1, 0},
+      {k_bond_t<8, false, true>, 8, 0, 1},  {k_bond_t<3, true, true>, 3, 1, 1},
+      {k_bond_t<4, true, true>, 4, 1, 1}};
+  return t;
 }
 
 // ---------------------------------------------------------------------------

@@ -111,3 +111,50 @@ def test_set_state_resume(oracle_lib):
     finally:
         g.close()
         o.close()
+
+
+def test_every_bond_shape_matches_oracle(oracle_lib):
+    """Each instantiated bond-kernel shape gives the oracle's bits."""
+    s = small_impact(seed=9)
+    o = oracle_lib.engine(s)
+    o.step(40)
+    ref = [o.state(p) for p in range(len(s.particles))]
+    refc = [o.connectivity(p) for p in range(len(s.particles))]
+    o.close()
+    for k, shape in enumerate(abi.bond_shapes()):
+        print(f"[parity] bond shape {k} {shape}", flush=True)
+        g = abi.gpu_engine(s)
+        try:
+            g.set_bond_shape(k)
+            g.step(40)
+            assert g.bond_shape() == k
+            for p in range(len(s.particles)):
+                st = g.state(p)
+                for f in ("disp", "currpos", "vel", "acc", "force"):
+                    assert_bitwise(st[f], ref[p][f], f"shape {k} particle {p} {f}")
+                assert np.array_equal(g.connectivity(p), refc[p]), f"shape {k} particle {p} bonds"
+        finally:
+            g.close()
+
+
+def test_autotune_leaves_state_untouched(oracle_lib):
+    s = small_impact(seed=12)
+    g = abi.gpu_engine(s)
+    o = oracle_lib.engine(s)
+    try:
+        g.step(15)
+        o.step(15)
+        before = [g.state(p) for p in range(len(s.particles))]
+        best, ms = g.autotune(2)
+        assert 0 <= best < len(abi.bond_shapes()) and g.bond_shape() == best
+        assert np.all(ms > 0)
+        for p in range(len(s.particles)):
+            after = g.state(p)
+            for f in ("disp", "vel", "acc", "force"):
+                assert_bitwise(after[f], before[p][f], f"autotune changed particle {p} {f}")
+        g.step(25)
+        o.step(25)
+        compare_engines(g, o, s, "after autotune")
+    finally:
+        g.close()
+        o.close()
