@@ -17,6 +17,19 @@ roofline: the bond-force kernel's algorithmic bytes per launch
 with HIP events on the library's stream in a separate profiled run.
 cpu_baseline: the CPU oracle (same algorithm, OpenMP threads emulating the
 reference's MPI ranks) on a bounded sample of the same workload.
+
+batched: 256 independent two-sphere impacts (node counts per sphere
+log-uniform on [500, 20k], speed U(5, 50) m/s, E U(20, 100) GPa with s0 and
+contact stiffness per instance) admitted into one job pool and stepped
+together, against the same jobs run one after another through the plain
+interface.  sims/hour counts whole jobs (upload, steps, read-back); every
+job's result digest is compared with its solo run.
+serving: a seeded Poisson stream of aggregate-breakage jobs (2-64 spheres and
+cubes of 2k-8k nodes each, step budgets drawn from 5k-50k and scaled by
+--serve-step-scale) through continuous batching with admission by device
+footprint, at several arrival rates; p50/p99 latency and sims/hour next to a
+FIFO queue of sequential plain-GPU runs with the same arrivals (service times
+measured).
 """
 import argparse
 import json
@@ -69,6 +82,90 @@ def cpu_baseline(nspheres, steps, threads):
                       f"{dt:.1f} s"}
 
 
+def batched(args):
+    from peridem_amd import gen, serve
+    t0 = time.perf_counter()
+    systems = gen.impact_batch(args.batch_jobs)
+    nodes = sum(x.nodes for x in systems)
+    bonds = sum(x.movable_bonds() for x in systems)
+    log(f"[bench] batched: {len(systems)} impacts, {nodes} nodes, {bonds} directed bonds "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
+    wall_b, dev_ms, dig_b, adm = serve.run_batch(systems, args.batch_steps, digests=True)
+    log(f"[bench] batched: {wall_b:.2f} s wall ({adm:.2f} s admission), "
+        f"{dev_ms / args.batch_steps:.3f} ms device per step")
+    t_seq, dig_s = serve.run_sequential(systems, args.batch_steps, digests=True)
+    wall_s = sum(t_seq)
+    same = sum(a == b for a, b in zip(dig_b, dig_s))
+    log(f"[bench] sequential: {wall_s:.2f} s wall; {same}/{len(systems)} results identical")
+    return {"workload": f"{len(systems)} two-sphere impacts, nodes per sphere log-uniform on "
+                        f"[500, 20k], v U(5,50) m/s, E U(20,100) GPa, {args.batch_steps} steps each, "
+                        f"FP64, dt=1e-8",
+            "jobs": len(systems), "steps_per_job": args.batch_steps, "nodes": nodes,
+            "directed_bonds": bonds,
+            "sims_per_hour": len(systems) / wall_b * 3600.0,
+            "sequential_sims_per_hour": len(systems) / wall_s * 3600.0,
+            "speedup_vs_sequential": wall_s / wall_b,
+            "batch_wall_s": wall_b, "batch_admission_s": adm, "sequential_wall_s": wall_s,
+            "batch_ms_per_step": dev_ms / args.batch_steps,
+            "batch_bond_updates_per_s": bonds * args.batch_steps / (dev_ms * 1e-3),
+            "results_identical_to_solo": f"{same}/{len(systems)}"}
+
+
+def serving(args):
+    from peridem_amd import abi, gen, serve
+    jobs = []
+    for i in range(args.serve_jobs):
+        sysm = gen.aggregate_job(i)
+        jobs.append((sysm, max(1, int(round(sysm.meta["steps"] * args.serve_step_scale)))))
+    nodes = [x.nodes for x, _ in jobs]
+    log(f"[bench] serving: {len(jobs)} jobs, {min(nodes)}-{max(nodes)} nodes, steps "
+        f"{min(n for _, n in jobs)}-{max(n for _, n in jobs)}")
+    t_seq, dig_s = serve.run_sequential([x for x, _ in jobs], [n for _, n in jobs], digests=True)
+    log(f"[bench] serving: sequential service {sum(t_seq):.2f} s total")
+    # pool sized by a device-memory budget; a job is admitted when its footprint fits
+    fps = [abi.footprint(x) for x, _ in jobs]
+    budget = args.serve_pool_gib * (1 << 30)
+    tot_b = sum(f[2] for f in fps)
+    frac = min(1.0, budget / tot_b)
+    cap = dict(max_nodes=max(max(f[0] for f in fps), int(sum(f[0] for f in fps) * frac)),
+               max_bond_entries=max(max(f[1] for f in fps), int(sum(f[1] for f in fps) * frac)),
+               max_particles=sum(len(x.particles) for x, _ in jobs) + 1, max_jobs=len(jobs))
+    out = {"workload": f"Poisson stream of {len(jobs)} aggregate-breakage jobs (2-64 seeded spheres and "
+                       f"cubes of 2k-8k nodes, walls, self-contact), step budgets 5k-50k scaled by "
+                       f"{args.serve_step_scale}, FP64, dt=1e-8",
+           "jobs": len(jobs), "pool_budget_gib": args.serve_pool_gib,
+           "pool_node_slots": cap["max_nodes"], "pool_bond_entries": cap["max_bond_entries"],
+           "sequential_sims_per_hour": len(jobs) / sum(t_seq) * 3600.0, "rates": []}
+    for rate in args.serve_rates:
+        arr = serve.poisson_arrivals(len(jobs), rate, seed=int(rate * 1000) + 1)
+        srv = serve.Server(chunk=args.serve_chunk, **cap)
+        t0 = time.perf_counter()
+        try:
+            done = srv.serve([(a, x, n) for a, (x, n) in zip(arr, jobs)])
+        finally:
+            srv.close()
+        span = max(d.finished for d in done) - float(arr[0])
+        lat = [d.latency for d in sorted(done, key=lambda d: d.rid)]
+        same = sum(d.digest == dig_s[d.rid] for d in done)
+        fifo = serve.fifo_latency(arr, t_seq)
+        t = 0.0
+        for a, sv in zip(arr, t_seq):
+            t = max(t, a) + sv
+        seq_span = t - float(arr[0])
+        rec = {"rate_jobs_per_s": rate, "served": serve.latency_stats(lat),
+               "sequential_fifo": serve.latency_stats(fifo),
+               "sims_per_hour": len(jobs) / span * 3600.0,
+               "sequential_sims_per_hour_same_arrivals": len(jobs) / seq_span * 3600.0,
+               "peak_resident_jobs": srv.peak_live, "results_identical_to_solo": f"{same}/{len(jobs)}",
+               "wall_s": time.perf_counter() - t0}
+        log(f"[bench] serving rate {rate}/s: p50 {rec['served']['p50_s']:.3f} s, p99 "
+            f"{rec['served']['p99_s']:.3f} s (sequential FIFO p50 {rec['sequential_fifo']['p50_s']:.3f}, "
+            f"p99 {rec['sequential_fifo']['p99_s']:.3f}); {rec['sims_per_hour']:.0f} sims/h; "
+            f"peak {srv.peak_live} resident; {same}/{len(jobs)} identical")
+        out["rates"].append(rec)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +177,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=24)
+    ap.add_argument("--serve-rates", type=lambda v: [float(x) for x in v.split(",")],
+                    default=[2.0, 8.0, 50.0])
+    ap.add_argument("--serve-step-scale", type=float, default=0.05)
+    ap.add_argument("--serve-pool-gib", type=float, default=1.0)
+    ap.add_argument("--serve-chunk", type=int, default=256)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (a plain copy "
                          "kernel of known size that calibrates FETCH_SIZE/WRITE_SIZE)")
@@ -137,6 +244,9 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copy probe: {args.copy_probe_mib} MiB device-to-device")
 
+    bat = None if args.no_batch else batched(args)
+    srv = None if args.no_serve else serving(args)
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: {args.cpu_spheres} spheres x {args.cpu_steps} steps, "
@@ -168,6 +278,8 @@ def main():
                      "kernel": "k_bond", "bytes_per_launch": force_bytes,
                      "avg_launch_ms": force_ms},
         "cpu_baseline": cpu,
+        "batched": bat,
+        "serving": srv,
     }
     print(json.dumps(out), flush=True)
 

@@ -207,8 +207,9 @@ int pd_set_graphs(pd_ctx* ctx, int on);
  * per batch, software pipelining of the index and neighbour loads, loads
  * switched off for dead entries); each
  * sums every row in row order, so all give the same bits.  By default the
- * first pd_step times every shape on the resident state (scratch copies of
- * the force and bond arrays, so nothing changes) and keeps the fastest.
+ * first pd_step of a pool with at least 2^18 resident nodes times every
+ * shape on the resident state (scratch copies of the force and bond arrays,
+ * so nothing changes) and keeps the fastest; smaller pools use shape 4.
  * pd_autotune re-runs that with reps timed launches per shape, returning the
  * choice and (when ms != NULL) the mean time of each shape [count];
  * pd_set_bond_shape forces one (-1: tune at the next step). */

@@ -32,6 +32,8 @@ using namespace pd;
 namespace {
 
 constexpr uint64_t kUnlimited = ~0ull;
+constexpr uint64_t kTuneMinNodes = 1u << 18;  // autotune pools of at least this many nodes
+constexpr int kDefaultBondShape = 4;          // pipelined, 4 entries per batch
 constexpr int kGraphSteps = 8;  // steps per captured graph
 // device bytes per node slot (X, U, V, A, F, E, pid, nflag, deg, wmask, ckey,
 // csorted, ~2 hash buckets) and per bond entry
@@ -929,7 +931,15 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
   for (auto& J : c->jobs) any |= J.used && J.remaining > 0;
   if (!any) return PD_OK;
   int rc;
-  if (c->bond_shape < 0 && (rc = autotune(c, 3))) return rc;
+  if (c->bond_shape < 0) {
+    // small pools: the launch is too short for shape to matter more than
+    // the tuning itself; keep the shape that is fastest on large pools
+    if (c->cnt.nodes >= kTuneMinNodes) {
+      if ((rc = autotune(c, 3))) return rc;
+    } else {
+      c->bond_shape = kDefaultBondShape;
+    }
+  }
   // steps_left drives reaction_step(): the reaction is formed on the last
   // step of this call (and on each job's own last step)
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));

@@ -468,7 +468,9 @@ __global__ __launch_bounds__(256) void k_contact(Dev d) {
               if (!selfc) continue;
             } else {
               if (!pairc) continue;
-              uint32_t lo = pj < p ? pj : p, hi = pj < p ? p : pj;
+              // pair (lo, hi) in the job's particle order, which is node order
+              const bool below = Q.node_off < P.node_off;
+              uint32_t lo = below ? pj : p, hi = below ? p : pj;
               if (!(d.part[lo].movable || d.part[hi].movable)) continue;
               if (!intersects_box_fext(d.lrtb + 6 * lo, d.lrtb + 6 * hi, I.Rc)) continue;
             }
@@ -476,7 +478,7 @@ __global__ __launch_bounds__(256) void k_contact(Dev d) {
             const double4 xj = d.X[j];
             const double4 uj = d.U[j];
             double dd[3];
-            if (pj < p) {   // we are P_N's node j
+            if (Q.node_off < P.node_off) {   // we are P_N's node j
               dd[0] = (ui.x - uj.x) + (xi.x - xj.x);
               dd[1] = (ui.y - uj.y) + (xi.y - xj.y);
               dd[2] = (ui.z - uj.z) + (xi.z - xj.z);
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(256) void k_contact(Dev d) {
     const double4 uj = d.U[j];
     const double vj[3] = {d.V[3 * j], d.V[3 * j + 1], d.V[3 * j + 2]};
     double dd[3], rv[3];
-    if (pj < p) {
+    if (Q.node_off < P.node_off) {
       // partner is P_C (node i), we are P_N (node j): reaction terms
       for (int k = 0; k < 3; k++) rv[k] = vj[k] - vi[k];
       dd[0] = (ui.x - uj.x) + (xi.x - xj.x);

@@ -377,3 +377,65 @@ def impact_batch(n=256, seed=4, nodes_range=(500, 20000), speed_range=(5.0, 50.0
         E = float(rng.uniform(*E_range))
         out.append(impact_pair(float(nA), float(nB), v, E, seed=int(seed * 100003 + k), **kw))
     return out
+
+
+# ---------------------------------------------------------------------------
+# served aggregate-breakage jobs: particles are translated copies of a few
+# cached lattice templates, so a job with 64 particles builds in milliseconds
+_TEMPLATES = {}
+
+
+def _template(kind, nodes, h, seed, E=72e9, nu=0.25, rho=2440.0, Gc=135.0):
+    key = (kind, int(nodes), h, seed, E)
+    if key not in _TEMPLATES:
+        rng = np.random.default_rng(seed)
+        if kind == "cube":
+            L = lattice_cube(0.5 * nodes ** (1.0 / 3.0) - 0.5)
+        else:
+            L = lattice_sphere((3.0 * nodes / (4.0 * math.pi)) ** (1.0 / 3.0))
+        p = make_particle(L, h, np.zeros(3), rng, E, nu, rho, Gc)
+        _TEMPLATES[key] = p
+    return _TEMPLATES[key]
+
+
+def aggregate_job(job_id, nparticles_range=(2, 64), nodes_range=(2000, 8000),
+                  steps_range=(5000, 50000), h=1e-4, speed=8.0, buckets=6, seed=7, **sys_kw):
+    """One served job: 2-64 seeded spheres and cubes of 2k-8k nodes on a
+    jittered grid inside walls, each thrown toward the cluster centre, with a
+    seeded step budget.  Node counts are drawn log-uniform and rounded to one
+    of `buckets` sizes per shape so the lattices can be reused."""
+    rng = np.random.default_rng(seed * 1000003 + job_id)
+    n = int(rng.integers(nparticles_range[0], nparticles_range[1] + 1))
+    lo, hi = np.log(nodes_range[0]), np.log(nodes_range[1])
+    sizes = np.exp(np.linspace(lo, hi, buckets))
+    picks = []
+    for _ in range(n):
+        kind = "cube" if rng.random() < 0.5 else "sphere"
+        b = int(np.argmin(np.abs(sizes - np.exp(rng.uniform(lo, hi)))))
+        picks.append(_template(kind, sizes[b], h, seed=b * 2 + (kind == "cube")))
+    ext = max(float(np.abs(t.pos).max()) for t in picks)
+    pitch = 2 * ext + 2.5 * h
+    m = int(math.ceil(n ** (1.0 / 3.0)))
+    parts = []
+    cells = [(i, j, k) for k in range(m) for j in range(m) for i in range(m)][:n]
+    centre = np.full(3, 0.5 * m * pitch)
+    for t, (i, j, k) in zip(picks, cells):
+        c = np.array([(i + 0.5) * pitch, (j + 0.5) * pitch, (k + 0.5) * pitch])
+        c = c + rng.uniform(-0.2, 0.2, size=3) * h
+        p = Particle(pos=t.pos + c, vol=t.vol, conn=t.conn, delta=t.delta, rho=t.rho, cnot=t.cnot,
+                     snot=t.snot)
+        d = centre - c
+        nd = float(np.linalg.norm(d))
+        v = (d / nd if nd > 0 else rng.normal(size=3)) * speed * rng.uniform(0.5, 1.0)
+        p.vel = np.broadcast_to(v, (p.n, 3)).copy()
+        parts.append(p)
+    Rc = 0.9 * h
+    box = m * pitch
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(72e9, 0.25, Rc),
+              friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=1,
+              allow_wall=1, wall_lrtb=(0, 0, 0, box, box, box), name="aggregate_job")
+    kw.update(sys_kw)
+    s = System(particles=parts, **kw)
+    s.meta.update(job_id=job_id, steps=int(rng.integers(steps_range[0], steps_range[1] + 1)),
+                  nparticles=n)
+    return s

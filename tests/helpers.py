@@ -89,3 +89,28 @@ def mixed_system(seed=7):
                    wall_lrtb=(-4.2e-4, -4.1e-4, -3.6e-4, 11.3e-4, 11.4e-4, 3.7e-4),
                    wall_speed=(1.0, 0.0, 0.0, 0.0, -2.0, -1.0), name="mixed")
     return s
+
+
+def small_jobs(n, seed, walls_every=3):
+    """Ragged two-sphere impacts; every walls_every-th one inside moving walls
+    with reaction, some with a ramped external force."""
+    out = gen.impact_batch(n, seed=seed, nodes_range=(60, 700), speed_range=(20.0, 120.0))
+    for k, s in enumerate(out):
+        if k % walls_every == 0:
+            pos = np.concatenate([p.pos for p in s.particles])
+            lo, hi = pos.min(0) - 0.3e-4, pos.max(0) + 0.3e-4
+            s.allow_wall = 1
+            s.compute_wall_reaction = 1
+            s.wall_lrtb = tuple(lo) + tuple(hi)
+            s.wall_speed = (0.0, 2.0, 0.0, -15.0, 0.0, -1.0)
+        if k % 4 == 1:
+            for p in s.particles:
+                p.ext = np.zeros((p.n, 3))
+                p.ext[:, 1] = -3e9
+            s.gradient_extforce = 1
+            s.extforce_maxstep = 17
+        if k % 5 == 2:
+            s.allow_friction = 0
+        if k % 5 == 3:
+            s.dt = 7e-9
+    return out

@@ -9,35 +9,10 @@ few jobs, with the oracle directly.  Tolerance: 0 ulp.
 import numpy as np
 import pytest
 
-from helpers import FIELDS, assert_bitwise
+from helpers import FIELDS, assert_bitwise, small_jobs
 from peridem_amd import abi, gen
 
 pytestmark = pytest.mark.gpu
-
-
-def small_jobs(n, seed, walls_every=3):
-    """Ragged two-sphere impacts; every walls_every-th one inside moving walls
-    with reaction, some with a ramped external force."""
-    out = gen.impact_batch(n, seed=seed, nodes_range=(60, 700), speed_range=(20.0, 120.0))
-    for k, s in enumerate(out):
-        if k % walls_every == 0:
-            pos = np.concatenate([p.pos for p in s.particles])
-            lo, hi = pos.min(0) - 0.3e-4, pos.max(0) + 0.3e-4
-            s.allow_wall = 1
-            s.compute_wall_reaction = 1
-            s.wall_lrtb = tuple(lo) + tuple(hi)
-            s.wall_speed = (0.0, 2.0, 0.0, -15.0, 0.0, -1.0)
-        if k % 4 == 1:
-            for p in s.particles:
-                p.ext = np.zeros((p.n, 3))
-                p.ext[:, 1] = -3e9
-            s.gradient_extforce = 1
-            s.extforce_maxstep = 17
-        if k % 5 == 2:
-            s.allow_friction = 0
-        if k % 5 == 3:
-            s.dt = 7e-9
-    return out
 
 
 def alone(system, nsteps):
@@ -176,3 +151,26 @@ def test_graph_replay_matches_eager_launches():
     finally:
         a.close()
         b.close()
+
+
+def test_reused_particle_slots_keep_pair_roles():
+    """A job admitted into freed particle slots gets them in another order
+    than its particle order; contact roles (which particle of a pair is
+    P_C, whose box is extended by R_c) must still follow the job's particle
+    order.  Two colliding sphere pairs with walls and reaction."""
+    from helpers import small_impact
+    a, b, c = (small_impact(seed=k, R_h=3.0) for k in (1, 2, 3))
+    nsteps = 140
+    ref = alone(c, nsteps)
+    pool = pool_for([a, b], slack=1.6)
+    try:
+        ja = pool.add_job(a, 1000)
+        pool.add_job(b, 1000)
+        pool.step(10)
+        pool.remove_job(ja)
+        jc = pool.add_job(c, nsteps)      # takes a's slots, in reverse order
+        while pool.remaining(jc):
+            pool.step(min(pool.remaining(jc), 24))
+        compare_job(pool, jc, ref, "job in reused slots")
+    finally:
+        pool.close()
