@@ -157,11 +157,13 @@ def serving(args):
                "sims_per_hour": len(jobs) / span * 3600.0,
                "sequential_sims_per_hour_same_arrivals": len(jobs) / seq_span * 3600.0,
                "peak_resident_jobs": srv.peak_live, "results_identical_to_solo": f"{same}/{len(jobs)}",
+               "pool_steps": srv.steps_run, "device_s": srv.device_ms * 1e-3,
                "wall_s": time.perf_counter() - t0}
         log(f"[bench] serving rate {rate}/s: p50 {rec['served']['p50_s']:.3f} s, p99 "
             f"{rec['served']['p99_s']:.3f} s (sequential FIFO p50 {rec['sequential_fifo']['p50_s']:.3f}, "
             f"p99 {rec['sequential_fifo']['p99_s']:.3f}); {rec['sims_per_hour']:.0f} sims/h; "
-            f"peak {srv.peak_live} resident; {same}/{len(jobs)} identical")
+            f"peak {srv.peak_live} resident; {same}/{len(jobs)} identical; {srv.steps_run} pool steps, "
+            f"{srv.device_ms * 1e-3:.2f} s on the device")
         out["rates"].append(rec)
     return out
 

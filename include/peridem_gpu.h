@@ -190,6 +190,19 @@ int pd_job_footprint(const pd_particle_desc* parts, uint32_t nparts, uint64_t* n
 int pd_add_job(pd_ctx* ctx, const pd_run_desc* run, const pd_particle_desc* parts,
                uint32_t nparts, uint64_t nsteps, uint32_t* job);
 int pd_remove_job(pd_ctx* ctx, uint32_t job);
+/* pd_add_job in two halves, so that a server can lay out the next job on
+ * the host while the pool steps: pd_job_prepare builds the rows, node arrays
+ * and bond slices of a job without any context (thread-safe; errors are
+ * read with pd_last_error(NULL) on the same thread); pd_add_prepared places
+ * a prepared job into a pool (PD_E_NOMEM when it does not fit: the blob
+ * stays valid and can be added later).  The caller frees blobs. */
+typedef struct pd_job_blob pd_job_blob;
+int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* parts, uint32_t nparts,
+                   pd_job_blob** out);
+int pd_job_blob_footprint(const pd_job_blob* blob, uint64_t* node_slots, uint64_t* bond_entries,
+                          uint64_t* bytes);
+int pd_add_prepared(pd_ctx* ctx, const pd_job_blob* blob, uint64_t nsteps, uint32_t* job);
+void pd_job_blob_free(pd_job_blob* blob);
 int pd_job_remaining(pd_ctx* ctx, uint32_t job, uint64_t* remaining);
 /* pd_get_state / pd_get_connectivity / pd_set_flag for particle p of one
  * job; pd_get_wall takes the job slot as its instance argument. */

@@ -154,7 +154,13 @@ def load_gpu_lib(path=GPU_LIB):
                                                  C.POINTER(C.c_int)]),
                 ("pd_set_bond_shape", C.c_int, [P, C.c_int]),
                 ("pd_get_bond_shape", C.c_int, [P, C.POINTER(C.c_int)]),
-                ("pd_autotune", C.c_int, [P, _u32, C.POINTER(C.c_int), _pd])):
+                ("pd_autotune", C.c_int, [P, _u32, C.POINTER(C.c_int), _pd]),
+                ("pd_job_prepare", C.c_int, [C.POINTER(RunDesc), C.POINTER(ParticleDesc), _u32,
+                                             C.POINTER(P)]),
+                ("pd_job_blob_footprint", C.c_int, [P, C.POINTER(_u64), C.POINTER(_u64),
+                                                    C.POINTER(_u64)]),
+                ("pd_add_prepared", C.c_int, [P, P, _u64, _pu]),
+                ("pd_job_blob_free", None, [P])):
             f = getattr(lib, name)
             f.restype, f.argtypes = res, args
         _LIBS[path] = (lib, _bind(lib, "pd_"))
@@ -309,6 +315,44 @@ def footprint(system):
     return ns.value, ne.value, nb.value
 
 
+class JobBlob:
+    """A job laid out on the host by pd_job_prepare, ready to be placed."""
+
+    def __init__(self, lib, ptr):
+        self.lib, self.ptr = lib, ptr
+
+    def footprint(self):
+        ns, ne, nb = _u64(), _u64(), _u64()
+        self.lib.pd_job_blob_footprint(self.ptr, C.byref(ns), C.byref(ne), C.byref(nb))
+        return ns.value, ne.value, nb.value
+
+    def free(self):
+        if self.ptr:
+            self.lib.pd_job_blob_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def prepare_job(system):
+    """Lay a system out for a pool without any context (thread-safe; the C
+    call runs without the GIL, so it overlaps a pool stepping on another thread)."""
+    lib, _ = load_gpu_lib()
+    keep = []
+    arr = _descs(system, keep)
+    rd = system.run_desc()
+    ptr = C.c_void_p()
+    rc = lib.pd_job_prepare(C.byref(rd), arr, len(system.particles), C.byref(ptr))
+    if rc != PD_OK:
+        msg = lib.pd_last_error(None) or b""
+        raise PDError(f"pd_job_prepare: {ERRORS.get(rc, rc)}: {msg.decode(errors='replace')}")
+    return JobBlob(lib, ptr)
+
+
 class Pool:
     """A job pool of the HIP library: independent systems advanced together."""
 
@@ -350,6 +394,16 @@ class Pool:
         if rc == -6:
             return None
         self._check(rc, "add_job")
+        self.systems[job.value] = system
+        return job.value
+
+    def add_prepared(self, blob, system, nsteps=UNLIMITED):
+        """Place a prepared job (see prepare_job); None when it does not fit."""
+        job = _u32()
+        rc = self.lib.pd_add_prepared(self.ctx, blob.ptr, nsteps, C.byref(job))
+        if rc == -6:
+            return None
+        self._check(rc, "add_prepared")
         self.systems[job.value] = system
         return job.value
 
@@ -418,6 +472,14 @@ class Pool:
         ms = C.c_double()
         self._check(self.lib.pd_last_step_ms(self.ctx, C.byref(ms)), "last_step_ms")
         return ms.value
+
+    def set_profiling(self, on=True):
+        self._check(self.lib.pd_set_profiling(self.ctx, int(on)), "set_profiling")
+
+    def profile(self):
+        pr = Profile()
+        self._check(self.lib.pd_get_profile(self.ctx, C.byref(pr)), "get_profile")
+        return pr.as_dict()
 
     def close(self):
         if self.ctx:

@@ -15,12 +15,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/peridem_gpu.h"
@@ -34,6 +36,7 @@ namespace {
 constexpr uint64_t kUnlimited = ~0ull;
 constexpr uint64_t kTuneMinNodes = 1u << 18;  // autotune pools of at least this many nodes
 constexpr int kDefaultBondShape = 4;          // pipelined, 4 entries per batch
+constexpr double kSkinFrac = 0.3;             // candidate-list skin / R_c
 constexpr int kGraphSteps = 8;  // steps per captured graph
 // device bytes per node slot (X, U, V, A, F, E, pid, nflag, deg, wmask, ckey,
 // csorted, ~2 hash buckets) and per bond entry
@@ -109,8 +112,6 @@ struct JobRows {
   std::vector<uint32_t> deg;     // row length
   std::vector<uint64_t> rstart;
   std::vector<uint32_t> rows;
-  std::vector<uint32_t> slot;    // per row entry: its position k in the node's sliced row
-  std::vector<uint32_t> sdeg;    // per node: sliced row length (last slot + 1)
   std::vector<uint32_t> width;   // per slice
   std::vector<uint64_t> poff;    // per particle: first node (job-relative)
 };
@@ -140,6 +141,7 @@ struct pd_ctx {
   hipEvent_t pev[PD_NPHASE + 1] = {};
   pd_profile acc{};
   bool use_graph = true, graph_dirty = true;
+  uint64_t hwm_slices = 0;      // end of the highest node range in use
   int bond_shape = -1;          // index into bond_shapes(); -1: tune at the next step
   uint64_t tuned_nodes = 0;     // resident nodes when the shape was tuned
   double tune_ms[kBondShapes] = {};
@@ -194,9 +196,17 @@ static void fill_inst(const pd_run_desc& r, DevInst& D) {
   D.Kn = r.normal_stiffness;
   D.mu = r.friction_coefficient;
   D.zeta = r.damping_ratio;
-  // cell edge slightly above R_c: the reference's pair distance
-  // (u_j-u_i)+(X_j-X_i) differs from CurrPos_j-CurrPos_i by a few ulp
-  D.cell = r.contact_rad * (1.0 + 1e-9) + 1e-300;
+  // candidate lists: radius R_c plus a skin (the 1e-9 covers the few ulp by
+  // which differently ordered distance formulas differ); rebuilt when a node
+  // has moved by half the skin, less a relative 1e-6
+  const double skin = kSkinFrac * r.contact_rad;
+  D.rb = r.contact_rad * (1.0 + 1e-9) + skin + 1e-300;
+  D.cell = D.rb;
+  const double h = 0.5 * skin * (1.0 - 1e-6);
+  D.du_lim2 = h * h;
+  D.du2 = 0;
+  D.rebuild = 0;
+  D.dirty = 1;
   D.dt = r.dt;
   D.allow_wall = r.allow_wall;
   D.allow_damping = r.allow_damping;
@@ -216,15 +226,29 @@ static int check_run(pd_ctx* c, const pd_run_desc& r) {
   return PD_OK;
 }
 
+template <typename F>
+static void parallel_for(uint64_t n, uint64_t work, F f);
+
 static int check_particles(pd_ctx* c, const pd_particle_desc* P, uint32_t n) {
+  // Connectivity entries, checked in parallel: first bad entry per particle
+  std::vector<uint64_t> bad(n, ~0ull);
+  uint64_t work = 0;
+  for (uint32_t i = 0; i < n; i++) work += P[i].conn ? P[i].nconn : 0;
+  parallel_for(n, work, [&](uint64_t i) {
+    const pd_particle_desc& d = P[i];
+    if (!d.conn) return;
+    for (uint64_t l = 0; l < 2 * d.nconn; l++)
+      if (d.conn[l] >= d.nnodes) {
+        bad[i] = l;
+        return;
+      }
+  });
   for (uint32_t i = 0; i < n; i++) {
     const pd_particle_desc& d = P[i];
     if (d.nnodes == 0 || !d.pos || !d.vol) return fail(c, PD_E_ARG, "particle %u needs nnodes, pos, vol", i);
     if (d.nconn && !d.conn) return fail(c, PD_E_ARG, "particle %u: nconn > 0 but conn is NULL", i);
-    for (uint64_t l = 0; l < 2 * d.nconn; l++)
-      if (d.conn[l] >= d.nnodes)
-        return fail(c, PD_E_ARG, "particle %u: Connectivity entry %llu out of range", i,
-                    (unsigned long long)l);
+    if (bad[i] != ~0ull)
+      return fail(c, PD_E_ARG, "particle %u: Connectivity entry %llu out of range", i, (unsigned long long)bad[i]);
     if (d.nclamped && !d.clamped) return fail(c, PD_E_ARG, "particle %u: clamped is NULL", i);
     for (uint32_t q = 0; q < d.nclamped; q++)
       if (d.clamped[q] >= d.nnodes) return fail(c, PD_E_ARG, "particle %u: clamped node out of range", i);
@@ -235,35 +259,56 @@ static int check_particles(pd_ctx* c, const pd_particle_desc* P, uint32_t n) {
   return PD_OK;
 }
 
-// conn2NArr rows of a job (load_hdf5.h:27-37) and its slice widths
+// f(i) for i < n on up to 16 host threads (one when the job is small);
+// items are handed out one at a time.
+template <typename F>
+static void parallel_for(uint64_t n, uint64_t work, F f) {
+  unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (work < (1u << 20)) nt = 1;
+  nt = (unsigned)std::min<uint64_t>(nt, n);
+  if (nt <= 1) {
+    for (uint64_t i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::atomic<uint64_t> next(0);
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; t++)
+    th.emplace_back([&]() {
+      for (uint64_t i; (i = next.fetch_add(1)) < n;) f(i);
+    });
+  for (auto& x : th) x.join();
+}
+
+// conn2NArr rows of a job (load_hdf5.h:27-37) and its slice widths.  Rows
+// of different particles are independent and built in parallel.
 static void build_rows(const pd_particle_desc* P, uint32_t np, JobRows& R) {
   R.poff.resize(np);
-  uint64_t nodes = 0;
+  uint64_t nodes = 0, work = 0;
   for (uint32_t i = 0; i < np; i++) {
     R.poff[i] = nodes;
     nodes += P[i].nnodes;
+    work += P[i].nconn;
   }
   R.nodes = nodes;
   R.nslices = (nodes + kSlice - 1) / kSlice;
   const uint64_t Np = R.nslices * kSlice;
   R.deg.assign(Np, 0);
-  for (uint32_t i = 0; i < np; i++)
-    for (uint64_t l = 0; l < 2 * P[i].nconn; l++) R.deg[R.poff[i] + P[i].conn[l]]++;
+  parallel_for(np, work, [&](uint64_t i) {
+    uint32_t* dg = R.deg.data() + R.poff[i];
+    for (uint64_t l = 0; l < 2 * P[i].nconn; l++) dg[P[i].conn[l]]++;
+  });
   R.rstart.assign(Np + 1, 0);
   for (uint64_t g = 0; g < Np; g++) R.rstart[g + 1] = R.rstart[g] + R.deg[g];
   R.rows.resize(R.rstart[Np]);
-  std::vector<uint64_t> fp(R.rstart.begin(), R.rstart.end() - 1);
-  for (uint32_t i = 0; i < np; i++)
+  parallel_for(np, work, [&](uint64_t i) {
+    const uint64_t o = R.poff[i];
+    std::vector<uint64_t> fp(R.rstart.begin() + o, R.rstart.begin() + o + P[i].nnodes);
     for (uint64_t l = 0; l < P[i].nconn; l++) {
-      uint64_t a = R.poff[i] + P[i].conn[2 * l], b = R.poff[i] + P[i].conn[2 * l + 1];
-      R.rows[fp[a]++] = (uint32_t)b;
-      R.rows[fp[b]++] = (uint32_t)a;
+      const uint32_t a = P[i].conn[2 * l], b = P[i].conn[2 * l + 1];
+      R.rows[fp[a]++] = (uint32_t)(o + b);
+      R.rows[fp[b]++] = (uint32_t)(o + a);
     }
-  // dense rows: entry k of a row in slot k
-  R.slot.resize(R.rows.size());
-  for (uint64_t g = 0; g < Np; g++)
-    for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) R.slot[k] = (uint32_t)(k - R.rstart[g]);
-  R.sdeg = R.deg;
+  });
   R.width.assign(R.nslices, 0);
   for (uint64_t s = 0; s < R.nslices; s++) {
     uint32_t W = 0;
@@ -328,13 +373,15 @@ static int upload_pairs(pd_ctx* c) {
 }
 
 static void recount(pd_ctx* c) {
-  uint64_t nodes = 0, parts = 0, bonds = 0;
+  uint64_t nodes = 0, parts = 0, bonds = 0, hwm = 0;
   for (auto& J : c->jobs)
     if (J.used) {
       nodes += J.nodes;
       parts += J.parts.size();
       bonds += J.total_bonds;
+      hwm = std::max<uint64_t>(hwm, J.node0 / kSlice + J.nslices);
     }
+  c->hwm_slices = hwm;
   c->cnt.nodes = nodes;
   c->cnt.particles = parts;
   c->cnt.total_bonds = bonds;
@@ -456,6 +503,9 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(reaction, 18 * I);
   ALLOC(wmask, N);
   ALLOC(bflag, N / 256 + 1);
+  ALLOC(clist, N * (uint64_t)kMaxCand);
+  ALLOC(ccnt, N);
+  ALLOC(UB, N);
   ALLOC(ckey, N);
   ALLOC(cstart, (size_t)T + 1);
   ALLOC(ccur, T);
@@ -486,6 +536,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipMemsetAsync(d.ctl, 0, sizeof(DevCtl), st));
   HIPCHK(c, hipMemsetAsync(d.reaction, 0, 18 * I * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.wmask, 0, N, st));
+  HIPCHK(c, hipMemsetAsync(d.ccnt, 0, N, st));
+  HIPCHK(c, hipMemsetAsync(d.UB, 0, N * sizeof(double4), st));
   HIPCHK(c, hipMemsetAsync(d.bflag, 0, N / 256 + 1, st));
   std::vector<DevPart> vparts(P);
   for (auto& q : vparts) void_part(q, vi);
@@ -511,46 +563,73 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   return PD_OK;
 }
 
-int pd_add_job(pd_ctx* c, const pd_run_desc* run, const pd_particle_desc* P, uint32_t np,
-               uint64_t nsteps, uint32_t* job) {
-  if (!c || !run || !P || !np) return PD_E_ARG;
-  if (!c->reserved) return fail(c, PD_E_STATE, "pd_add_job before pd_reserve");
+}  // extern "C"
+
+// A job laid out on the host, independent of where it will sit in a pool:
+// node arrays, the particles' records and the bond slices with job-relative
+// node indices.  Building it needs no context (pd_job_prepare may run on
+// another thread while a pool steps); placing it is a few copies.
+struct pd_job_blob {
+  pd_run_desc run{};
+  uint32_t np = 0;
+  uint64_t nodes = 0, nslices = 0, entries = 0;
+  std::vector<uint64_t> poff;     // per particle: first node (job-relative)
+  std::vector<uint32_t> deg, width;
+  std::vector<double4> X, U;
+  std::vector<double> V, A, E;
+  std::vector<uint32_t> lpid;     // per node slot: particle index in the job, ~0 = void
+  std::vector<uint8_t> nflag;
+  std::vector<DevPart> dparts;    // node_off job-relative
+  std::vector<double> lrtb;
+  std::vector<HostPart> parts;    // off job-relative
+  std::vector<uint64_t> soff;     // relative to the job's first bond entry
+  std::vector<uint32_t> bond;     // job-relative node indices, kPad padding
+  uint32_t fl = 0;
+  uint64_t movable_bonds = 0, total_bonds = 0;
+};
+
+namespace pd {
+// bond entries of a newly placed job: job-relative -> pool node indices
+__global__ __launch_bounds__(256) void k_rebase(uint32_t* b, uint64_t n, uint32_t g0) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && b[i] != kPad) b[i] += g0;
+}
+}  // namespace pd
+
+static thread_local std::string t_err;
+
+extern "C" {
+
+int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t np, pd_job_blob** out) {
+  if (!run || !P || !np || !out) return PD_E_ARG;
+  *out = nullptr;
+  pd_ctx tmp;   // carries the error message only
   int rc;
-  if ((rc = check_run(c, *run)) || (rc = check_particles(c, P, np))) return rc;
-  uint32_t jid = 0;
-  while (jid < c->cap_jobs && c->jobs[jid].used) jid++;
-  if (jid == c->cap_jobs)
-    return fail(c, PD_E_NOMEM, "all %llu job slots are in use", (unsigned long long)c->cap_jobs);
-  if (c->part_free.size() < np) return fail(c, PD_E_NOMEM, "not enough particle slots");
+  if ((rc = check_run(&tmp, *run)) || (rc = check_particles(&tmp, P, np))) {
+    t_err = tmp.err;
+    return rc;
+  }
+  pd_job_blob* B = new pd_job_blob();
+  B->run = *run;
+  B->np = np;
   JobRows R;
   build_rows(P, np, R);
-  uint64_t s0 = 0, b0 = 0;
-  if (!c->node_free.alloc(R.nslices, &s0))
-    return fail(c, PD_E_NOMEM, "no free range of %llu node slices", (unsigned long long)R.nslices);
-  if (!c->bond_free.alloc(R.entries, &b0)) {
-    c->node_free.release(s0, R.nslices);
-    return fail(c, PD_E_NOMEM, "no free range of %llu bond entries", (unsigned long long)R.entries);
-  }
-  Job& J = c->jobs[jid];
-  J = Job();
-  J.used = true;
-  J.run = *run;
-  J.node0 = s0 * kSlice;
-  J.nslices = R.nslices;
-  J.bond0 = b0;
-  J.nentries = R.entries;
-  J.remaining = nsteps;
-  J.nodes = R.nodes;
-  const uint64_t g0 = J.node0, Np = R.nslices * kSlice;
-  const uint32_t vpart = (uint32_t)(c->d.P - 1);
-  // node arrays of the job; the tail of the last slice stays void
-  std::vector<double4> X(Np, make_double4(0, 0, 0, 0)), U(Np, make_double4(0, 0, 0, 0));
-  std::vector<double> V(3 * Np, 0.0), A(3 * Np, 0.0), E(3 * Np, 0.0);
-  std::vector<uint32_t> pid(Np, vpart);
-  std::vector<uint8_t> nflag(Np, 0);
-  std::vector<DevPart> dparts(np);
-  std::vector<double> lrtb(6 * np);
-  uint32_t fl = 0;
+  B->nodes = R.nodes;
+  B->nslices = R.nslices;
+  B->entries = R.entries;
+  B->poff = R.poff;
+  const uint64_t Np = R.nslices * kSlice;
+  B->X.assign(Np, make_double4(0, 0, 0, 0));
+  B->U.assign(Np, make_double4(0, 0, 0, 0));
+  B->V.assign(3 * Np, 0.0);
+  B->A.assign(3 * Np, 0.0);
+  B->E.assign(3 * Np, 0.0);
+  B->lpid.assign(Np, ~0u);
+  B->nflag.assign(Np, 0);
+  B->dparts.resize(np);
+  B->lrtb.resize(6 * np);
+  // the particles' host-side records (serial), then the node arrays, the
+  // fast-path domain checks and the bond slices (parallel)
   for (uint32_t i = 0; i < np; i++) {
     const pd_particle_desc& d = P[i];
     HostPart H;
@@ -561,9 +640,7 @@ int pd_add_job(pd_ctx* c, const pd_run_desc* run, const pd_particle_desc* P, uin
     H.movable = d.movable;
     H.breakable = d.breakable;
     H.stoppable = d.stoppable;
-    H.off = g0 + R.poff[i];
-    H.slot = c->part_free.back();
-    c->part_free.pop_back();
+    H.off = R.poff[i];
     H.nbonds = R.rstart[R.poff[i] + d.nnodes] - R.rstart[R.poff[i]];
     H.cl0.resize(9 * H.clamped.size());
     for (size_t q = 0; q < H.clamped.size(); q++)
@@ -573,13 +650,18 @@ int pd_add_job(pd_ctx* c, const pd_run_desc* run, const pd_particle_desc* P, uin
         H.cl0[9 * q + 3 + k] = d.vel ? d.vel[3 * cn + k] : 0.0;
         H.cl0[9 * q + 6 + k] = d.acc ? d.acc[3 * cn + k] : 0.0;
       }
-    J.total_bonds += H.nbonds;
-    if (H.movable) J.movable_bonds += H.nbonds;
-    DevPart& D = dparts[i];
+    B->total_bonds += H.nbonds;
+    if (H.movable) B->movable_bonds += H.nbonds;
+    B->parts.push_back(std::move(H));
+  }
+  const uint64_t work = R.rows.size();
+  std::vector<uint8_t> pfl(np, 0);
+  parallel_for(np, work, [&](uint64_t i) {
+    const pd_particle_desc& d = P[i];
+    DevPart& D = B->dparts[i];
     memset(&D, 0, sizeof D);
-    D.node_off = H.off;
+    D.node_off = R.poff[i];
     D.nnodes = d.nnodes;
-    D.inst = jid;
     D.rho = d.rho;
     D.rinv = 1 / d.rho;
     D.cnot = d.cnot;
@@ -588,20 +670,21 @@ int pd_add_job(pd_ctx* c, const pd_run_desc* run, const pd_particle_desc* P, uin
     D.breakable = d.breakable;
     D.stoppable = d.stoppable;
     D.rows_sorted = rows_sorted(R, R.poff[i], R.poff[i] + d.nnodes);
+    bool slow = false;
     double o[6] = {0, 0, 0, 0, 0, 0};
     for (uint32_t v = 0; v < d.nnodes; v++) {
       const uint64_t r = R.poff[i] + v;
       const double u0 = d.disp ? d.disp[3 * v] : 0.0, u1 = d.disp ? d.disp[3 * v + 1] : 0.0,
                    u2 = d.disp ? d.disp[3 * v + 2] : 0.0;
-      X[r] = make_double4(d.pos[3 * v], d.pos[3 * v + 1], d.pos[3 * v + 2], d.vol[v]);
-      U[r] = make_double4(u0, u1, u2, 0.0);
+      B->X[r] = make_double4(d.pos[3 * v], d.pos[3 * v + 1], d.pos[3 * v + 2], d.vol[v]);
+      B->U[r] = make_double4(u0, u1, u2, 0.0);
       for (int k = 0; k < 3; k++) {
-        V[3 * r + k] = d.vel ? d.vel[3 * v + k] : 0.0;
-        A[3 * r + k] = d.acc ? d.acc[3 * v + k] : 0.0;
-        E[3 * r + k] = d.extforce ? d.extforce[3 * v + k] : 0.0;
+        B->V[3 * r + k] = d.vel ? d.vel[3 * v + k] : 0.0;
+        B->A[3 * r + k] = d.acc ? d.acc[3 * v + k] : 0.0;
+        B->E[3 * r + k] = d.extforce ? d.extforce[3 * v + k] : 0.0;
       }
-      pid[r] = H.slot;
-      if (!(in_fast_range(X[r].x) && in_fast_range(X[r].y) && in_fast_range(X[r].z))) fl = kFlagSafeMath;
+      B->lpid[r] = (uint32_t)i;
+      slow |= !(in_fast_range(d.pos[3 * v]) && in_fast_range(d.pos[3 * v + 1]) && in_fast_range(d.pos[3 * v + 2]));
       // initial lagged box: min/max of pos + disp (timeloop.h:389-393)
       const double cp[3] = {d.pos[3 * v] + u0, d.pos[3 * v + 1] + u1, d.pos[3 * v + 2] + u2};
       for (int k = 0; k < 3; k++) {
@@ -610,94 +693,169 @@ int pd_add_job(pd_ctx* c, const pd_run_desc* run, const pd_particle_desc* P, uin
         o[3 + k] = (o[3 + k] < cp[k]) ? cp[k] : o[3 + k];
       }
     }
-    for (auto q : H.clamped) nflag[R.poff[i] + q] |= 1;
-    for (int k = 0; k < 6; k++) lrtb[6 * i + k] = o[k];
+    for (uint32_t q = 0; q < d.nclamped; q++) B->nflag[R.poff[i] + d.clamped[q]] |= 1;
+    for (int k = 0; k < 6; k++) B->lrtb[6 * i + k] = o[k];
+    // fast-path domain of the bond arithmetic: reference bond lengths
+    // (displacements are checked by the position kernel)
+    for (uint64_t g = R.poff[i]; g < R.poff[i] + d.nnodes && !slow; g++)
+      for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) {
+        const uint64_t q = R.rows[k] - R.poff[i], v = g - R.poff[i];
+        const double x0 = d.pos[3 * q] - d.pos[3 * v];
+        const double x1 = d.pos[3 * q + 1] - d.pos[3 * v + 1];
+        const double x2 = d.pos[3 * q + 2] - d.pos[3 * v + 2];
+        const double l2 = (x0 * x0 + x1 * x1) + x2 * x2;
+        if (!(l2 >= kFastLo && l2 <= kFastHi)) {
+          slow = true;
+          break;
+        }
+      }
+    pfl[i] = slow;
+  });
+  for (uint32_t i = 0; i < np; i++)
+    if (pfl[i]) B->fl = kFlagSafeMath;
+  // bond slices, entry k of lane l at soff[s] + 64k + l
+  B->soff.resize(R.nslices);
+  B->bond.assign(R.entries, kPad);
+  uint64_t at = 0;
+  for (uint64_t s = 0; s < R.nslices; s++) {
+    B->soff[s] = at;
+    at += (uint64_t)R.width[s] * kSlice;
+  }
+  parallel_for(R.nslices, work, [&](uint64_t s) {
+    for (uint64_t g = s * kSlice; g < std::min<uint64_t>(R.nodes, (s + 1) * kSlice); g++) {
+      uint32_t* col = B->bond.data() + B->soff[s] + (g % kSlice);
+      for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) col[(k - R.rstart[g]) * kSlice] = R.rows[k];
+    }
+  });
+  B->deg = std::move(R.deg);
+  B->width = std::move(R.width);
+  *out = B;
+  return PD_OK;
+}
+
+int pd_job_blob_footprint(const pd_job_blob* B, uint64_t* node_slots, uint64_t* bond_entries, uint64_t* bytes) {
+  if (!B) return PD_E_ARG;
+  if (node_slots) *node_slots = B->nslices * kSlice;
+  if (bond_entries) *bond_entries = B->entries;
+  if (bytes) *bytes = B->nslices * kSlice * kBytesPerNode + B->entries * kBytesPerEntry;
+  return PD_OK;
+}
+
+void pd_job_blob_free(pd_job_blob* B) { delete B; }
+
+int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* job) {
+  if (!c || !B) return PD_E_ARG;
+  if (!c->reserved) return fail(c, PD_E_STATE, "pd_add_prepared before pd_reserve");
+  int rc;
+  if ((rc = check_run(c, B->run))) return rc;
+  const uint32_t np = B->np;
+  uint32_t jid = 0;
+  while (jid < c->cap_jobs && c->jobs[jid].used) jid++;
+  if (jid == c->cap_jobs)
+    return fail(c, PD_E_NOMEM, "all %llu job slots are in use", (unsigned long long)c->cap_jobs);
+  if (c->part_free.size() < np) return fail(c, PD_E_NOMEM, "not enough particle slots");
+  if (B->nodes + c->cnt.nodes >= 0x7FFFFFFFull) return fail(c, PD_E_OVERFLOW, "more than 2^31-1 nodes");
+  uint64_t s0 = 0, b0 = 0;
+  if (!c->node_free.alloc(B->nslices, &s0))
+    return fail(c, PD_E_NOMEM, "no free range of %llu node slices", (unsigned long long)B->nslices);
+  if (!c->bond_free.alloc(B->entries, &b0)) {
+    c->node_free.release(s0, B->nslices);
+    return fail(c, PD_E_NOMEM, "no free range of %llu bond entries", (unsigned long long)B->entries);
+  }
+  Job& J = c->jobs[jid];
+  J = Job();
+  J.used = true;
+  J.run = B->run;
+  J.node0 = s0 * kSlice;
+  J.nslices = B->nslices;
+  J.bond0 = b0;
+  J.nentries = B->entries;
+  J.remaining = nsteps;
+  J.nodes = B->nodes;
+  J.movable_bonds = B->movable_bonds;
+  J.total_bonds = B->total_bonds;
+  const uint64_t g0 = J.node0, Np = B->nslices * kSlice;
+  // place: particle slots, pool node indices, slice offsets
+  std::vector<uint32_t> slot(np);
+  for (uint32_t i = 0; i < np; i++) {
+    slot[i] = c->part_free.back();
+    c->part_free.pop_back();
+    HostPart H = B->parts[i];
+    H.off = g0 + B->poff[i];
+    H.slot = slot[i];
     J.parts.push_back(std::move(H));
   }
-  // fast-path domain of the bond arithmetic: reference bond lengths
-  // (coordinates above; displacements are checked by the position kernel)
-  for (uint64_t g = 0; g < R.nodes && !fl; g++)
-    for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) {
-      const double4& a = X[g];
-      const double4& b = X[R.rows[k]];
-      double x0 = b.x - a.x, x1 = b.y - a.y, x2 = b.z - a.z;
-      double l2 = (x0 * x0 + x1 * x1) + x2 * x2;
-      if (!(l2 >= kFastLo && l2 <= kFastHi)) {
-        fl = kFlagSafeMath;
-        break;
-      }
-    }
-  // bond slices of the job, entry k of lane l at soff[s] + 64k + l
-  std::vector<uint64_t> soff(R.nslices);
-  std::vector<uint32_t> bond(R.entries, kPad);
-  {
-    uint64_t at = b0;
-    for (uint64_t s = 0; s < R.nslices; s++) {
-      soff[s] = at;
-      at += (uint64_t)R.width[s] * kSlice;
-    }
-    for (uint64_t g = 0; g < R.nodes; g++) {
-      const uint64_t s = g / kSlice, l = g % kSlice;
-      uint32_t* col = bond.data() + (soff[s] - b0) + l;
-      for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++)
-        col[(uint64_t)R.slot[k] * kSlice] = (uint32_t)(g0 + R.rows[k]);
-      // skipped slots: dead entries carrying the next neighbour's index, so
-      // that a sorted row stays sorted for the bonded-pair search
-      uint32_t next = kPad;
-      for (uint32_t k = R.sdeg[g]; k-- > 0;) {
-        uint32_t& e = col[(uint64_t)k * kSlice];
-        if (e == kPad) e = next;
-        else next = e | kBroken;
-      }
-    }
+  std::vector<uint32_t> pid(Np);
+  const uint32_t vpart = (uint32_t)(c->d.P - 1);
+  for (uint64_t r = 0; r < Np; r++) pid[r] = B->lpid[r] == ~0u ? vpart : slot[B->lpid[r]];
+  std::vector<uint64_t> soff(B->nslices);
+  for (uint64_t s = 0; s < B->nslices; s++) soff[s] = b0 + B->soff[s];
+  std::vector<DevPart> dparts(B->dparts);
+  for (uint32_t i = 0; i < np; i++) {
+    dparts[i].node_off = g0 + B->poff[i];
+    dparts[i].inst = jid;
   }
   DevInst I;
   memset(&I, 0, sizeof I);
-  fill_inst(*run, I);
-  memcpy(I.wall, run->wall_lrtb, sizeof I.wall);
-  memcpy(I.speed, run->wall_speed, sizeof I.speed);
+  fill_inst(B->run, I);
+  memcpy(I.wall, B->run.wall_lrtb, sizeof I.wall);
+  memcpy(I.speed, B->run.wall_speed, sizeof I.speed);
   I.remaining = nsteps;
   I.node0 = g0;
-  I.node1 = g0 + R.nodes;
-  I.t = run->first_step ? run->first_step : 1;
+  I.node1 = g0 + B->nodes;
+  I.t = B->run.first_step ? B->run.first_step : 1;
   const double zero18[18] = {0};
   Dev& d = c->d;
   HIPCHK(c, hipStreamSynchronize(c->stream));
 #define UP(dst, src, n) HIPCHK(c, hipMemcpy((dst), (src), (n) * sizeof(*(dst)), hipMemcpyHostToDevice))
-  UP(d.X + g0, X.data(), Np);
-  UP(d.U + g0, U.data(), Np);
-  UP(d.V + 3 * g0, V.data(), 3 * Np);
-  UP(d.A + 3 * g0, A.data(), 3 * Np);
-  UP(d.E + 3 * g0, E.data(), 3 * Np);
-  {
-    const std::vector<double> Z(3 * Np, 0.0);
-    UP(d.F + 3 * g0, Z.data(), 3 * Np);
-  }
+  UP(d.X + g0, B->X.data(), Np);
+  UP(d.U + g0, B->U.data(), Np);
+  UP(d.V + 3 * g0, B->V.data(), 3 * Np);
+  UP(d.A + 3 * g0, B->A.data(), 3 * Np);
+  UP(d.E + 3 * g0, B->E.data(), 3 * Np);
+  HIPCHK(c, hipMemsetAsync(d.F + 3 * g0, 0, 3 * Np * sizeof(double), c->stream));
   UP(d.pid + g0, pid.data(), Np);
-  UP(d.nflag + g0, nflag.data(), Np);
-  UP(d.deg + g0, R.sdeg.data(), Np);
-  UP(d.soff + s0, soff.data(), R.nslices);
-  UP(d.swid + s0, R.width.data(), R.nslices);
-  if (R.entries) UP(d.bond + b0, bond.data(), R.entries);
+  UP(d.nflag + g0, B->nflag.data(), Np);
+  UP(d.deg + g0, B->deg.data(), Np);
+  UP(d.soff + s0, soff.data(), B->nslices);
+  UP(d.swid + s0, B->width.data(), B->nslices);
+  if (B->entries) {
+    UP(d.bond + b0, B->bond.data(), B->entries);
+    hipLaunchKernelGGL(k_rebase, dim3(grid_of(B->entries, 256)), dim3(256), 0, c->stream, d.bond + b0, B->entries,
+                       (uint32_t)g0);
+    HIPCHK(c, hipGetLastError());
+  }
   for (uint32_t i = 0; i < np; i++) {
-    const uint32_t sl = J.parts[i].slot;
-    UP(d.part + sl, &dparts[i], 1);
-    UP(d.lrtb + 6 * sl, &lrtb[6 * i], 6);
+    UP(d.part + slot[i], &dparts[i], 1);
+    UP(d.lrtb + 6 * slot[i], &B->lrtb[6 * i], 6);
   }
   UP(d.inst + jid, &I, 1);
   UP(d.reaction + 18 * jid, zero18, 18);
 #undef UP
-  if (fl) {
+  if (B->fl) {
     uint32_t cur = 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(&cur, d.flags, sizeof cur, hipMemcpyDeviceToHost));
-    cur |= fl;
+    cur |= B->fl;
     HIPCHK(c, hipMemcpy(d.flags, &cur, sizeof cur, hipMemcpyHostToDevice));
   }
-  if ((rc = upload_pairs(c))) return rc;
+  if ((rc = upload_pairs(c))) return rc;   // synchronises the stream
   recount(c);
   c->graph_dirty = true;
   if (job) *job = jid;
   return PD_OK;
+}
+
+int pd_add_job(pd_ctx* c, const pd_run_desc* run, const pd_particle_desc* P, uint32_t np, uint64_t nsteps,
+               uint32_t* job) {
+  if (!c || !run || !P || !np) return PD_E_ARG;
+  if (!c->reserved) return fail(c, PD_E_STATE, "pd_add_job before pd_reserve");
+  pd_job_blob* B = nullptr;
+  int rc = pd_job_prepare(run, P, np, &B);
+  if (rc) return fail(c, rc, "%s", t_err.c_str());
+  rc = pd_add_prepared(c, B, nsteps, job);
+  pd_job_blob_free(B);
+  return rc;
 }
 
 int pd_remove_job(pd_ctx* c, uint32_t job) {
@@ -785,8 +943,20 @@ int pd_commit(pd_ctx* c) {
 
 // One explicit step of every resident job (timeloop.h:395-797).  Contains no
 // memset and no host synchronisation unless profiling, so it can be captured.
+// The pool as the step kernels see it: node slots up to the highest one in
+// use, and a cell hash table sized to them.
+static Dev step_view(const pd_ctx* c) {
+  Dev d = c->d;
+  d.S = c->hwm_slices;
+  d.N = d.S * kSlice;
+  uint32_t T = 1024;
+  while (T < 2 * d.N && T < c->d.T) T <<= 1;
+  d.T = T;
+  return d;
+}
+
 static int launch_step(pd_ctx* c) {
-  Dev& d = c->d;
+  const Dev d = step_view(c);
   hipStream_t st = c->stream;
   const uint64_t Np = d.S * kSlice;
   auto mark = [&](int k) -> int {
@@ -797,16 +967,18 @@ static int launch_step(pd_ctx* c) {
   if ((rc = mark(0))) return rc;
   hipLaunchKernelGGL(k_position, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if ((rc = mark(1))) return rc;
+  hipLaunchKernelGGL(k_decide, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
   if (d.npairs) hipLaunchKernelGGL(k_pairs, dim3(grid_of(d.npairs, 256)), dim3(256), 0, st, d);
   if (any_contact_possible(c)) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_of((uint64_t)d.T + 1, 256)), dim3(256), 0, st, d.cstart,
-                       (uint64_t)d.T + 1, 0u);
+    // every kernel of the list rebuild exits at once unless a job rebuilds
+    hipLaunchKernelGGL(k_clear_cells, dim3(grid_of((uint64_t)d.T + 1, 256)), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_cellkey, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
     const uint32_t nb = grid_of(d.T, kScanBlock);
-    hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(kScanBlock), 0, st, d.cstart, d.T, d.scan_tmp);
-    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kScanBlock), 0, st, d.scan_tmp, nb, d.cstart + d.T);
-    hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(kScanBlock), 0, st, d.cstart, d.T, d.scan_tmp, d.ccur);
+    hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(kScanBlock), 0, st, d.ctl, d.cstart, d.T, d.scan_tmp);
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kScanBlock), 0, st, d.ctl, d.scan_tmp, nb, d.cstart + d.T);
+    hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(kScanBlock), 0, st, d.ctl, d.cstart, d.T, d.scan_tmp, d.ccur);
     hipLaunchKernelGGL(k_scatter, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_build, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
   }
   if ((rc = mark(2))) return rc;
   // grid padded to a multiple of the XCD count (the bond kernel remaps blocks)
@@ -836,7 +1008,7 @@ static int launch_step(pd_ctx* c) {
 // The timed launches write their force sums and breaks into scratch copies,
 // so the state is untouched.
 static int autotune(pd_ctx* c, uint32_t reps) {
-  Dev t = c->d;
+  Dev t = step_view(c);
   const uint64_t Np = t.S * kSlice;
   int rc = PD_OK;
   double* F = nullptr;
@@ -970,7 +1142,7 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
   uint32_t err = 0;
   HIPCHK(c, hipMemcpy(&err, c->d.err, sizeof err, hipMemcpyDeviceToHost));
   if (err & kErrCandOverflow)
-    return fail(c, PD_E_OVERFLOW, "more than %d contact partners within R_c for a node", kMaxCand);
+    return fail(c, PD_E_OVERFLOW, "more than %d contact candidates within R_c + skin of a node", kMaxCand);
   return PD_OK;
 }
 
@@ -1075,6 +1247,7 @@ static int job_set_flag(pd_ctx* c, uint32_t j, int flag, int value) {
     D.allow_damping = r.allow_damping;
     D.allow_friction = r.allow_friction;
     D.compute_reaction = r.compute_wall_reaction;
+    D.dirty = 1;   // self-contact eligibility may have changed
   });
   c->graph_dirty = true;
   return rc;
@@ -1107,7 +1280,7 @@ static int edit_part(pd_ctx* c, uint32_t job, uint32_t p, int which, int value, 
   D.breakable = H->breakable;
   D.snot = H->snot;
   HIPCHK(c, hipMemcpy(c->d.part + H->slot, &D, sizeof D, hipMemcpyHostToDevice));
-  return PD_OK;
+  return edit_inst(c, job, [](DevInst& I) { I.dirty = 1; });
 }
 
 extern "C" {
@@ -1263,7 +1436,7 @@ int pd_set_state(pd_ctx* c, uint32_t pi, const double* disp, const double* vel, 
       if (acc) p.cl0[9 * q + 6 + k] = acc[3 * i + k];
     }
   c->jobs[job].steps_since_set = 0;
-  return PD_OK;
+  return edit_inst(c, job, [](DevInst& I) { I.dirty = 1; });
 }
 
 int pd_job_get_connectivity(pd_ctx* c, uint32_t job, uint32_t pi, uint32_t* pairs, uint64_t cap, uint64_t* m) {
@@ -1343,6 +1516,9 @@ int pd_get_counters(pd_ctx* c, pd_counters* out) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(&pw, c->d.counters, sizeof pw, hipMemcpyDeviceToHost));
     out->pair_computations = pw;
+    unsigned long long rb = 0;
+    HIPCHK(c, hipMemcpy(&rb, c->d.counters + 2, sizeof rb, hipMemcpyDeviceToHost));
+    out->cell_rebuilds = rb;
     unsigned long long* dcount = c->d.counters + 1;
     HIPCHK(c, hipMemsetAsync(dcount, 0, sizeof(unsigned long long), c->stream));
     if (c->cap_entries) {
@@ -1410,7 +1586,7 @@ int pd_last_step_ms(pd_ctx* c, double* ms) {
   return PD_OK;
 }
 
-const char* pd_last_error(pd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* pd_last_error(pd_ctx* c) { return c ? c->err.c_str() : t_err.c_str(); }
 
 void pd_destroy(pd_ctx* c) {
   if (!c) return;

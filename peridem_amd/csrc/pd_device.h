@@ -12,7 +12,7 @@ constexpr uint32_t kBroken = 0x80000000u;   // high bit of a bond entry: bond er
 constexpr uint32_t kPad = 0xFFFFFFFFu;      // slice padding entry
 constexpr uint32_t kIdxMask = 0x7FFFFFFFu;
 constexpr int kSlice = 64;                  // nodes per bond slice (= wavefront width)
-constexpr int kMaxCand = 48;                // contact partners within R_c kept per node
+constexpr int kMaxCand = 48;                // contact candidates kept per node (list capacity)
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 
 // error bits raised by kernels, read back by the host after pd_step
@@ -35,7 +35,12 @@ struct DevPart {
 // slot): every kernel leaves its nodes untouched.
 struct DevInst {
   double Rc, Kn, mu, zeta;
-  double cell;              // contact cell edge (>= Rc)
+  double cell;              // cell edge of the candidate search (= rb)
+  double rb;                // candidate-list radius: Rc (1 + 1e-9) + skin
+  double du_lim2;           // rebuild when some |u - u_build|^2 exceeds this
+  unsigned long long du2;   // max over steps of |u - u_build|^2 since the build (bits of a double >= 0)
+  uint32_t rebuild;         // candidate lists are rebuilt in this step
+  uint32_t dirty;           // rebuild at the next step (set by the host)
   double dt;
   double wall[6];           // x_min, y_min, z_min, x_max, y_max, z_max
   double speed[6];
@@ -51,7 +56,7 @@ struct DevInst {
 // Per-launch control words shared by the kernels of a step.
 struct DevCtl {
   uint32_t steps_left;      // steps left in the current pd_step call
-  uint32_t pad;
+  uint32_t any_rebuild;     // some job rebuilds its candidate lists this step
 };
 
 struct Dev {
@@ -85,6 +90,11 @@ struct Dev {
   double* reaction;
   uint8_t* wmask;    // [N] planes in contact this step (reaction steps only)
   uint8_t* bflag;    // [N/256+1] block has a wall contact
+  // contact candidate lists (Verlet lists with a skin): entry k of node g at
+  // clist[(g >> 6) * 64 * kMaxCand + 64 k + (g & 63)], ccnt[g] entries
+  uint32_t* clist;
+  uint8_t* ccnt;
+  double4* UB;       // [N] displacement when the node's list was built
   // contact cell list
   uint32_t* ckey;    // [N]
   uint32_t* cstart;  // [T+1]

@@ -2,8 +2,10 @@
 //
 // One step (timeloop.h:395-797) is the launch sequence
 //   k_position   Verlet position half + per-particle box of CurrPos
+//   k_decide     per job: rebuild the contact candidate lists this step?
 //   k_pairs      particle-pair box test on the lagged boxes (timeloop.h:636-653)
-//   cell list    k_cellkey -> k_count -> scan -> k_scatter (contact candidates)
+//   cell list    k_clear_cells -> k_cellkey -> scan -> k_scatter -> k_build
+//                (only in steps where some job rebuilds its candidate lists)
 //   k_bond       PD bond force + fused irreversible break, per-node row sums
 //   k_contact    self/wall/ext/pair contact added in the reference's order
 //   k_update     Verlet velocity half, clamped nodes
@@ -35,10 +37,12 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
   // the block's wall-contact flag of this step starts clear
   if (threadIdx.x == 0 && g < d.N) d.bflag[g >> 8] = 0;
   double cp[3] = {0, 0, 0};
-  uint32_t p = 0xFFFFFFFFu;
+  uint32_t p = 0xFFFFFFFFu, jid = 0xFFFFFFFFu;
+  unsigned long long dkey = 0ull;
   if (g < d.N && live(d.inst[d.part[d.pid[g]].inst])) {
     p = d.pid[g];
-    const double dt = d.inst[d.part[p].inst].dt;
+    jid = d.part[p].inst;
+    const double dt = d.inst[jid].dt;
     double4 u = d.U[g];
     const double4 x = d.X[g];
     const double h = (dt * dt) * 0.5;
@@ -49,11 +53,26 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
     u.z = u.z + ((dt * v2) + (h * a2));
     d.U[g] = u;
     if (!(in_fast_range(u.x) & in_fast_range(u.y) & in_fast_range(u.z))) atomicOr(d.flags, kFlagSafeMath);
+    const double4 ub = d.UB[g];
+    const double e0 = u.x - ub.x, e1 = u.y - ub.y, e2 = u.z - ub.z;
+    dkey = (unsigned long long)__double_as_longlong((e0 * e0 + e1 * e1) + e2 * e2);
     cp[0] = x.x + u.x;
     cp[1] = x.y + u.y;
     cp[2] = x.z + u.z;
     if (!d.part[p].movable) p = 0xFFFFFFFFu;
   }
+  // largest displacement since the candidate lists were built, per job: a
+  // wavefront is one 64-node slice, and a slice belongs to one job
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(dkey, off, 64);
+    const uint32_t oj = __shfl_xor(jid, off, 64);
+    dkey = o > dkey ? o : dkey;
+    jid = oj < jid ? oj : jid;
+  }
+  // du2 is a running maximum since the last build: only a wave that exceeds
+  // the value it reads updates it, so a large job costs few atomics per step
+  if ((threadIdx.x & 63) == 0 && jid != 0xFFFFFFFFu && dkey > d.inst[jid].du2)
+    atomicMax(&d.inst[jid].du2, dkey);
   // block-level min/max per particle: nodes of one block belong to few
   // particles; reduce runs of equal particle id through LDS.
   int t = threadIdx.x;
@@ -92,6 +111,32 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
 }
 
 // ---------------------------------------------------------------------------
+// Contact candidates come from per-node Verlet lists: every node j of the
+// same job within rb = R_c (1 + 1e-9) + skin of node i when the list was
+// built.  While no node of the job has moved more than skin/2 since (less a
+// relative 1e-6 margin for rounding), every pair within R_c is in the lists,
+// so the exact reference tests applied to the list members find exactly the
+// reference's contact pairs.  One thread per job decides, from the largest
+// displacement since the build, whether its lists are rebuilt this step.
+__global__ void k_decide(Dev d) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.ninst) return;
+  DevInst& I = d.inst[q];
+  uint32_t r = 0;
+  if (live(I)) {
+    const double du2 = __longlong_as_double((long long)I.du2);
+    r = (I.dirty != 0) || (du2 > I.du_lim2);
+    I.dirty = 0;
+    if (r) {
+      atomicOr(&d.ctl->any_rebuild, 1u);
+      atomicAdd(&d.counters[2], 1ull);
+      I.du2 = 0ull;   // the lists built this step start from the current displacement
+    }
+  }
+  I.rebuild = r;
+}
+
+// ---------------------------------------------------------------------------
 // Pair box test on the lagged boxes, pair list in the reference's pair order
 // restricted to one instance (timeloop.h:243-255, 636-653).
 __global__ __launch_bounds__(256) void k_pairs(Dev d) {
@@ -118,17 +163,22 @@ __device__ __forceinline__ void cell_of(const double* cp, double cell, int64_t* 
   for (int k = 0; k < 3; k++) c[k] = (int64_t)floor(cp[k] / cell);
 }
 
-// Cell keys of the nodes that can take part in contact this step.
+// Cell list of the nodes of the jobs that rebuild their candidate lists.
+__global__ __launch_bounds__(256) void k_clear_cells(Dev d) {
+  if (!d.ctl->any_rebuild) return;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= d.T) d.cstart[i] = 0;
+}
+
 __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
+  if (!d.ctl->any_rebuild) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
   uint32_t p = d.pid[g];
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
-  // a node is a candidate for others when its particle has an active pair
-  // (pactive marks only stoppable ones, so test the pair flag bit separately)
   uint32_t key = kNoKey;
-  if (live(I) && (d.pactive[p] & 2u || self_active(P, I))) {
+  if (live(I) && I.rebuild) {
     double4 x = d.X[g];
     double4 u = d.U[g];
     double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
@@ -143,7 +193,8 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
 // exclusive scan of cstart[0..T) (counts) in place; cstart[T] = total.
 // Three phases: per-block scan, scan of block sums, add-back.
 constexpr int kScanBlock = 1024;
-__global__ __launch_bounds__(kScanBlock) void k_scan1(uint32_t* a, uint32_t n, uint32_t* sums) {
+__global__ __launch_bounds__(kScanBlock) void k_scan1(const DevCtl* ctl, uint32_t* a, uint32_t n, uint32_t* sums) {
+  if (!ctl->any_rebuild) return;
   __shared__ uint32_t s[kScanBlock];
   uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
   uint32_t v = i < n ? a[i] : 0;
@@ -158,7 +209,8 @@ __global__ __launch_bounds__(kScanBlock) void k_scan1(uint32_t* a, uint32_t n, u
   if (i < n) a[i] = s[threadIdx.x] - v;
   if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = s[threadIdx.x];
 }
-__global__ __launch_bounds__(kScanBlock) void k_scan2(uint32_t* sums, uint32_t nb, uint32_t* total) {
+__global__ __launch_bounds__(kScanBlock) void k_scan2(const DevCtl* ctl, uint32_t* sums, uint32_t nb, uint32_t* total) {
+  if (!ctl->any_rebuild) return;
   __shared__ uint32_t s[kScanBlock];
   uint32_t carry = 0;
   for (uint32_t base = 0; base < nb; base += kScanBlock) {
@@ -179,8 +231,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan2(uint32_t* sums, uint32_t n
   }
   if (threadIdx.x == 0) *total = carry;
 }
-__global__ __launch_bounds__(kScanBlock) void k_scan3(uint32_t* a, uint32_t n, const uint32_t* sums,
+__global__ __launch_bounds__(kScanBlock) void k_scan3(const DevCtl* ctl, uint32_t* a, uint32_t n, const uint32_t* sums,
                                                       uint32_t* cur) {
+  if (!ctl->any_rebuild) return;
   uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
   if (i < n) {
     uint32_t v = a[i] + sums[blockIdx.x];
@@ -190,12 +243,74 @@ __global__ __launch_bounds__(kScanBlock) void k_scan3(uint32_t* a, uint32_t n, c
 }
 
 __global__ __launch_bounds__(256) void k_scatter(Dev d) {
+  if (!d.ctl->any_rebuild) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
   uint32_t key = d.ckey[g];
   if (key == kNoKey) return;
   uint32_t slot = atomicAdd(&d.ccur[key], 1u);
   d.csorted[slot] = (uint32_t)g;
+}
+
+// Candidate list of node g: the nodes j != g of its job within rb, same
+// particle ones only while self-contact is on, sorted by the reference's
+// accumulation order (same particle first by node index, then the others by
+// global node index = particle order, then node order).
+__global__ __launch_bounds__(256) void k_build(Dev d) {
+  if (!d.ctl->any_rebuild) return;
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.N) return;
+  const uint32_t p = d.pid[g];
+  const DevPart& P = d.part[p];
+  const DevInst& I = d.inst[P.inst];
+  if (!live(I) || !I.rebuild) return;
+  const double4 xi = d.X[g];
+  const double4 ui = d.U[g];
+  d.UB[g] = ui;
+  const bool selfc = self_active(P, I);
+  const double cpi[3] = {xi.x + ui.x, xi.y + ui.y, xi.z + ui.z};
+  int64_t c[3];
+  cell_of(cpi, I.cell, c);
+  uint32_t cand[kMaxCand];
+  uint32_t ncand = 0;
+  uint32_t seen[27];
+  int nseen = 0;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const uint32_t key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
+        bool dup = false;
+        for (int q = 0; q < nseen; q++) dup |= (seen[q] == key);
+        if (dup) continue;
+        seen[nseen++] = key;
+        const uint32_t b0 = d.cstart[key], b1 = d.cstart[key + 1];
+        for (uint32_t it = b0; it < b1; it++) {
+          const uint32_t j = d.csorted[it];
+          if (j == g) continue;
+          const uint32_t pj = d.pid[j];
+          if (d.part[pj].inst != P.inst) continue;
+          if (pj == p && !selfc) continue;
+          const double4 xj = d.X[j];
+          const double4 uj = d.U[j];
+          const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
+                                 (uj.z - ui.z) + (xj.z - xi.z));
+          if (!(dn <= I.rb)) continue;
+          const uint32_t keyj = (pj == p ? 0u : kBroken) | j;
+          if (ncand == kMaxCand) {
+            atomicOr(d.err, kErrCandOverflow);
+            continue;
+          }
+          uint32_t w = ncand++;
+          while (w > 0 && cand[w - 1] > keyj) {
+            cand[w] = cand[w - 1];
+            w--;
+          }
+          cand[w] = keyj;
+        }
+      }
+  uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
+  for (uint32_t k = 0; k < ncand; k++) cl[(uint64_t)k * kSlice] = cand[k];
+  d.ccnt[g] = (uint8_t)ncand;
 }
 
 // Is j an intact bond of node g's row?  Rows hold the post-break state of
@@ -445,63 +560,39 @@ __global__ __launch_bounds__(256) void k_contact(Dev d) {
   uint32_t cand[kMaxCand];
   uint32_t ncand = 0;
   if (selfc || pairc) {
-    int64_t c[3];
-    cell_of(cpi, I.cell, c);
-    uint32_t seen[27];
-    int nseen = 0;
-    for (int dz = -1; dz <= 1; dz++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++) {
-          uint32_t key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
-          bool dup = false;
-          for (int q = 0; q < nseen; q++) dup |= (seen[q] == key);
-          if (dup) continue;
-          seen[nseen++] = key;
-          uint32_t b0 = d.cstart[key], b1 = d.cstart[key + 1];
-          for (uint32_t it = b0; it < b1; it++) {
-            uint32_t j = d.csorted[it];
-            if (j == g) continue;
-            uint32_t pj = d.pid[j];
-            const DevPart& Q = d.part[pj];
-            if (Q.inst != P.inst) continue;
-            if (pj == p) {
-              if (!selfc) continue;
-            } else {
-              if (!pairc) continue;
-              // pair (lo, hi) in the job's particle order, which is node order
-              const bool below = Q.node_off < P.node_off;
-              uint32_t lo = below ? pj : p, hi = below ? p : pj;
-              if (!(d.part[lo].movable || d.part[hi].movable)) continue;
-              if (!intersects_box_fext(d.lrtb + 6 * lo, d.lrtb + 6 * hi, I.Rc)) continue;
-            }
-            // exact reference distance test, in the reference orientation
-            const double4 xj = d.X[j];
-            const double4 uj = d.U[j];
-            double dd[3];
-            if (Q.node_off < P.node_off) {   // we are P_N's node j
-              dd[0] = (ui.x - uj.x) + (xi.x - xj.x);
-              dd[1] = (ui.y - uj.y) + (xi.y - xj.y);
-              dd[2] = (ui.z - uj.z) + (xi.z - xj.z);
-            } else {        // we are P_C's node i
-              dd[0] = (uj.x - ui.x) + (xj.x - xi.x);
-              dd[1] = (uj.y - ui.y) + (xj.y - xi.y);
-              dd[2] = (uj.z - ui.z) + (xj.z - xi.z);
-            }
-            double dn = nrm3(dd[0], dd[1], dd[2]);
-            if (!(dn <= I.Rc)) continue;
-            uint32_t keyj = (pj == p ? 0u : kBroken) | j;
-            if (ncand == kMaxCand) {
-              atomicOr(d.err, kErrCandOverflow);
-              continue;
-            }
-            uint32_t w = ncand++;
-            while (w > 0 && cand[w - 1] > keyj) {
-              cand[w] = cand[w - 1];
-              w--;
-            }
-            cand[w] = keyj;
-          }
-        }
+    const uint32_t n = d.ccnt[g];
+    const uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
+    for (uint32_t k = 0; k < n; k++) {
+      const uint32_t keyj = cl[(uint64_t)k * kSlice];
+      const uint32_t j = keyj & kIdxMask;
+      const uint32_t pj = d.pid[j];
+      const DevPart& Q = d.part[pj];
+      if (!(keyj & kBroken)) {
+        if (!selfc) continue;
+      } else {
+        if (!pairc) continue;
+        // pair (lo, hi) in the job's particle order, which is node order
+        const bool below = Q.node_off < P.node_off;
+        const uint32_t lo = below ? pj : p, hi = below ? p : pj;
+        if (!(d.part[lo].movable || d.part[hi].movable)) continue;
+        if (!intersects_box_fext(d.lrtb + 6 * lo, d.lrtb + 6 * hi, I.Rc)) continue;
+      }
+      // exact reference distance test, in the reference orientation
+      const double4 xj = d.X[j];
+      const double4 uj = d.U[j];
+      double dd[3];
+      if (Q.node_off < P.node_off) {   // we are P_N's node j
+        dd[0] = (ui.x - uj.x) + (xi.x - xj.x);
+        dd[1] = (ui.y - uj.y) + (xi.y - xj.y);
+        dd[2] = (ui.z - uj.z) + (xi.z - xj.z);
+      } else {                         // we are P_C's node i
+        dd[0] = (uj.x - ui.x) + (xj.x - xi.x);
+        dd[1] = (uj.y - ui.y) + (xj.y - xi.y);
+        dd[2] = (uj.z - ui.z) + (xj.z - xi.z);
+      }
+      if (!(nrm3(dd[0], dd[1], dd[2]) <= I.Rc)) continue;
+      cand[ncand++] = keyj;   // the list is sorted, so is its subsequence
+    }
   }
 
   uint32_t ci = 0;
@@ -628,7 +719,10 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d) {
       I.remaining -= 1;
     }
   }
-  if (q == 0 && d.ctl->steps_left) d.ctl->steps_left -= 1;
+  if (q == 0) {
+    if (d.ctl->steps_left) d.ctl->steps_left -= 1;
+    d.ctl->any_rebuild = 0;
+  }
 }
 
 // Wall reaction of one step in the reference order (particle, node, plane,

@@ -22,6 +22,7 @@ from __future__ import annotations
 import hashlib
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -139,13 +140,16 @@ class Server:
 
     chunk: most steps per pd_step call; a call never runs past the earliest
     budget end among the resident jobs, so finished jobs leave (and waiting
-    ones enter) at the next call boundary."""
+    ones enter) at the next call boundary.
+    prepare_ahead: requests laid out on the host (pd_job_prepare, on a worker
+    thread) while the pool steps; admission then only places them."""
     max_nodes: int
     max_bond_entries: int
     max_particles: int
     max_jobs: int
     chunk: int = 256
     digests: bool = True
+    prepare_ahead: int = 2
     pool: abi.Pool = field(init=False)
     queue: deque = field(init=False, default_factory=deque)
     live: dict = field(init=False, default_factory=dict)
@@ -156,24 +160,44 @@ class Server:
         self.steps_run = 0
         self.device_ms = 0.0
         self.peak_live = 0
+        self.workers = ThreadPoolExecutor(max_workers=1) if self.prepare_ahead else None
 
     def close(self):
+        if self.workers:
+            self.workers.shutdown(wait=True)
         self.pool.close()
 
     def submit(self, rid, system, nsteps, arrival):
-        self.queue.append((rid, arrival, system, nsteps))
+        self.queue.append([rid, arrival, system, nsteps, None])
+        self._prefetch()
+
+    def _prefetch(self):
+        if not self.workers:
+            return
+        for q in list(self.queue)[:self.prepare_ahead]:
+            if q[4] is None:
+                q[4] = self.workers.submit(abi.prepare_job, q[2])
 
     def admit(self, now):
         """Admit waiting requests in arrival order while they fit."""
         while self.queue:
-            rid, arrival, system, nsteps = self.queue[0]
-            job = self.pool.try_add_job(system, nsteps)
+            rid, arrival, system, nsteps, fut = self.queue[0]
+            if fut is None:
+                job = self.pool.try_add_job(system, nsteps)
+            else:
+                if self.live and not fut.done():
+                    break          # keep stepping; place it once it is laid out
+                blob = fut.result()
+                job = self.pool.add_prepared(blob, system, nsteps)
+                if job is not None:
+                    blob.free()
             if job is None:
                 if not self.live:
                     raise RuntimeError(f"request {rid} does not fit an empty pool")
                 break
             self.queue.popleft()
             self.live[job] = (rid, arrival, now, system, nsteps)
+            self._prefetch()
         self.peak_live = max(self.peak_live, len(self.live))
 
     def advance(self):
