@@ -50,18 +50,25 @@ def log(msg):
 WORKLOAD = dict(sphere_speed=0.5, node_speed=0.01)
 
 
-def traffic_bytes(nodes, bonds):
-    """HBM bytes per k_bond launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, when it was taken on this same workload."""
+def traffic_bytes(nodes, bonds, kernel):
+    """HBM bytes per bond-kernel launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary, when it was taken on this same workload
+    with the same kernel shape."""
     path = os.path.join(ROOT, "profiles", "k_bond_gravity_settling_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("nodes") != nodes or t.get("movable_bonds") != bonds:
+    if t.get("nodes") != nodes or t.get("movable_bonds") != bonds or t.get("bond_kernel") != kernel:
         return None
     return t.get("hbm_bytes_per_launch")
+
+
+def shape_name(k):
+    from peridem_amd import abi
+    u, pipe, mask = abi.bond_shapes()[k]
+    return f"pd::k_bond_t<{u}, {'true' if pipe else 'false'}, {'true' if mask else 'false'}>"
 
 
 def cpu_baseline(nspheres, steps, threads):
@@ -237,6 +244,7 @@ def main():
     log(f"[bench] phases per step (ms): " +
         ", ".join(f"{k}={v['ms'] / max(v['launches'], 1):.4f}" for k, v in prof.items()))
     cnt = e.counters()
+    kname = shape_name(e.bond_shape())
     e.close()
 
     if probe is not None:
@@ -276,8 +284,8 @@ def main():
                    "step_hbm_fraction": step_bytes / (step_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
                    "phase_ms": {k: v["ms"] / max(v["launches"], 1) for k, v in prof.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(N, Bm),
-                     "kernel": "k_bond", "bytes_per_launch": force_bytes,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(N, Bm, kname),
+                     "kernel": kname, "bytes_per_launch": force_bytes,
                      "avg_launch_ms": force_ms},
         "cpu_baseline": cpu,
         "batched": bat,

@@ -22,7 +22,8 @@ def _rows(d, name):
 
 
 def _short(k):
-    return k.split("(")[0]
+    k = k.split("(")[0]
+    return k[5:] if k.startswith("void ") else k
 
 
 def trace(d):
@@ -70,8 +71,12 @@ def traffic(fd, wd, copy_mib, nodes, bonds):
         wb = sum(w[k]["WRITE_SIZE"]) / len(w[k]["WRITE_SIZE"]) * 1024 * ws
         res[k] = fb + wb
         print(f"| `{k}` | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} |")
+    # the bond kernel: the shape with the most dispatches (the autotuner
+    # times every shape a few times before the run settles on one)
+    bond = [k for k in f if k.startswith("pd::k_bond")]
+    bond = max(bond, key=lambda k: len(f[k]["FETCH_SIZE"])) if bond else None
     return {"nodes": nodes, "movable_bonds": bonds, "fetch_scale": fs, "write_scale": ws,
-            "hbm_bytes_per_launch": res.get("pd::k_bond"),
+            "bond_kernel": bond, "hbm_bytes_per_launch": res.get(bond),
             "per_kernel_bytes": res}
 
 

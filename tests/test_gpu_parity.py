@@ -158,3 +158,12 @@ def test_autotune_leaves_state_untouched(oracle_lib):
     finally:
         g.close()
         o.close()
+
+
+def test_candidate_lists_rebuild_and_match_oracle(oracle_lib):
+    """Contact candidates come from lists rebuilt when a node has moved half
+    the skin: a fast impact rebuilds several times and reuses the lists in
+    between, and still matches the oracle's all-pairs search bit for bit."""
+    s = small_impact(seed=17, v=150.0)
+    cg, co = run_pair(s, [40, 90, 160], oracle_lib, "candidate lists")
+    assert 3 <= cg["cell_rebuilds"] < 160, cg["cell_rebuilds"]
