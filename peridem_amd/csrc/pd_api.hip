@@ -216,12 +216,11 @@ static void fill_inst(const pd_run_desc& r, DevInst& D) {
   D.gradient_extforce = r.gradient_extforce;
   D.compute_reaction = r.compute_wall_reaction;
   D.extforce_maxstep = r.extforce_maxstep;
-  D.pad0 = 0;
+  D.enable_torque = r.enable_torque;
 }
 
 static int check_run(pd_ctx* c, const pd_run_desc& r) {
   if (r.nl_bdry_only) return fail(c, PD_E_UNSUPPORTED, "nl_bdry_only is not built yet");
-  if (r.enable_torque) return fail(c, PD_E_UNSUPPORTED, "enable_torque is not built yet");
   if (!(r.dt > 0)) return fail(c, PD_E_ARG, "dt must be positive");
   return PD_OK;
 }
@@ -246,6 +245,7 @@ static int check_particles(pd_ctx* c, const pd_particle_desc* P, uint32_t n) {
   for (uint32_t i = 0; i < n; i++) {
     const pd_particle_desc& d = P[i];
     if (d.nnodes == 0 || !d.pos || !d.vol) return fail(c, PD_E_ARG, "particle %u needs nnodes, pos, vol", i);
+    if (d.torque_axis > 2) return fail(c, PD_E_ARG, "particle %u: torque_axis must be 0, 1 or 2", i);
     if (d.nconn && !d.conn) return fail(c, PD_E_ARG, "particle %u: nconn > 0 but conn is NULL", i);
     if (bad[i] != ~0ull)
       return fail(c, PD_E_ARG, "particle %u: Connectivity entry %llu out of range", i, (unsigned long long)bad[i]);
@@ -330,6 +330,12 @@ static bool any_contact_possible(pd_ctx* c) {
   if (c->d.npairs) return true;
   for (auto& J : c->jobs)
     if (J.used && J.run.self_contact && J.run.enable_fracture) return true;
+  return false;
+}
+
+static bool any_torque(pd_ctx* c) {
+  for (auto& J : c->jobs)
+    if (J.used && J.run.enable_torque) return true;
   return false;
 }
 
@@ -494,6 +500,7 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(part, P);
   ALLOC(inst, I);
   ALLOC(lrtb, 6 * P);
+  ALLOC(cent, 3 * P);
   ALLOC(lrtb_next, 6 * P);
   ALLOC(pactive, P);
   ALLOC(counters, 8);
@@ -670,6 +677,8 @@ int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t n
     D.breakable = d.breakable;
     D.stoppable = d.stoppable;
     D.rows_sorted = rows_sorted(R, R.poff[i], R.poff[i] + d.nnodes);
+    D.tval = d.torque_val;
+    D.taxis = d.torque_axis;
     bool slow = false;
     double o[6] = {0, 0, 0, 0, 0, 0};
     for (uint32_t v = 0; v < d.nnodes; v++) {
@@ -968,6 +977,7 @@ static int launch_step(pd_ctx* c) {
   hipLaunchKernelGGL(k_position, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if ((rc = mark(1))) return rc;
   hipLaunchKernelGGL(k_decide, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
+  if (any_torque(c)) hipLaunchKernelGGL(k_centroid, dim3(grid_of(d.P, 64)), dim3(64), 0, st, d);
   if (d.npairs) hipLaunchKernelGGL(k_pairs, dim3(grid_of(d.npairs, 256)), dim3(256), 0, st, d);
   if (any_contact_possible(c)) {
     // every kernel of the list rebuild exits at once unless a job rebuilds
@@ -1247,6 +1257,7 @@ static int job_set_flag(pd_ctx* c, uint32_t j, int flag, int value) {
     D.allow_damping = r.allow_damping;
     D.allow_friction = r.allow_friction;
     D.compute_reaction = r.compute_wall_reaction;
+    D.enable_torque = r.enable_torque;
     D.dirty = 1;   // self-contact eligibility may have changed
   });
   c->graph_dirty = true;

@@ -27,6 +27,8 @@ struct DevPart {
   uint32_t inst;
   double rho, rinv, cnot, snot;   // rinv = 1/rho as the reference forms it (timeloop.h:673)
   int32_t movable, breakable, stoppable, rows_sorted;
+  double tval;                    // torque_val, torque_axis (particle2.h:57-59)
+  uint32_t taxis, pad;
 };
 
 // Per-job (independent system) parameters: Contact + RectWall
@@ -50,7 +52,7 @@ struct DevInst {
   int32_t allow_wall, allow_damping, allow_friction, self_contact;
   int32_t enable_fracture, gradient_extforce, compute_reaction;
   uint32_t extforce_maxstep;
-  uint32_t pad0;
+  int32_t enable_torque;
 };
 
 // Per-launch control words shared by the kernels of a step.
@@ -79,6 +81,7 @@ struct Dev {
   DevPart* part;
   DevInst* inst;
   double* lrtb;      // [P*6] lagged boxes (get_minmax order)
+  double* cent;      // [P*3] mean CurrPos of a particle (torque steps only)
   unsigned long long* lrtb_next;  // [P*6] ordered-key min/max accumulators
   uint32_t* pactive; // [P] particle has an active contact pair this step
   uint32_t* pairs;   // [npairs*2] particle pairs (i<j) of the same instance

@@ -137,6 +137,29 @@ __global__ void k_decide(Dev d) {
 }
 
 // ---------------------------------------------------------------------------
+// mean_CurrPos (particle2.h:94-103) of every particle of a job with torque on:
+// one thread per particle sums pos + disp in node order, as the reference.
+__global__ __launch_bounds__(64) void k_centroid(Dev d) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.P) return;
+  const DevPart& P = d.part[q];
+  const DevInst& I = d.inst[P.inst];
+  if (!live(I) || !I.enable_torque) return;
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+  for (uint64_t g = P.node_off; g < P.node_off + P.nnodes; g++) {
+    const double4 x = d.X[g];
+    const double4 u = d.U[g];
+    m0 += (x.x + u.x);
+    m1 += (x.y + u.y);
+    m2 += (x.z + u.z);
+  }
+  const double n = (double)P.nnodes;
+  d.cent[3 * q] = m0 / n;
+  d.cent[3 * q + 1] = m1 / n;
+  d.cent[3 * q + 2] = m2 / n;
+}
+
+// ---------------------------------------------------------------------------
 // Pair box test on the lagged boxes, pair list in the reference's pair order
 // restricted to one instance (timeloop.h:243-255, 636-653).
 __global__ __launch_bounds__(256) void k_pairs(Dev d) {
@@ -642,6 +665,28 @@ __global__ __launch_bounds__(256) void k_contact(Dev d) {
       f[1] += d.E[3 * g + 1];
       f[2] += d.E[3 * g + 2];
     }
+  }
+
+  // (d') external torque about the centroid, every particle (timeloop.h:601-633)
+  if (I.enable_torque) {
+    const uint32_t ta = P.taxis;
+    double r[3] = {cpi[0] - d.cent[3 * p], cpi[1] - d.cent[3 * p + 1], cpi[2] - d.cent[3 * p + 2]};
+    const double rn = nrm3(r[0], r[1], r[2]);
+    const double rinv = rn != 0 ? 1 / rn : 0;
+    r[ta] = 0;
+    double rp[3];
+    rp[ta] = 0;
+    rp[(ta + 1) % 3] = -r[(ta + 2) % 3];
+    rp[(ta + 2) % 3] = r[(ta + 1) % 3];
+    const double pn = nrm3(rp[0], rp[1], rp[2]);
+    if (pn > 0) {
+      for (int k = 0; k < 3; k++) rp[k] = rp[k] / pn;
+    } else {
+      rp[(ta + 1) % 3] = 0;
+      rp[(ta + 2) % 3] = 0;
+    }
+    const double sc = rinv * P.tval;
+    for (int k = 0; k < 3; k++) f[k] += sc * rp[k];
   }
 
   // (e) inter-particle contact in pair order (timeloop.h:636-651)

@@ -264,8 +264,6 @@ static int run(Config& C) {
                   C.speed_wall_x_max, C.speed_wall_y_max, C.speed_wall_z_max};
   for (int k = 0; k < 6; k++) W.speed[k] = sp[k];
 
-  if (C.enable_torque || C.set_torque_timestep != -1)
-    throw std::runtime_error("enable_torque is not supported by the GPU library yet");
   if (C.particle_w_fixed_velocity != -1 && C.set_fix_velocity_timestep != -1)
     throw std::runtime_error("prescribed velocity (particle_w_fixed_velocity) is not supported yet");
   if (C.brazil || C.reset_partzero_y)
@@ -375,12 +373,17 @@ static int run(Config& C) {
 
   // the steps at which the host acts before the step (timeloop.h:405-535)
   auto is_switch = [&](unsigned t) {
-    for (int s : {C.set_fracture_timestep, C.set_self_contact_timestep, C.set_damping_timestep,
-                  C.set_zero_wall_speed_timestep, C.set_movable_timestep, C.set_stoppable_timestep})
+    for (int s : {C.set_torque_timestep, C.set_fracture_timestep, C.set_self_contact_timestep,
+                  C.set_damping_timestep, C.set_zero_wall_speed_timestep, C.set_movable_timestep,
+                  C.set_stoppable_timestep})
       if (s != -1 && (unsigned)s == t) return true;
     return false;
   };
   auto apply_switches = [&](unsigned t) {
+    if (C.set_torque_timestep != -1 && t == (unsigned)C.set_torque_timestep) {
+      check(ctx, pd_set_flag(ctx, PD_FLAG_ENABLE_TORQUE, 1), "pd_set_flag");
+      std::cout << "Setting torque to 1 on timestep " << t << std::endl;
+    }
     if (C.set_fracture_timestep != -1 && t == (unsigned)C.set_fracture_timestep) {
       check(ctx, pd_set_flag(ctx, PD_FLAG_ENABLE_FRACTURE, 1), "pd_set_flag");
       std::cout << "Setting fracture to 1 on timestep " << t << std::endl;

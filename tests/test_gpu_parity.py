@@ -167,3 +167,27 @@ def test_candidate_lists_rebuild_and_match_oracle(oracle_lib):
     s = small_impact(seed=17, v=150.0)
     cg, co = run_pair(s, [40, 90, 160], oracle_lib, "candidate lists")
     assert 3 <= cg["cell_rebuilds"] < 160, cg["cell_rebuilds"]
+
+
+def test_torque_matches_oracle(oracle_lib):
+    """External torque about each particle's centroid (timeloop.h:601-633), on
+    all four axes settings incl. a non-movable particle, switched on mid-run
+    like set_torque_timestep."""
+    s = mixed_system(seed=21)
+    for p, (ax, tv) in zip(s.particles, [(2, 3.0e9), (0, -2.0e9), (1, 5.0e9), (2, 0.0)]):
+        p.torque_axis, p.torque_val = ax, tv
+    g = abi.gpu_engine(s)
+    o = oracle_lib.engine(s)
+    try:
+        for eng in (g, o):
+            eng.step(12)
+            eng.set_flag(abi.PD_FLAG_ENABLE_TORQUE, 1)
+        compare_engines(g, o, s, "torque switch")
+        for c in (20, 60):
+            g.step(c)
+            o.step(c)
+            rg, ro = compare_engines(g, o, s, f"torque +{c}")
+            assert_bitwise(rg, ro, f"torque +{c} reaction")
+    finally:
+        g.close()
+        o.close()
