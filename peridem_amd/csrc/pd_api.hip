@@ -217,10 +217,11 @@ static void fill_inst(const pd_run_desc& r, DevInst& D) {
   D.compute_reaction = r.compute_wall_reaction;
   D.extforce_maxstep = r.extforce_maxstep;
   D.enable_torque = r.enable_torque;
+  D.nl_bdry_only = r.nl_bdry_only;
+  D.pad1 = 0;
 }
 
 static int check_run(pd_ctx* c, const pd_run_desc& r) {
-  if (r.nl_bdry_only) return fail(c, PD_E_UNSUPPORTED, "nl_bdry_only is not built yet");
   if (!(r.dt > 0)) return fail(c, PD_E_ARG, "dt must be positive");
   return PD_OK;
 }
@@ -492,6 +493,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(F, 3 * N);
   ALLOC(E, 3 * N);
   ALLOC(pid, N);
+  ALLOC(brank, N);
+  ALLOC(blist, N);
   ALLOC(nflag, N);
   ALLOC(deg, N);
   ALLOC(soff, S);
@@ -531,6 +534,7 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipMemsetAsync(d.F, 0, 3 * N * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.E, 0, 3 * N * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.nflag, 0, N, st));
+  HIPCHK(c, hipMemsetAsync(d.brank, 0xFF, N * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.deg, 0, N * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.soff, 0, S * sizeof(uint64_t), st));
   HIPCHK(c, hipMemsetAsync(d.swid, 0, S * sizeof(uint32_t), st));
@@ -585,6 +589,9 @@ struct pd_job_blob {
   std::vector<double4> X, U;
   std::vector<double> V, A, E;
   std::vector<uint32_t> lpid;     // per node slot: particle index in the job, ~0 = void
+  std::vector<uint32_t> brank;    // per node slot: rank in its boundary list (nl_bdry_only jobs)
+  std::vector<uint32_t> blist;    // boundary lists, job-relative node indices, particle by particle
+  std::vector<uint64_t> bofs;     // per particle: first entry of its list in blist
   std::vector<uint8_t> nflag;
   std::vector<DevPart> dparts;    // node_off job-relative
   std::vector<double> lrtb;
@@ -632,6 +639,25 @@ int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t n
   B->A.assign(3 * Np, 0.0);
   B->E.assign(3 * Np, 0.0);
   B->lpid.assign(Np, ~0u);
+  B->brank.assign(Np, kNoKey);
+  B->bofs.assign(np, 0);
+  if (run->nl_bdry_only) {
+    // boundary lists (load_hdf5.h bdry_nodes), in list order; a node listed
+    // twice would be visited twice by the reference loops
+    for (uint32_t i = 0; i < np; i++) {
+      B->bofs[i] = B->blist.size();
+      for (uint32_t r = 0; r < P[i].nbdry; r++) {
+        const uint64_t g = R.poff[i] + P[i].bdry[r];
+        if (B->brank[g] != kNoKey) {
+          delete B;
+          t_err = "nl_bdry_only with a node listed twice in bdry_nodes is not supported";
+          return PD_E_UNSUPPORTED;
+        }
+        B->brank[g] = r;
+        B->blist.push_back((uint32_t)g);
+      }
+    }
+  }
   B->nflag.assign(Np, 0);
   B->dparts.resize(np);
   B->lrtb.resize(6 * np);
@@ -679,6 +705,8 @@ int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t n
     D.rows_sorted = rows_sorted(R, R.poff[i], R.poff[i] + d.nnodes);
     D.tval = d.torque_val;
     D.taxis = d.torque_axis;
+    D.nbdry = run->nl_bdry_only ? d.nbdry : 0;
+    D.boff = B->bofs[i];
     bool slow = false;
     double o[6] = {0, 0, 0, 0, 0, 0};
     for (uint32_t v = 0; v < d.nnodes; v++) {
@@ -803,7 +831,10 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
   for (uint32_t i = 0; i < np; i++) {
     dparts[i].node_off = g0 + B->poff[i];
     dparts[i].inst = jid;
+    dparts[i].boff = g0 + B->bofs[i];
   }
+  std::vector<uint32_t> blist(B->blist);
+  for (auto& b : blist) b += (uint32_t)g0;
   DevInst I;
   memset(&I, 0, sizeof I);
   fill_inst(B->run, I);
@@ -825,6 +856,8 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
   HIPCHK(c, hipMemsetAsync(d.F + 3 * g0, 0, 3 * Np * sizeof(double), c->stream));
   UP(d.pid + g0, pid.data(), Np);
   UP(d.nflag + g0, B->nflag.data(), Np);
+  UP(d.brank + g0, B->brank.data(), Np);
+  if (!blist.empty()) UP(d.blist + g0, blist.data(), blist.size());
   UP(d.deg + g0, B->deg.data(), Np);
   UP(d.soff + s0, soff.data(), B->nslices);
   UP(d.swid + s0, B->width.data(), B->nslices);

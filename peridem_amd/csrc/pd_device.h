@@ -28,7 +28,9 @@ struct DevPart {
   double rho, rinv, cnot, snot;   // rinv = 1/rho as the reference forms it (timeloop.h:673)
   int32_t movable, breakable, stoppable, rows_sorted;
   double tval;                    // torque_val, torque_axis (particle2.h:57-59)
-  uint32_t taxis, pad;
+  uint32_t taxis;
+  uint32_t nbdry;                 // boundary_nodes: blist[boff .. boff + nbdry) in list order
+  uint64_t boff;
 };
 
 // Per-job (independent system) parameters: Contact + RectWall
@@ -53,6 +55,8 @@ struct DevInst {
   int32_t enable_fracture, gradient_extforce, compute_reaction;
   uint32_t extforce_maxstep;
   int32_t enable_torque;
+  int32_t nl_bdry_only;     // contact and wall terms on boundary nodes only (contact.h:751-760)
+  uint32_t pad1;
 };
 
 // Per-launch control words shared by the kernels of a step.
@@ -70,6 +74,8 @@ struct Dev {
   double* F;         // force density [3N]
   double* E;         // extforce [3N]
   uint32_t* pid;     // particle of node
+  uint32_t* brank;   // position of the node in its particle's boundary list, kNoKey if absent
+  uint32_t* blist;   // boundary lists, stored in the node range of their job
   uint8_t* nflag;    // bit0 clamped
   uint32_t* deg;     // row length at commit
   // bonds: sliced rows, 64 nodes per slice, entry k of lane l at

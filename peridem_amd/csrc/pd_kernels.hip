@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
   uint32_t key = kNoKey;
-  if (live(I) && I.rebuild) {
+  if (live(I) && I.rebuild && (!I.nl_bdry_only || d.brank[g] != kNoKey)) {
     double4 x = d.X[g];
     double4 u = d.U[g];
     double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
@@ -290,11 +290,16 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
   const double4 xi = d.X[g];
   const double4 ui = d.U[g];
   d.UB[g] = ui;
+  if (I.nl_bdry_only && d.brank[g] == kNoKey) {   // not a boundary node: no contact
+    d.ccnt[g] = 0;
+    return;
+  }
   const bool selfc = self_active(P, I);
   const double cpi[3] = {xi.x + ui.x, xi.y + ui.y, xi.z + ui.z};
   int64_t c[3];
   cell_of(cpi, I.cell, c);
   uint32_t cand[kMaxCand];
+  uint64_t ckey[kMaxCand];   // order key: other particle, particle order, rank in the particle
   uint32_t ncand = 0;
   uint32_t seen[27];
   int nseen = 0;
@@ -311,24 +316,32 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
           const uint32_t j = d.csorted[it];
           if (j == g) continue;
           const uint32_t pj = d.pid[j];
-          if (d.part[pj].inst != P.inst) continue;
+          const DevPart& Q = d.part[pj];
+          if (Q.inst != P.inst) continue;
           if (pj == p && !selfc) continue;
           const double4 xj = d.X[j];
           const double4 uj = d.U[j];
           const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
                                  (uj.z - ui.z) + (xj.z - xi.z));
           if (!(dn <= I.rb)) continue;
-          const uint32_t keyj = (pj == p ? 0u : kBroken) | j;
+          // the reference visits partners particle by particle (pair order =
+          // node order of the particles), each in node order or, with
+          // nl_bdry_only, in boundary-list order
+          const uint32_t rank = I.nl_bdry_only ? d.brank[j] : (uint32_t)(j - Q.node_off);
+          const uint64_t key =
+              ((uint64_t)(pj != p) << 63) | ((uint64_t)Q.node_off << 32) | (uint64_t)rank;
           if (ncand == kMaxCand) {
             atomicOr(d.err, kErrCandOverflow);
             continue;
           }
           uint32_t w = ncand++;
-          while (w > 0 && cand[w - 1] > keyj) {
+          while (w > 0 && ckey[w - 1] > key) {
             cand[w] = cand[w - 1];
+            ckey[w] = ckey[w - 1];
             w--;
           }
-          cand[w] = keyj;
+          cand[w] = (pj == p ? 0u : kBroken) | j;
+          ckey[w] = key;
         }
       }
   uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
@@ -642,7 +655,7 @@ __global__ __launch_bounds__(256) void k_contact(Dev d) {
   while (ci < ncand && !(cand[ci] & kBroken)) ci++;
 
   // (c) wall contact (timeloop.h:561-570, contact.h:1060-1124)
-  if (P.movable && P.stoppable && I.allow_wall &&
+  if (P.movable && P.stoppable && I.allow_wall && (!I.nl_bdry_only || d.brank[g] != kNoKey) &&
       !within_interior_fext(d.lrtb + 6 * p, I.wall, I.Rc)) {
     uint32_t m = wall_node(I, dt, cpi, vi, xi.w, P.rho, f, nullptr);
     if (reaction_step(d, I)) {
@@ -782,6 +795,26 @@ __global__ void k_reaction(Dev d) {
   double reac[18];
   for (int k = 0; k < 18; k++) reac[k] = 0.0;
   const uint64_t g0 = I.node0, g1 = I.node1;
+  if (I.nl_bdry_only) {
+    // particle by particle, each in boundary-list order (contact.h:1065-1069)
+    for (uint64_t g = g0; g < g1;) {
+      const DevPart& P = d.part[d.pid[g]];
+      for (uint32_t r = 0; r < P.nbdry; r++) {
+        const uint32_t h = d.blist[P.boff + r];
+        if (!d.wmask[h]) continue;
+        const double4 x = d.X[h];
+        const double4 u = d.U[h];
+        const double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
+        const double v[3] = {d.V[3 * h], d.V[3 * h + 1], d.V[3 * h + 2]};
+        double fdummy[3] = {0, 0, 0};
+        wall_node(I, I.dt, cp, v, x.w, P.rho, fdummy, reac);
+        d.wmask[h] = 0;
+      }
+      g = P.node_off + P.nnodes;
+    }
+    for (int k = 0; k < 18; k++) d.reaction[18 * q + k] = reac[k];
+    return;
+  }
   for (uint64_t g = g0; g < g1;) {
     uint64_t blk = g >> 8;
     if (!d.bflag[blk]) {

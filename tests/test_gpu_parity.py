@@ -191,3 +191,16 @@ def test_torque_matches_oracle(oracle_lib):
     finally:
         g.close()
         o.close()
+
+
+def test_nl_bdry_only_matches_oracle(oracle_lib):
+    """Contact and wall terms on boundary nodes only, accumulated in the order
+    of each particle's boundary list (shuffled here), self-contact included."""
+    s = small_impact(seed=23)
+    s.nl_bdry_only = 1
+    rng = np.random.default_rng(5)
+    for p in s.particles:
+        deg = np.bincount(p.conn.reshape(-1), minlength=p.n)
+        surf = np.nonzero(deg < deg.max())[0].astype(np.uint32)
+        p.bdry = rng.permutation(surf)
+    run_pair(s, [30, 90, 150], oracle_lib, "nl_bdry_only", reaction=True)

@@ -339,3 +339,55 @@ def test_rank_emulation_matches_single_rank(oracle_lib):
         a, b = e1.state(p), e4.state(p)
         scale = np.abs(a["force"]).max()
         assert np.allclose(a["force"], b["force"], rtol=1e-9, atol=1e-9 * scale)
+
+
+# ---------------------------------------------------------------- nl_bdry_only
+def _impact_with_walls():
+    s = gen.two_sphere_impact(R=3e-4, h=1e-4, v=150.0, seed=3, gap_h=1.0, b_movable=1,
+                              self_contact=1)
+    s.allow_wall, s.compute_wall_reaction = 1, 1
+    s.wall_lrtb = (-3.4e-4, -3.5e-4, -3.6e-4, 10.6e-4, 3.5e-4, 3.6e-4)
+    s.wall_speed = (0.0, 4.0, 0.0, -20.0, 0.0, -2.0)
+    return s
+
+
+def test_nl_bdry_only_with_every_node_listed_is_the_full_contact(oracle_lib):
+    """boundary_nodes = all nodes in node order makes the nl_bdry_only loops
+    (contact.h:751-760, 1065-1069) the full loops: bitwise the same run."""
+    full = _impact_with_walls()
+    nl = _impact_with_walls()
+    nl.nl_bdry_only = 1
+    for p in nl.particles:
+        p.bdry = np.arange(p.n, dtype=np.uint32)
+    a, b = oracle_lib.engine(full), oracle_lib.engine(nl)
+    a.step(220)
+    b.step(220)
+    for p in range(2):
+        sa, sb = a.state(p), b.state(p)
+        for k in ("disp", "vel", "force"):
+            assert np.array_equal(sa[k], sb[k]), k
+    assert np.array_equal(a.wall()[1], b.wall()[1])
+    assert a.counters()["pair_computations"] > 0
+    a.close()
+    b.close()
+
+
+def test_nl_bdry_only_skips_unlisted_nodes(oracle_lib):
+    """A node missing from its particle's boundary list takes no contact and
+    no wall terms, in either role of the pair."""
+    for listed in ((0, 1), (1, 0), (0, 0)):
+        A, B, s = two_nodes(6e-5, nl_bdry_only=1, allow_wall=1, compute_wall_reaction=1,
+                            wall_lrtb=(-5e-5, -1, -1, 1, 1, 1))
+        A.bdry = np.arange(listed[0], dtype=np.uint32)
+        B.bdry = np.arange(listed[1], dtype=np.uint32)
+        e, st = force_after_one_step(oracle_lib, s)
+        assert np.all(st[0]["force"] == 0) and np.all(st[1]["force"] == 0), listed
+        assert listed[0] or np.all(e.wall()[1] == 0)
+        e.close()
+    # listed on both sides: the two-node closed form again
+    A, B, s = two_nodes(6e-5, nl_bdry_only=1)
+    A.bdry = np.zeros(1, np.uint32)
+    B.bdry = np.zeros(1, np.uint32)
+    e, st = force_after_one_step(oracle_lib, s)
+    assert st[0]["force"][0, 0] == pytest.approx(4.6e31 * 9e-5 * ((6e-5 - 9e-5) / 9e-5) * 5e-12, rel=1e-14)
+    e.close()
