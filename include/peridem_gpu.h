@@ -128,6 +128,7 @@ int pd_step(pd_ctx* ctx, uint32_t nsteps);
 #define PD_FLAG_ALLOW_FRICTION 4
 #define PD_FLAG_ENABLE_TORQUE 5
 #define PD_FLAG_COMPUTE_WALL_REACTION 6
+#define PD_FLAG_VELOCITY_CONSTRAINT 7   /* TL.enable_velocity_constraint (set_fix_velocity_timestep) */
 int pd_set_flag(pd_ctx* ctx, int flag, int value);
 int pd_set_wall_speed(pd_ctx* ctx, const double speed[6]);
 int pd_set_wall(pd_ctx* ctx, const double lrtb[6]);
@@ -137,6 +138,15 @@ int pd_set_wall(pd_ctx* ctx, const double lrtb[6]);
 int pd_set_particle_flag(pd_ctx* ctx, uint32_t p, int which, int value);
 /* TL.new_snot override (timeloop.h:314-318). */
 int pd_set_snot(pd_ctx* ctx, uint32_t p, double snot);
+/* Prescribed velocity of one particle (timeloop.h:685-764): the particle
+ * CFGV.particle_w_fixed_velocity with fix_velocity_mean_x /
+ * prescribed_velocity_mean_x and fix_velocity_angular /
+ * prescribed_angular_vel.  Active while PD_FLAG_VELOCITY_CONSTRAINT is on;
+ * pd_get_applied_force reads its applied_force_density [nnodes*3]
+ * (zeros before the constraint has acted). */
+int pd_set_velocity_constraint(pd_ctx* ctx, uint32_t p, int fix_mean_x, double mean_x,
+                               int fix_angular, double angular_vel);
+int pd_get_applied_force(pd_ctx* ctx, uint32_t p, double* afd);
 
 /* State read-back for output (timeloop.h:826-851).  Any pointer may be NULL.
  * Arrays are [nnodes*3].  force is the force density, as in the reference. */

@@ -71,6 +71,14 @@ struct or_ctx {
   int committed, nranks;
   pd_counters cnt;
   char err[256];
+  /* prescribed velocity (timeloop.h:685-764): CFGV.particle_w_fixed_velocity,
+   * fix_velocity_mean_x, prescribed_velocity_mean_x, fix_velocity_angular,
+   * prescribed_angular_vel; TL.enable_velocity_constraint */
+  int64_t vc_part;
+  int vc_on, vc_fix_mean_x, vc_fix_ang;
+  double vc_mean_x, vc_omega;
+  double* afd;      /* applied_force_density of the constrained particle */
+  double* aold;     /* scratch: its acc before the update */
 };
 
 static int fail(or_ctx* c, int code, const char* fmt, ...) {
@@ -106,6 +114,7 @@ int or_create(or_ctx** out, const pd_run_desc* run) {
   memcpy(c->speed, run->wall_speed, sizeof c->speed);
   c->t = run->first_step ? run->first_step : 1;
   c->nranks = 1;
+  c->vc_part = -1;
   *out = c;
   return PD_OK;
 }
@@ -509,6 +518,49 @@ static uint64_t rank_work(or_ctx* c, int rank, int nranks, double* gf, double re
   return pw;
 }
 
+/* Prescribed velocity of one particle (timeloop.h:685-764), applied after
+ * its velocity update and before its clamped nodes are reset.  The radial /
+ * tangential split of the reference (v_r, v_t, t_unit) feeds nothing and is
+ * left out; the applied force density is formed from the assigned velocity
+ * exactly as the reference forms it. */
+static void prescribe(or_ctx* c, opart* p, double dt) {
+  const uint32_t n = p->n;
+  double mean[3] = {0, 0, 0};
+  for (uint32_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) mean[k] += p->vel[3 * (size_t)i + k];
+  for (int k = 0; k < 3; k++) mean[k] /= (double)n;
+  double m[3] = {mean[0], mean[1], mean[2]};
+  if (c->vc_fix_mean_x) m[0] = c->vc_mean_x;
+  double cen[3] = {0, 0, 0};
+  if (c->vc_fix_ang) {
+    for (uint32_t i = 0; i < n; i++)
+      for (int k = 0; k < 3; k++) cen[k] += (p->pos[3 * (size_t)i + k] + p->disp[3 * (size_t)i + k]);
+    for (int k = 0; k < 3; k++) cen[k] /= (double)n;
+  }
+  for (uint32_t i = 0; i < n; i++) {
+    double* v = p->vel + 3 * (size_t)i;
+    double vr[3] = {v[0] - mean[0], v[1] - mean[1], v[2] - mean[2]};
+    if (c->vc_fix_ang) {
+      double rv[3];
+      for (int k = 0; k < 3; k++) rv[k] = p->cp[3 * (size_t)i + k] - cen[k];
+      double r = nrm3(rv[0], rv[1], rv[2]);
+      double ru[3] = {0, 0, 0};
+      if (r > 0)
+        for (int k = 0; k < 3; k++) ru[k] = rv[k] / r;
+      vr[0] = (-(r * c->vc_omega)) * ru[1];
+      vr[1] = (r * c->vc_omega) * ru[0];
+      vr[2] = 0;
+    }
+    double pv[3];
+    for (int k = 0; k < 3; k++) pv[k] = vr[k] + m[k];
+    for (int k = 0; k < 3; k++) {
+      v[k] = pv[k];
+      c->afd[3 * (size_t)i + k] =
+          p->rho * (((2 / dt) * (pv[k] - v[k])) - c->aold[3 * (size_t)i + k]) - p->force[3 * (size_t)i + k];
+    }
+  }
+}
+
 static void one_step(or_ctx* c) {
   const pd_run_desc* R = &c->run;
   const double dt = R->dt;
@@ -560,12 +612,15 @@ static void one_step(or_ctx* c) {
     memcpy(p->force, g, 3ull * p->n * sizeof(double));
     if (!p->movable) continue;
     double rinv = 1 / p->rho;
+    const int vc = c->vc_on && (int64_t)i == c->vc_part;
     for (uint64_t k = 0; k < 3ull * p->n; k++) {
       double aold = p->acc[k];
       double anew = rinv * p->force[k];
+      if (vc) c->aold[k] = aold;
       p->acc[k] = anew;
       p->vel[k] = p->vel[k] + ((aold + anew) * (0.5 * dt));
     }
+    if (vc) prescribe(c, p, dt);
     for (uint32_t q = 0; q < p->nclamped; q++) {
       uint32_t cid = p->clamped[q];
       for (int k = 0; k < 3; k++) p->disp[3 * (size_t)cid + k] = p->vel[3 * (size_t)cid + k] = p->acc[3 * (size_t)cid + k] = 0;
@@ -605,6 +660,7 @@ int or_set_flag(or_ctx* c, int flag, int value) {
     case PD_FLAG_ALLOW_FRICTION: c->run.allow_friction = value; break;
     case PD_FLAG_ENABLE_TORQUE: c->run.enable_torque = value; break;
     case PD_FLAG_COMPUTE_WALL_REACTION: c->run.compute_wall_reaction = value; break;
+    case PD_FLAG_VELOCITY_CONSTRAINT: c->vc_on = value; break;
     default: return fail(c, PD_E_ARG, "unknown flag %d", flag);
   }
   return PD_OK;
@@ -717,10 +773,36 @@ int or_get_counters(or_ctx* c, pd_counters* out) {
   return PD_OK;
 }
 
+int or_set_velocity_constraint(or_ctx* c, uint32_t pi, int fix_mean_x, double mean_x, int fix_angular,
+                               double angular_vel) {
+  if (!c) return PD_E_ARG;
+  if (!c->committed) return fail(c, PD_E_STATE, "or_set_velocity_constraint before commit");
+  if (pi >= c->np) return fail(c, PD_E_ARG, "no particle %u", pi);
+  free(c->afd);
+  free(c->aold);
+  c->afd = (double*)calloc(3 * (size_t)c->P[pi].n, sizeof(double));
+  c->aold = (double*)calloc(3 * (size_t)c->P[pi].n, sizeof(double));
+  c->vc_part = pi;
+  c->vc_fix_mean_x = fix_mean_x;
+  c->vc_mean_x = mean_x;
+  c->vc_fix_ang = fix_angular;
+  c->vc_omega = angular_vel;
+  return PD_OK;
+}
+
+int or_get_applied_force(or_ctx* c, uint32_t pi, double* afd) {
+  if (!c || !afd) return PD_E_ARG;
+  if ((int64_t)pi != c->vc_part || !c->afd) return fail(c, PD_E_ARG, "particle %u has no prescribed velocity", pi);
+  memcpy(afd, c->afd, 3 * (size_t)c->P[pi].n * sizeof(double));
+  return PD_OK;
+}
+
 const char* or_last_error(or_ctx* c) { return c ? c->err : "null context"; }
 
 void or_destroy(or_ctx* c) {
   if (!c) return;
+  free(c->afd);
+  free(c->aold);
   for (uint32_t i = 0; i < c->np; i++) {
     opart* p = &c->P[i];
     free(p->pos); free(p->disp); free(p->cp); free(p->vel); free(p->acc);

@@ -45,6 +45,9 @@ int or_get_connectivity(or_ctx* ctx, uint32_t p, uint32_t* pairs,
  * (the reference's ParticleN::stretch, particle2.h:153); *m = entries. */
 int or_get_stretch(or_ctx* ctx, uint32_t p, double* out, uint64_t cap, uint64_t* m);
 int or_get_wall(or_ctx* ctx, uint32_t instance, double lrtb[6], double reaction[18]);
+int or_set_velocity_constraint(or_ctx* ctx, uint32_t p, int fix_mean_x, double mean_x,
+                               int fix_angular, double angular_vel);
+int or_get_applied_force(or_ctx* ctx, uint32_t p, double* afd);
 int or_get_counters(or_ctx* ctx, pd_counters* out);
 const char* or_last_error(or_ctx* ctx);
 void or_destroy(or_ctx* ctx);

@@ -26,6 +26,7 @@ PD_FLAG_ALLOW_DAMPING = 3
 PD_FLAG_ALLOW_FRICTION = 4
 PD_FLAG_ENABLE_TORQUE = 5
 PD_FLAG_COMPUTE_WALL_REACTION = 6
+PD_FLAG_VELOCITY_CONSTRAINT = 7
 PD_PFLAG_MOVABLE = 1
 PD_PFLAG_STOPPABLE = 2
 PD_PFLAG_BREAKABLE = 3
@@ -104,6 +105,8 @@ def _bind(lib, prefix):
         "get_connectivity": (C.c_int, [P, _u32, _pu, _u64, C.POINTER(_u64)]),
         "get_wall": (C.c_int, [P, _u32, _pd, _pd]),
         "get_counters": (C.c_int, [P, C.POINTER(Counters)]),
+        "set_velocity_constraint": (C.c_int, [P, _u32, C.c_int, _dbl, C.c_int, _dbl]),
+        "get_applied_force": (C.c_int, [P, _u32, _pd]),
         "last_error": (C.c_char_p, [P]),
         "destroy": (None, [P]),
     }
@@ -211,6 +214,16 @@ class Engine:
 
     def set_snot(self, p, snot):
         self._check(self.f["set_snot"](self.ctx, p, float(snot)), "set_snot")
+
+    def set_velocity_constraint(self, p, fix_mean_x=0, mean_x=0.0, fix_angular=0, angular_vel=0.0):
+        self._check(self.f["set_velocity_constraint"](self.ctx, p, int(fix_mean_x), float(mean_x),
+                                                      int(fix_angular), float(angular_vel)),
+                    "set_velocity_constraint")
+
+    def applied_force(self, p):
+        out = np.empty((self.sys.particles[p].n, 3))
+        self._check(self.f["get_applied_force"](self.ctx, p, _ptr(out)), "get_applied_force")
+        return out
 
     def state(self, p):
         n = self.sys.particles[p].n

@@ -64,6 +64,8 @@ struct Job {
   uint64_t remaining = 0, steps_done = 0, steps_since_set = 0;
   uint64_t movable_bonds = 0, total_bonds = 0, nodes = 0;
   bool reaction_valid = false;
+  bool vc_on = false;        // a prescribed-velocity particle is set and switched on
+  int64_t vc_part = -1;      // its particle index in the job
 };
 
 // first-fit allocator over [0, cap)
@@ -189,6 +191,7 @@ static void void_part(DevPart& q, uint32_t void_inst) {
 static void void_inst(DevInst& q) {
   memset(&q, 0, sizeof q);
   q.dt = 1.0;
+  q.vc_slot = kNoKey;
 }
 
 static void fill_inst(const pd_run_desc& r, DevInst& D) {
@@ -218,7 +221,7 @@ static void fill_inst(const pd_run_desc& r, DevInst& D) {
   D.extforce_maxstep = r.extforce_maxstep;
   D.enable_torque = r.enable_torque;
   D.nl_bdry_only = r.nl_bdry_only;
-  D.pad1 = 0;
+  D.vc_slot = kNoKey;
 }
 
 static int check_run(pd_ctx* c, const pd_run_desc& r) {
@@ -337,6 +340,12 @@ static bool any_contact_possible(pd_ctx* c) {
 static bool any_torque(pd_ctx* c) {
   for (auto& J : c->jobs)
     if (J.used && J.run.enable_torque) return true;
+  return false;
+}
+
+static bool any_vc(pd_ctx* c) {
+  for (auto& J : c->jobs)
+    if (J.used && J.vc_on && J.vc_part >= 0) return true;
   return false;
 }
 
@@ -1032,6 +1041,10 @@ static int launch_step(pd_ctx* c) {
   if ((rc = mark(4))) return rc;
   if (any_reaction(c)) hipLaunchKernelGGL(k_reaction, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
   hipLaunchKernelGGL(k_update, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
+  if (any_vc(c)) {
+    hipLaunchKernelGGL(k_pv_mean, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
+    hipLaunchKernelGGL(k_pv_apply, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
+  }
   hipLaunchKernelGGL(k_finalize, dim3(grid_of(std::max<uint64_t>(d.P, d.ninst), 256)), dim3(256), 0, st, d);
   if ((rc = mark(5))) return rc;
   HIPCHK(c, hipGetLastError());
@@ -1279,6 +1292,11 @@ static void apply_flag(pd_run_desc& r, int flag, int value) {
 
 static int job_set_flag(pd_ctx* c, uint32_t j, int flag, int value) {
   Job& J = c->jobs[j];
+  if (flag == PD_FLAG_VELOCITY_CONSTRAINT) {
+    J.vc_on = value != 0;
+    c->graph_dirty = true;
+    return edit_inst(c, j, [&](DevInst& D) { D.vc_on = value != 0; });
+  }
   pd_run_desc r = J.run;
   apply_flag(r, flag, value);
   int rc = check_run(c, r);
@@ -1331,7 +1349,7 @@ extern "C" {
 
 int pd_set_flag(pd_ctx* c, int flag, int value) {
   if (!c) return PD_E_ARG;
-  if (flag < PD_FLAG_ENABLE_FRACTURE || flag > PD_FLAG_COMPUTE_WALL_REACTION)
+  if (flag < PD_FLAG_ENABLE_FRACTURE || flag > PD_FLAG_VELOCITY_CONSTRAINT)
     return fail(c, PD_E_ARG, "unknown flag %d", flag);
   for (uint32_t j = 0; j < c->jobs.size(); j++)
     if (c->jobs[j].used) {
@@ -1344,7 +1362,7 @@ int pd_set_flag(pd_ctx* c, int flag, int value) {
 
 int pd_job_set_flag(pd_ctx* c, uint32_t job, int flag, int value) {
   if (!c) return PD_E_ARG;
-  if (flag < PD_FLAG_ENABLE_FRACTURE || flag > PD_FLAG_COMPUTE_WALL_REACTION)
+  if (flag < PD_FLAG_ENABLE_FRACTURE || flag > PD_FLAG_VELOCITY_CONSTRAINT)
     return fail(c, PD_E_ARG, "unknown flag %d", flag);
   if (job >= c->jobs.size() || !c->jobs[job].used) return fail(c, PD_E_ARG, "no job %u", job);
   return job_set_flag(c, job, flag, value);
@@ -1370,6 +1388,42 @@ int pd_set_wall(pd_ctx* c, const double lrtb[6]) {
       if (rc) return rc;
     }
   memcpy(c->run0.wall_lrtb, lrtb, 6 * sizeof(double));
+  return PD_OK;
+}
+
+int pd_set_velocity_constraint(pd_ctx* c, uint32_t pi, int fix_mean_x, double mean_x, int fix_angular,
+                               double angular_vel) {
+  if (!c) return PD_E_ARG;
+  uint32_t job, lp;
+  int rc = plain(c, pi, &job, &lp);
+  if (rc) return rc;
+  if (!c->d.AFD) {
+    if ((rc = dalloc(c, &c->d.AFD, 3 * c->d.N))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d.AFD, 0, 3 * c->d.N * sizeof(double), c->stream));
+  }
+  Job& J = c->jobs[job];
+  J.vc_part = lp;
+  const uint32_t slot = J.parts[lp].slot;
+  c->graph_dirty = true;
+  return edit_inst(c, job, [&](DevInst& D) {
+    D.vc_slot = slot;
+    D.vc_fix_mean_x = fix_mean_x != 0;
+    D.vc_mean_x = mean_x;
+    D.vc_fix_ang = fix_angular != 0;
+    D.vc_omega = angular_vel;
+  });
+}
+
+int pd_get_applied_force(pd_ctx* c, uint32_t pi, double* afd) {
+  if (!c || !afd) return PD_E_ARG;
+  uint32_t job, lp;
+  int rc = plain(c, pi, &job, &lp);
+  if (rc) return rc;
+  if (c->jobs[job].vc_part != (int64_t)lp || !c->d.AFD)
+    return fail(c, PD_E_ARG, "particle %u has no prescribed velocity", pi);
+  const HostPart& H = c->jobs[job].parts[lp];
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(afd, c->d.AFD + 3 * H.off, 3ull * H.n * sizeof(double), hipMemcpyDeviceToHost));
   return PD_OK;
 }
 

@@ -56,7 +56,10 @@ struct DevInst {
   uint32_t extforce_maxstep;
   int32_t enable_torque;
   int32_t nl_bdry_only;     // contact and wall terms on boundary nodes only (contact.h:751-760)
-  uint32_t pad1;
+  uint32_t vc_slot;         // prescribed-velocity particle (timeloop.h:685-764), kNoKey if none
+  int32_t vc_on, vc_fix_mean_x, vc_fix_ang, pad2;
+  double vc_mean_x, vc_omega;
+  double vc_mean[3], vc_cent[3];   // its mean velocity and mean CurrPos of this step
 };
 
 // Per-launch control words shared by the kernels of a step.
@@ -72,6 +75,7 @@ struct Dev {
   double* V;         // vel [3N]
   double* A;         // acc [3N]
   double* F;         // force density [3N]
+  double* AFD;       // applied_force_density of prescribed-velocity particles [3N]
   double* E;         // extforce [3N]
   uint32_t* pid;     // particle of node
   uint32_t* brank;   // position of the node in its particle's boundary list, kNoKey if absent

@@ -743,7 +743,10 @@ __global__ __launch_bounds__(256) void k_update(Dev d) {
   const DevInst& I = d.inst[P.inst];
   if (!live(I) || !P.movable) return;
   const double hdt = 0.5 * I.dt;
-  if (d.nflag[g] & 1u) {
+  // the prescribed-velocity particle is updated in full here and clamped
+  // after its prescription (k_pv_apply)
+  const bool vc = I.vc_on && d.pid[g] == I.vc_slot;
+  if ((d.nflag[g] & 1u) && !vc) {
     d.U[g] = make_double4(0.0, 0.0, 0.0, 0.0);
     for (int k = 0; k < 3; k++) d.V[3 * g + k] = d.A[3 * g + k] = 0.0;
     return;
@@ -751,8 +754,75 @@ __global__ __launch_bounds__(256) void k_update(Dev d) {
   for (int k = 0; k < 3; k++) {
     double aold = d.A[3 * g + k];
     double anew = P.rinv * d.F[3 * g + k];
+    if (vc) d.AFD[3 * g + k] = aold;
     d.A[3 * g + k] = anew;
     d.V[3 * g + k] = d.V[3 * g + k] + ((aold + anew) * hdt);
+  }
+}
+
+// Prescribed velocity (timeloop.h:685-764), one thread per job: the mean
+// velocity and, for the angular part, mean_CurrPos of the particle, summed
+// in node order like the reference.
+__global__ void k_pv_mean(Dev d) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.ninst) return;
+  DevInst& I = d.inst[q];
+  if (!live(I) || !I.vc_on || I.vc_slot == kNoKey) return;
+  const DevPart& P = d.part[I.vc_slot];
+  if (!P.movable) return;
+  double m[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+  for (uint64_t g = P.node_off; g < P.node_off + P.nnodes; g++)
+    for (int k = 0; k < 3; k++) m[k] += d.V[3 * g + k];
+  const double n = (double)P.nnodes;
+  for (int k = 0; k < 3; k++) I.vc_mean[k] = m[k] / n;
+  if (I.vc_fix_ang) {
+    for (uint64_t g = P.node_off; g < P.node_off + P.nnodes; g++) {
+      const double4 x = d.X[g];
+      const double4 u = d.U[g];
+      c[0] += (x.x + u.x);
+      c[1] += (x.y + u.y);
+      c[2] += (x.z + u.z);
+    }
+    for (int k = 0; k < 3; k++) I.vc_cent[k] = c[k] / n;
+  }
+}
+
+// The prescription per node, the applied force density formed from the
+// assigned velocity as the reference forms it, then the clamped nodes.  The
+// reference's radial / tangential split (v_r, v_t, t_unit) feeds nothing.
+__global__ __launch_bounds__(256) void k_pv_apply(Dev d) {
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.N) return;
+  const uint32_t p = d.pid[g];
+  const DevPart& P = d.part[p];
+  const DevInst& I = d.inst[P.inst];
+  if (!live(I) || !I.vc_on || p != I.vc_slot || !P.movable) return;
+  const double* mean = I.vc_mean;
+  double m[3] = {mean[0], mean[1], mean[2]};
+  if (I.vc_fix_mean_x) m[0] = I.vc_mean_x;
+  double v[3] = {d.V[3 * g], d.V[3 * g + 1], d.V[3 * g + 2]};
+  double vr[3] = {v[0] - mean[0], v[1] - mean[1], v[2] - mean[2]};
+  if (I.vc_fix_ang) {
+    const double4 x = d.X[g];
+    const double4 u = d.U[g];
+    const double rv[3] = {(x.x + u.x) - I.vc_cent[0], (x.y + u.y) - I.vc_cent[1], (x.z + u.z) - I.vc_cent[2]};
+    const double r = nrm3(rv[0], rv[1], rv[2]);
+    double ru[3] = {0, 0, 0};
+    if (r > 0)
+      for (int k = 0; k < 3; k++) ru[k] = rv[k] / r;
+    vr[0] = (-(r * I.vc_omega)) * ru[1];
+    vr[1] = (r * I.vc_omega) * ru[0];
+    vr[2] = 0;
+  }
+  for (int k = 0; k < 3; k++) {
+    const double pv = vr[k] + m[k];
+    v[k] = pv;
+    d.V[3 * g + k] = pv;
+    d.AFD[3 * g + k] = P.rho * (((2 / I.dt) * (pv - v[k])) - d.AFD[3 * g + k]) - d.F[3 * g + k];
+  }
+  if (d.nflag[g] & 1u) {
+    d.U[g] = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int k = 0; k < 3; k++) d.V[3 * g + k] = d.A[3 * g + k] = 0.0;
   }
 }
 

@@ -264,8 +264,6 @@ static int run(Config& C) {
                   C.speed_wall_x_max, C.speed_wall_y_max, C.speed_wall_z_max};
   for (int k = 0; k < 6; k++) W.speed[k] = sp[k];
 
-  if (C.particle_w_fixed_velocity != -1 && C.set_fix_velocity_timestep != -1)
-    throw std::runtime_error("prescribed velocity (particle_w_fixed_velocity) is not supported yet");
   if (C.brazil || C.reset_partzero_y)
     throw std::runtime_error("brazil / reset_partzero_y schedules are not supported (2-D / wheel only)");
 
@@ -360,6 +358,15 @@ static int run(Config& C) {
     check(ctx, pd_add_particle(ctx, &d, nullptr), "pd_add_particle");
   }
   check(ctx, pd_commit(ctx), "pd_commit");
+  // prescribed velocity of one particle (timeloop.h:685-764), switched on at
+  // set_fix_velocity_timestep
+  const bool vc_particle = C.particle_w_fixed_velocity >= 0 && (unsigned)C.particle_w_fixed_velocity < P.size();
+  if (vc_particle)
+    check(ctx,
+          pd_set_velocity_constraint(ctx, C.particle_w_fixed_velocity, C.fix_velocity_mean_x,
+                                     C.prescribed_velocity_mean_x, C.fix_velocity_angular,
+                                     C.prescribed_angular_vel),
+          "pd_set_velocity_constraint");
 
   const unsigned cw_ = 12;
   time_t now = time(NULL);
@@ -373,9 +380,9 @@ static int run(Config& C) {
 
   // the steps at which the host acts before the step (timeloop.h:405-535)
   auto is_switch = [&](unsigned t) {
-    for (int s : {C.set_torque_timestep, C.set_fracture_timestep, C.set_self_contact_timestep,
-                  C.set_damping_timestep, C.set_zero_wall_speed_timestep, C.set_movable_timestep,
-                  C.set_stoppable_timestep})
+    for (int s : {C.set_torque_timestep, C.set_fix_velocity_timestep, C.set_fracture_timestep,
+                  C.set_self_contact_timestep, C.set_damping_timestep, C.set_zero_wall_speed_timestep,
+                  C.set_movable_timestep, C.set_stoppable_timestep})
       if (s != -1 && (unsigned)s == t) return true;
     return false;
   };
@@ -383,6 +390,10 @@ static int run(Config& C) {
     if (C.set_torque_timestep != -1 && t == (unsigned)C.set_torque_timestep) {
       check(ctx, pd_set_flag(ctx, PD_FLAG_ENABLE_TORQUE, 1), "pd_set_flag");
       std::cout << "Setting torque to 1 on timestep " << t << std::endl;
+    }
+    if (C.set_fix_velocity_timestep != -1 && t == (unsigned)C.set_fix_velocity_timestep) {
+      check(ctx, pd_set_flag(ctx, PD_FLAG_VELOCITY_CONSTRAINT, 1), "pd_set_flag");
+      std::cout << "Setting velocity constraint to 1 on timestep " << t << std::endl;
     }
     if (C.set_fracture_timestep != -1 && t == (unsigned)C.set_fracture_timestep) {
       check(ctx, pd_set_flag(ctx, PD_FLAG_ENABLE_FRACTURE, 1), "pd_set_flag");
@@ -436,6 +447,13 @@ static int run(Config& C) {
         out->write_f64(g + "vel", v, {P[i].n, 3});
         out->write_f64(g + "acc", a, {P[i].n, 3});
         out->write_f64(g + "force", f, {P[i].n, 3});   // force density, as the reference stores
+        if (vc_particle && i == (unsigned)C.particle_w_fixed_velocity) {
+          // zeros until the constraint first acts (the reference leaves the
+          // array uninitialised until then)
+          vector<double> af(n3);
+          check(ctx, pd_get_applied_force(ctx, i, af.data()), "pd_get_applied_force");
+          out->write_f64(g + "applied_force_density", af, {P[i].n, 3});
+        }
         uint64_t m = 0;
         check(ctx, pd_get_connectivity(ctx, i, nullptr, 0, &m), "pd_get_connectivity");
         vector<uint32_t> conn(2 * m);

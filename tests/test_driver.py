@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from helpers import assert_bitwise, small_impact
-from peridem_amd import setupio
+from peridem_amd import abi, setupio
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "peridem_amd", "driver", "simulate3d")
@@ -120,3 +120,60 @@ def test_driver_run_matches_oracle(tmp_path, oracle_lib, container):
         assert_bitwise(gw["wall_info"].ravel(), lrtb, f"step {t} wall_info")
         assert_bitwise(gw["reaction"], reac, f"step {t} reaction")
     assert (out / "plotinfo.h5").exists() and (out / "run_time.h5").exists()
+
+
+@pytest.mark.gpu
+def test_driver_torque_velocity_constraint_boundary_contact(tmp_path, oracle_lib):
+    """Config keys enable_torque, nl_bdry_only, particle_w_fixed_velocity with
+    fix_velocity_mean_x / fix_velocity_angular and set_fix_velocity_timestep;
+    tc files (incl. applied_force_density) bitwise against the oracle."""
+    s = small_impact(seed=19)
+    s.self_contact = 1
+    s.nl_bdry_only = 1
+    s.enable_torque = 1
+    rng = np.random.default_rng(2)
+    for p in s.particles:
+        deg = np.bincount(p.conn.reshape(-1), minlength=p.n)
+        p.bdry = rng.permutation(np.nonzero(deg < deg.max())[0]).astype(np.uint32)
+    s.particles[0].torque_axis, s.particles[0].torque_val = 2, 2.0e9
+    s.particles[1].torque_axis, s.particles[1].torque_val = 0, -1.5e9
+    setup = tmp_path / "setup.pdbin"
+    setupio.write_setup(s, setup)
+    T, modulo, vc_at = 24, 8, 6
+    vc = dict(fix_velocity_mean_x=1, prescribed_velocity_mean_x=-5.0, fix_velocity_angular=1,
+              prescribed_angular_vel=3.0e4)
+    cfg = tmp_path / "main.conf"
+    setupio.write_config(cfg, timesteps=T, modulo=modulo, dt=s.dt, enable_fracture=1, self_contact=1,
+                         allow_damping=1, allow_friction=1, compute_wall_reaction=1, nl_bdry_only=1,
+                         enable_torque=1, particle_w_fixed_velocity=0, set_fix_velocity_timestep=vc_at,
+                         speed_wall_x_min=s.wall_speed[0], speed_wall_y_min=s.wall_speed[1],
+                         speed_wall_z_min=s.wall_speed[2], speed_wall_x_max=s.wall_speed[3],
+                         speed_wall_y_max=s.wall_speed[4], speed_wall_z_max=s.wall_speed[5], **vc)
+    out = tmp_path / "out"
+    run_driver("-c", str(cfg), "-i", str(setup), "-o", str(out))
+    has_h5 = "hdf5 yes" in run_driver("--features").stdout
+    o = oracle_lib.engine(s)
+    o.set_velocity_constraint(0, vc["fix_velocity_mean_x"], vc["prescribed_velocity_mean_x"],
+                              vc["fix_velocity_angular"], vc["prescribed_angular_vel"])
+    try:
+        for t in range(1, T + 1):
+            if t == vc_at:
+                o.set_flag(abi.PD_FLAG_VELOCITY_CONSTRAINT, 1)
+            o.step(1)
+            if t % modulo:
+                continue
+            tc = out / f"tc_{t // modulo:05d}.h5"
+            if has_h5:
+                run_driver("--convert", str(tc), str(tmp_path / "tc.pdbin"))
+                tc = tmp_path / "tc.pdbin"
+            got = setupio.read_pdbin(tc)
+            for p in range(len(s.particles)):
+                g, st = f"P_{p:05d}/", o.state(p)
+                for k, key in (("CurrPos", "currpos"), ("vel", "vel"), ("acc", "acc"), ("force", "force")):
+                    assert_bitwise(got[g + k], st[key], f"step {t} {g}{k}")
+                assert np.array_equal(got[g + "Connectivity"], o.connectivity(p)), f"step {t} {g}bonds"
+            af = o.applied_force(0) if t >= vc_at else np.zeros((s.particles[0].n, 3))
+            assert_bitwise(got["P_00000/applied_force_density"], af, f"step {t} applied_force_density")
+            assert "P_00001/applied_force_density" not in got
+    finally:
+        o.close()

@@ -204,3 +204,27 @@ def test_nl_bdry_only_matches_oracle(oracle_lib):
         surf = np.nonzero(deg < deg.max())[0].astype(np.uint32)
         p.bdry = rng.permutation(surf)
     run_pair(s, [30, 90, 150], oracle_lib, "nl_bdry_only", reaction=True)
+
+
+@pytest.mark.parametrize("mean_x,angular", [(1, 0), (0, 1), (1, 1), (0, 0)])
+def test_prescribed_velocity_matches_oracle(oracle_lib, mean_x, angular):
+    """particle_w_fixed_velocity (timeloop.h:685-764): particle 0 (with clamped
+    nodes) gets its mean x velocity and/or an angular velocity prescribed from
+    a switch step on; state and applied_force_density bitwise."""
+    s = mixed_system(seed=33)
+    g = abi.gpu_engine(s)
+    o = oracle_lib.engine(s)
+    try:
+        for eng in (g, o):
+            eng.set_velocity_constraint(0, fix_mean_x=mean_x, mean_x=-12.5, fix_angular=angular,
+                                        angular_vel=4.0e4)
+            eng.step(9)
+            eng.set_flag(abi.PD_FLAG_VELOCITY_CONSTRAINT, 1)
+        for c in (1, 16, 40):
+            g.step(c)
+            o.step(c)
+            rg, ro = compare_engines(g, o, s, f"prescribed {mean_x}{angular} +{c}")
+            assert_bitwise(g.applied_force(0), o.applied_force(0), f"applied force +{c}")
+    finally:
+        g.close()
+        o.close()

@@ -391,3 +391,63 @@ def test_nl_bdry_only_skips_unlisted_nodes(oracle_lib):
     e, st = force_after_one_step(oracle_lib, s)
     assert st[0]["force"][0, 0] == pytest.approx(4.6e31 * 9e-5 * ((6e-5 - 9e-5) / 9e-5) * 5e-12, rel=1e-14)
     e.close()
+
+
+# ---------------------------------------------------------------- prescribed velocity
+def _spinning_block(v0=(3.0, -2.0, 1.0)):
+    rng = np.random.default_rng(9)
+    L = gen.lattice_cube(2.0)
+    p = gen.make_particle(L, 1e-4, np.zeros(3), rng)
+    p.vel = np.broadcast_to(np.array(v0, float), (p.n, 3)).copy()
+    p.clamped = np.array([3, 40], dtype=np.uint32)
+    return p
+
+
+def test_prescribed_mean_x_velocity(oracle_lib):
+    """fix_velocity_mean_x on a rigidly moving particle: vel_x becomes the
+    prescribed mean, the other components keep the motion (timeloop.h:685-764)."""
+    p = _spinning_block()
+    s = system([p], enable_fracture=0)
+    e = oracle_lib.engine(s)
+    e.set_velocity_constraint(0, fix_mean_x=1, mean_x=7.5)
+    e.set_flag(abi.PD_FLAG_VELOCITY_CONSTRAINT, 1)
+    acc0 = e.state(0)["acc"]
+    e.step(1)
+    st = e.state(0)
+    free = np.setdiff1d(np.arange(p.n), p.clamped)
+    assert np.allclose(st["vel"][free, 0], 7.5, rtol=1e-12, atol=0)
+    assert np.allclose(st["vel"][free, 1:], [-2.0, 1.0], rtol=1e-12, atol=0)
+    assert np.all(st["vel"][p.clamped] == 0) and np.all(st["disp"][p.clamped] == 0)
+    afd = e.applied_force(0)
+    assert np.allclose(afd, -p.rho * acc0 - st["force"], rtol=1e-15, atol=1e-300)
+    e.close()
+
+
+def test_prescribed_angular_velocity(oracle_lib):
+    """fix_velocity_angular: the velocity becomes omega x (CurrPos - centroid)
+    about z plus the mean velocity."""
+    p = _spinning_block(v0=(0.0, 0.0, 0.0))
+    p.clamped = None
+    s = system([p], enable_fracture=0)
+    e = oracle_lib.engine(s)
+    e.set_velocity_constraint(0, fix_angular=1, angular_vel=250.0)
+    e.set_flag(abi.PD_FLAG_VELOCITY_CONSTRAINT, 1)
+    e.step(1)
+    st = e.state(0)
+    r = st["currpos"] - st["currpos"].mean(0)
+    want = np.stack([-250.0 * r[:, 1], 250.0 * r[:, 0], np.zeros(p.n)], 1)
+    assert np.allclose(st["vel"], want, rtol=1e-9, atol=1e-12)
+    e.close()
+
+
+def test_velocity_constraint_off_changes_nothing(oracle_lib):
+    p = _spinning_block()
+    a = oracle_lib.engine(system([p], enable_fracture=0))
+    b = oracle_lib.engine(system([p], enable_fracture=0))
+    b.set_velocity_constraint(0, fix_mean_x=1, mean_x=1.0, fix_angular=1, angular_vel=9.0)
+    a.step(3)
+    b.step(3)
+    for k in ("disp", "vel", "acc", "force"):
+        assert np.array_equal(a.state(0)[k], b.state(0)[k]), k
+    a.close()
+    b.close()
