@@ -264,8 +264,14 @@ static int run(Config& C) {
                   C.speed_wall_x_max, C.speed_wall_y_max, C.speed_wall_z_max};
   for (int k = 0; k < 6; k++) W.speed[k] = sp[k];
 
-  if (C.brazil || C.reset_partzero_y)
-    throw std::runtime_error("brazil / reset_partzero_y schedules are not supported (2-D / wheel only)");
+  // the brazil-nut schedule (timeloop.h:466-479) writes Wall.bottom and
+  // Wall.speed_bottom, which a 3-D RectWall never reads (contact.h:221-238):
+  // in 3-D it changes nothing
+  if (C.brazil) std::cout << "brazil: acts on the 2-D wall members only, no effect in 3-D" << std::endl;
+  // reset_partzero_y (timeloop.h:496-535) moves particle 0's reference
+  // positions while the bond terms keep the stored xi
+  if (C.reset_partzero_y)
+    throw std::runtime_error("reset_partzero_y (wheel reset of particle 0's reference positions) is not supported");
 
   // resume (timeloop.h:262-312)
   unsigned counter = 1, first_counter = 1, last_counter = C.timesteps / C.modulo;
