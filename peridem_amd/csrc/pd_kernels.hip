@@ -491,7 +491,7 @@ __device__ __forceinline__ void bond_terms(uint32_t* bp, uint32_t k, const uint3
 // k+2 and the neighbour records of batch k+1 are in flight while batch k is
 // evaluated, so the index stream (HBM) and the gathers (L2) overlap the FP64
 // arithmetic instead of stalling every batch.
-template <int U, bool PIPE, bool MASK>
+template <int U, int PIPE, bool MASK>
 __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
   const uint64_t nbp = gridDim.x;   // a multiple of kXcds
   const uint64_t blk = (blockIdx.x % kXcds) * (nbp / kXcds) + blockIdx.x / kXcds;
@@ -516,13 +516,36 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     uint32_t* bp = d.bond + base + (g & 63);
     const double4 xi = d.X[g];
     const double4 ui = d.U[g];
-    if (!PIPE) {
+    if (PIPE == 0) {
       for (uint32_t k = 0; k < W; k += U) {
         uint32_t e[U];
         double4 xj[U], uj[U];
         bond_idx<U>(bp, k, W, e);
         bond_gather<U, MASK>(d, g, e, xi, ui, xj, uj);
         bond_terms<U>(bp, k, e, xj, uj, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+      }
+    } else if (PIPE == 2 && W) {
+      // the same pipeline unrolled twice: the neighbour records alternate
+      // between two register sets instead of being copied every batch
+      uint32_t ea[U], eb[U];
+      double4 xa[U], ua[U], xb[U], ub[U];
+      bond_idx<U>(bp, 0, W, ea);
+      bond_idx<U>(bp, U, W, eb);
+      bond_gather<U, MASK>(d, g, ea, xi, ui, xa, ua);
+      for (uint32_t k = 0; k < W; k += 2 * U) {
+        uint32_t ec[U];
+        bond_idx<U>(bp, k + 2 * U, W, ec);
+        bond_gather<U, MASK>(d, g, eb, xi, ui, xb, ub);
+        bond_terms<U>(bp, k, ea, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+        if (k + U >= W) break;
+        bond_idx<U>(bp, k + 3 * U, W, ea);
+        bond_gather<U, MASK>(d, g, ec, xi, ui, xa, ua);
+        bond_terms<U>(bp, k + U, eb, xb, ub, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          eb[u] = ea[u];   // batch k + 3U
+          ea[u] = ec[u];   // batch k + 2U, its records already in xa/ua
+        }
       }
     } else if (W) {
       uint32_t e0[U], e1[U];
@@ -556,16 +579,17 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
 // so all give the same bits.
 struct BondShape {
   void (*fn)(Dev);
-  int unroll, pipe, mask;
+  int unroll, pipe, mask;   // pipe: 0 none, 1 pipelined, 2 pipelined and unrolled twice
 };
-constexpr int kBondShapes = 9;
+constexpr int kBondShapes = 12;
 __host__ inline const BondShape* bond_shapes() {
   static const BondShape t[kBondShapes] = {
-      {k_bond_t<8, false, false>, 8, 0, 0}, {k_bond_t<4, false, false>, 4, 0, 0},
-      {k_bond_t<6, false, false>, 6, 0, 0}, {k_bond_t<3, true, false>, 3, 1, 0},
-      {k_bond_t<4, true, false>, 4, 1, 0},  {k_bond_t<5, true, false>, 5, 1, 0},
-      {k_bond_t<8, false, true>, 8, 0, 1},  {k_bond_t<3, true, true>, 3, 1, 1},
-      {k_bond_t<4, true, true>, 4, 1, 1}};
+      {k_bond_t<8, 0, false>, 8, 0, 0}, {k_bond_t<4, 0, false>, 4, 0, 0},
+      {k_bond_t<6, 0, false>, 6, 0, 0}, {k_bond_t<3, 1, false>, 3, 1, 0},
+      {k_bond_t<4, 1, false>, 4, 1, 0}, {k_bond_t<5, 1, false>, 5, 1, 0},
+      {k_bond_t<8, 0, true>, 8, 0, 1},  {k_bond_t<3, 1, true>, 3, 1, 1},
+      {k_bond_t<4, 1, true>, 4, 1, 1},  {k_bond_t<3, 2, false>, 3, 2, 0},
+      {k_bond_t<4, 2, false>, 4, 2, 0}, {k_bond_t<2, 2, false>, 2, 2, 0}};
   return t;
 }
 
