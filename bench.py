@@ -60,15 +60,16 @@ def traffic_bytes(nodes, bonds, kernel):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("nodes") != nodes or t.get("movable_bonds") != bonds or t.get("bond_kernel") != kernel:
+    if t.get("nodes") != nodes or t.get("movable_bonds") != bonds:
         return None
-    return t.get("hbm_bytes_per_launch")
+    # every shape was timed by the autotuner in the profiled run
+    return t.get("per_kernel_bytes", {}).get(kernel)
 
 
 def shape_name(k):
     from peridem_amd import abi
     u, pipe, mask = abi.bond_shapes()[k]
-    return f"pd::k_bond_t<{u}, {'true' if pipe else 'false'}, {'true' if mask else 'false'}>"
+    return f"pd::k_bond_t<{u}, {pipe}, {'true' if mask else 'false'}>"
 
 
 def cpu_baseline(s, nspheres, steps, threads):
@@ -184,7 +185,7 @@ def main():
     ap.add_argument("--spheres", type=int, default=125)
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-spheres", type=int, default=125)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")

@@ -507,13 +507,14 @@ class Pool:
 
 
 def bond_shapes():
-    """[(unroll, pipelined, masked)] of the bond-kernel shapes the library was built with."""
+    """[(unroll, pipeline, masked)] of the bond-kernel shapes the library was built with
+    (pipeline: 0 none, 1 software-pipelined, 2 pipelined and unrolled twice)."""
     lib, _ = load_gpu_lib()
     out = []
     for k in range(lib.pd_bond_shape_count()):
         u, p, m = C.c_int(), C.c_int(), C.c_int()
         lib.pd_bond_shape_info(k, C.byref(u), C.byref(p), C.byref(m))
-        out.append((u.value, bool(p.value), bool(m.value)))
+        out.append((u.value, p.value, bool(m.value)))
     return out
 
 
