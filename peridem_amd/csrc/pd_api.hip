@@ -38,6 +38,8 @@ constexpr uint64_t kTuneMinNodes = 1u << 18;  // autotune pools of at least this
 constexpr int kDefaultBondShape = 4;          // pipelined, 4 entries per batch
 constexpr double kSkinFrac = 0.3;             // candidate-list skin / R_c
 constexpr int kGraphSteps = 8;  // steps per captured graph
+constexpr int kProfRing = 16;   // event sets of profiled steps in flight
+constexpr int kProfLag = 8;     // a profiled step's marks are read this many steps later
 // device bytes per node slot (X, U, V, A, F, E, pid, nflag, deg, wmask, ckey,
 // csorted, ~2 hash buckets) and per bond entry
 constexpr uint64_t kBytesPerNode = 32 + 32 + 4 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16;
@@ -140,7 +142,7 @@ struct pd_ctx {
   pd_counters cnt{};
   double last_ms = 0;
   bool prof = false;
-  hipEvent_t pev[PD_NPHASE + 1] = {};
+  hipEvent_t pev[kProfRing][PD_NPHASE + 1] = {};    // phase marks of profiled steps
   pd_profile acc{};
   bool use_graph = true, graph_dirty = true;
   uint64_t hwm_slices = 0;      // end of the highest node range in use
@@ -1006,12 +1008,26 @@ static Dev step_view(const pd_ctx* c) {
   return d;
 }
 
-static int launch_step(pd_ctx* c) {
+// Phase times of one profiled step (waits for its last mark).
+static int accumulate_marks(pd_ctx* c, hipEvent_t* ev) {
+  HIPCHK(c, hipEventSynchronize(ev[PD_NPHASE]));
+  for (int k = 0; k < PD_NPHASE; k++) {
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+    c->acc.ms[k] += ms;
+    c->acc.launches[k] += 1;
+  }
+  return PD_OK;
+}
+
+// One step's launches; with `ev` set, a mark is recorded before each phase
+// and after the last.
+static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
   const Dev d = step_view(c);
   hipStream_t st = c->stream;
   const uint64_t Np = d.S * kSlice;
   auto mark = [&](int k) -> int {
-    if (c->prof) HIPCHK(c, hipEventRecord(c->pev[k], st));
+    if (ev) HIPCHK(c, hipEventRecord(ev[k], st));
     return PD_OK;
   };
   int rc;
@@ -1048,15 +1064,6 @@ static int launch_step(pd_ctx* c) {
   hipLaunchKernelGGL(k_finalize, dim3(grid_of(std::max<uint64_t>(d.P, d.ninst), 256)), dim3(256), 0, st, d);
   if ((rc = mark(5))) return rc;
   HIPCHK(c, hipGetLastError());
-  if (c->prof) {
-    HIPCHK(c, hipEventSynchronize(c->pev[PD_NPHASE]));
-    for (int k = 0; k < PD_NPHASE; k++) {
-      float ms = 0;
-      HIPCHK(c, hipEventElapsedTime(&ms, c->pev[k], c->pev[k + 1]));
-      c->acc.ms[k] += ms;
-      c->acc.launches[k] += 1;
-    }
-  }
   return PD_OK;
 }
 
@@ -1174,12 +1181,25 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, &c->d.ctl->steps_left, (uint64_t)1, nsteps);
   HIPCHK(c, hipGetLastError());
   uint32_t done = 0;
-  if (c->use_graph && !c->prof && nsteps >= (uint32_t)kGraphSteps) {
-    if ((rc = ensure_graph(c))) return rc;
-    for (; done + kGraphSteps <= nsteps; done += kGraphSteps) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  if (c->prof) {
+    // marks cannot be timed from inside a graph: profiled steps launch one
+    // by one, and each step's marks are read kProfLag steps later, so the
+    // host waits on work that finished long ago and the queue stays full
+    for (; done < nsteps; done++) {
+      if (done >= (uint32_t)kProfLag && (rc = accumulate_marks(c, c->pev[(done - kProfLag) % kProfRing])))
+        return rc;
+      if ((rc = launch_step(c, c->pev[done % kProfRing]))) return rc;
+    }
+    for (uint32_t i = nsteps > (uint32_t)kProfLag ? nsteps - kProfLag : 0; i < nsteps; i++)
+      if ((rc = accumulate_marks(c, c->pev[i % kProfRing]))) return rc;
+  } else {
+    if (c->use_graph && nsteps >= (uint32_t)kGraphSteps) {
+      if ((rc = ensure_graph(c))) return rc;
+      for (; done + kGraphSteps <= nsteps; done += kGraphSteps) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    }
+    for (; done < nsteps; done++)
+      if ((rc = launch_step(c))) return rc;
   }
-  for (; done < nsteps; done++)
-    if ((rc = launch_step(c))) return rc;
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
@@ -1634,8 +1654,9 @@ int pd_get_counters(pd_ctx* c, pd_counters* out) {
 
 int pd_set_profiling(pd_ctx* c, int on) {
   if (!c) return PD_E_ARG;
-  if (on && !c->pev[0])
-    for (int k = 0; k <= PD_NPHASE; k++) HIPCHK(c, hipEventCreate(&c->pev[k]));
+  if (on && !c->pev[0][0])
+    for (int s = 0; s < kProfRing; s++)
+      for (int k = 0; k <= PD_NPHASE; k++) HIPCHK(c, hipEventCreate(&c->pev[s][k]));
   c->prof = on != 0;
   c->acc = pd_profile{};
   return PD_OK;
@@ -1693,8 +1714,9 @@ void pd_destroy(pd_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  for (int k = 0; k <= PD_NPHASE; k++)
-    if (c->pev[k]) (void)hipEventDestroy(c->pev[k]);
+  for (int s = 0; s < kProfRing; s++)
+    for (int k = 0; k <= PD_NPHASE; k++)
+      if (c->pev[s][k]) (void)hipEventDestroy(c->pev[s][k]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -174,3 +174,40 @@ def test_reused_particle_slots_keep_pair_roles():
         compare_job(pool, jc, ref, "job in reused slots")
     finally:
         pool.close()
+
+
+def test_profiled_graph_replay_keeps_results_and_times_every_step():
+    """With profiling on, steps launch one by one with phase marks that are
+    read some steps later; the bits equal an unprofiled (graph) run, every
+    step of every phase is counted once, calls shorter and longer than the
+    read lag included, and the phase times add up to no more than the
+    device time of the calls."""
+    s = small_jobs(4, seed=37)[0]
+    a, b = abi.gpu_engine(s), abi.gpu_engine(s)
+    try:
+        a.set_profiling(True)
+        dev_ms, total = 0.0, 0
+        for n in (8, 24, 5, 16):
+            a.step(n)
+            b.step(n)
+            dev_ms += a.last_step_ms()
+            total += n
+        pr = a.profile()
+        for name, ph in pr.items():
+            assert ph["launches"] == total, (name, ph)
+            assert ph["ms"] >= 0.0, (name, ph)
+        summed = sum(ph["ms"] for ph in pr.values())
+        assert 0.0 < summed <= dev_ms * 1.02 + 0.05, (summed, dev_ms)
+        assert pr["bond"]["ms"] > 0.0
+        for p in range(len(s.particles)):
+            sa, sb = a.state(p), b.state(p)
+            for k in FIELDS:
+                assert_bitwise(sa[k], sb[k], f"particle {p} {k}")
+        a.set_profiling(False)
+        a.step(8)
+        b.step(8)
+        for p in range(len(s.particles)):
+            assert_bitwise(a.state(p)["disp"], b.state(p)["disp"], f"after profiling, particle {p}")
+    finally:
+        a.close()
+        b.close()
