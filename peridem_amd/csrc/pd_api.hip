@@ -42,7 +42,7 @@ constexpr int kProfRing = 16;   // event sets of profiled steps in flight
 constexpr int kProfLag = 8;     // a profiled step's marks are read this many steps later
 // device bytes per node slot (X, U, V, A, F, E, pid, nflag, deg, wmask, ckey,
 // csorted, ~2 hash buckets) and per bond entry
-constexpr uint64_t kBytesPerNode = 32 + 32 + 4 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16;
+constexpr uint64_t kBytesPerNode = 32 + 32 + 4 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16 + 56;
 constexpr uint64_t kBytesPerEntry = 4;
 
 struct HostPart {
@@ -499,6 +499,10 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   if ((rc = dalloc(c, &d.f, (n)))) return rc;
   ALLOC(X, N);
   ALLOC(U, N);
+  ALLOC(XY, N);
+  ALLOC(ZV, N);
+  ALLOC(UXY, N);
+  ALLOC(UZ, N);
   ALLOC(V, 3 * N);
   ALLOC(A, 3 * N);
   ALLOC(F, 3 * N);
@@ -540,6 +544,10 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemsetAsync(d.X, 0, N * sizeof(double4), st));
   HIPCHK(c, hipMemsetAsync(d.U, 0, N * sizeof(double4), st));
+  HIPCHK(c, hipMemsetAsync(d.XY, 0, N * sizeof(double2), st));
+  HIPCHK(c, hipMemsetAsync(d.ZV, 0, N * sizeof(double2), st));
+  HIPCHK(c, hipMemsetAsync(d.UXY, 0, N * sizeof(double2), st));
+  HIPCHK(c, hipMemsetAsync(d.UZ, 0, N * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.V, 0, 3 * N * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.A, 0, 3 * N * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.F, 0, 3 * N * sizeof(double), st));
@@ -861,6 +869,8 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
 #define UP(dst, src, n) HIPCHK(c, hipMemcpy((dst), (src), (n) * sizeof(*(dst)), hipMemcpyHostToDevice))
   UP(d.X + g0, B->X.data(), Np);
   UP(d.U + g0, B->U.data(), Np);
+  hipLaunchKernelGGL(k_split, dim3(grid_of(Np, 256)), dim3(256), 0, c->stream, d, g0, Np);
+  HIPCHK(c, hipGetLastError());
   UP(d.V + 3 * g0, B->V.data(), 3 * Np);
   UP(d.A + 3 * g0, B->A.data(), 3 * Np);
   UP(d.E + 3 * g0, B->E.data(), 3 * Np);
@@ -1008,6 +1018,9 @@ static Dev step_view(const pd_ctx* c) {
   return d;
 }
 
+// Node records of pools this large are addressed with 64-bit offsets.
+static bool wide_pool(const pd_ctx* c) { return c->cap_slices * kSlice > kOff32MaxNodes; }
+
 // Phase times of one profiled step (waits for its last mark).
 static int accumulate_marks(pd_ctx* c, hipEvent_t* ev) {
   HIPCHK(c, hipEventSynchronize(ev[PD_NPHASE]));
@@ -1050,7 +1063,7 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
   }
   if ((rc = mark(2))) return rc;
   // grid padded to a multiple of the XCD count (the bond kernel remaps blocks)
-  hipLaunchKernelGGL(bond_shapes()[c->bond_shape < 0 ? 0 : c->bond_shape].fn,
+  hipLaunchKernelGGL(wide_pool(c) ? bond_wide().fn : bond_shapes()[c->bond_shape < 0 ? 0 : c->bond_shape].fn,
                      dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
   if ((rc = mark(3))) return rc;
   hipLaunchKernelGGL(k_contact, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
@@ -1069,8 +1082,13 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
 
 // Time every bond-kernel shape on the resident pool and keep the fastest.
 // The timed launches write their force sums and breaks into scratch copies,
-// so the state is untouched.
+// so the state is untouched.  Pools too large for 32-bit record offsets have
+// one shape only.
 static int autotune(pd_ctx* c, uint32_t reps) {
+  if (wide_pool(c)) {
+    c->bond_shape = kDefaultBondShape;
+    return PD_OK;
+  }
   Dev t = step_view(c);
   const uint64_t Np = t.S * kSlice;
   int rc = PD_OK;
@@ -1543,6 +1561,8 @@ int pd_set_state(pd_ctx* c, uint32_t pi, const double* disp, const double* vel, 
     std::vector<double4> U(p.n);
     for (uint32_t i = 0; i < p.n; i++) U[i] = make_double4(disp[3 * i], disp[3 * i + 1], disp[3 * i + 2], 0.0);
     HIPCHK(c, hipMemcpy(c->d.U + p.off, U.data(), p.n * sizeof(double4), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_split, dim3(grid_of(p.n, 256)), dim3(256), 0, c->stream, c->d, p.off, (uint64_t)p.n);
+    HIPCHK(c, hipGetLastError());
   }
   if (vel) HIPCHK(c, hipMemcpy(c->d.V + 3 * p.off, vel, 3ull * p.n * sizeof(double), hipMemcpyHostToDevice));
   if (acc) HIPCHK(c, hipMemcpy(c->d.A + 3 * p.off, acc, 3ull * p.n * sizeof(double), hipMemcpyHostToDevice));

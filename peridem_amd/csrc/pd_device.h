@@ -72,6 +72,13 @@ struct Dev {
   // node state, global node index g
   double4* X;        // x, y, z, vol
   double4* U;        // disp x, y, z, (unused)
+  // the bond kernel's neighbour copies of X and U in 16- and 8-byte records:
+  // a wavefront's gather of one field then spans fewer cache lines than the
+  // same field read out of 32-byte records
+  double2* XY;       // x, y
+  double2* ZV;       // z, vol
+  double2* UXY;      // disp x, y   (rewritten by the position kernel each step)
+  double* UZ;        // disp z
   double* V;         // vel [3N]
   double* A;         // acc [3N]
   double* F;         // force density [3N]

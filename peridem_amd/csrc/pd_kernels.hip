@@ -26,6 +26,20 @@ __device__ __forceinline__ bool reaction_step(const Dev& d, const DevInst& I) {
 }
 
 // ---------------------------------------------------------------------------
+// The bond kernel's neighbour copies of nodes [g0, g0+n) from X and U (after
+// an upload; the position kernel keeps the displacement copies current).
+__global__ __launch_bounds__(256) void k_split(Dev d, uint64_t g0, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t g = g0 + i;
+  const double4 x = d.X[g], u = d.U[g];
+  d.XY[g] = make_double2(x.x, x.y);
+  d.ZV[g] = make_double2(x.z, x.w);
+  d.UXY[g] = make_double2(u.x, u.y);
+  d.UZ[g] = u.z;
+}
+
+// ---------------------------------------------------------------------------
 // Verlet position half (timeloop.h:537-540; overloads.h:51-70,149-159):
 //   disp += dt*vel + (dt*dt*0.5)*acc ; CurrPos = pos + disp
 // plus the min/max of CurrPos that becomes the particle's lagged box at the
@@ -52,6 +66,8 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
     u.y = u.y + ((dt * v1) + (h * a1));
     u.z = u.z + ((dt * v2) + (h * a2));
     d.U[g] = u;
+    d.UXY[g] = make_double2(u.x, u.y);
+    d.UZ[g] = u.z;
     if (!(in_fast_range(u.x) & in_fast_range(u.y) & in_fast_range(u.z))) atomicOr(d.flags, kFlagSafeMath);
     const double4 ub = d.UB[g];
     const double e0 = u.x - ub.x, e1 = u.y - ub.y, e2 = u.z - ub.z;
@@ -404,65 +420,128 @@ __device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) 
 // gather the same neighbour records, share that XCD's L2.
 constexpr int kXcds = 8;
 
-// The k-th entries of the U rows-batches starting at k (kPad past the width).
+// High word of a double, and a double with its high word replaced.
+__device__ __forceinline__ uint32_t hi_word(double v) {
+  return (uint32_t)((unsigned long long)__double_as_longlong(v) >> 32);
+}
+__device__ __forceinline__ double with_hi(double v, uint32_t h) {
+  const unsigned long long lo = (unsigned long long)__double_as_longlong(v) & 0xFFFFFFFFull;
+  return __longlong_as_double((long long)(((unsigned long long)h << 32) | lo));
+}
+// |e|^2 in [2^-600, 2^600) <=> (high word - kN2Lo) < kN2Span as unsigned
+// (0, denormals, NaN, inf and negative values all fall outside).
+constexpr uint32_t kN2Lo = 0x1A700000u;                 // high word of 2^-600
+constexpr uint32_t kN2Span = 0x65700000u - kN2Lo;       // high word of 2^600, minus kN2Lo
+constexpr uint32_t kOneHi = 0x3FF00000u;                // high word of 1.0 (low word 0)
+
+// The slice's bond entries as one uniform pointer per batch entry plus the
+// lane's byte offset, so index loads and break stores take no vector address
+// arithmetic.
+struct SliceRows {
+  const char* base;   // entry 0 of the slice (wave-uniform)
+  uint32_t lane4;     // 4 * lane
+  __device__ __forceinline__ uint32_t* at(uint32_t k) const {
+    return (uint32_t*)(base + (uint64_t)k * (kSlice * sizeof(uint32_t)) + lane4);
+  }
+};
+
+// The k-th entries of the U rows-batches starting at k (kPad past the width,
+// W >= 1).  Every load is issued -- past the width it rereads entry W-1 and
+// the value is replaced -- so the loop body has no load under a branch and
+// the compiler can count outstanding loads exactly instead of draining them.
 template <int U>
-__device__ __forceinline__ void bond_idx(const uint32_t* bp, uint32_t k, uint32_t W, uint32_t (&e)[U]) {
+__device__ __forceinline__ void bond_idx(const SliceRows& R, uint32_t k, uint32_t W, uint32_t (&e)[U]) {
 #pragma unroll
-  for (int u = 0; u < U; u++) e[u] = (k + u < W) ? bp[(uint64_t)(k + u) * kSlice] : kPad;
+  for (int u = 0; u < U; u++) {
+    const bool in = k + u < W;
+    const uint32_t v = *R.at(in ? k + u : W - 1);
+    e[u] = in ? v : kPad;
+  }
+}
+
+// Node j's neighbour record, read from the split copies (x,y | z,vol |
+// ux,uy | uz).  OFF32: the pool spans fewer than 2^27 node slots, so each
+// byte offset is a 32-bit shift and the loads take the array bases from
+// scalar registers.
+template <bool OFF32>
+__device__ __forceinline__ void rec(const Dev& d, uint32_t j, double4& x, double4& u) {
+  double2 xy, zv, uxy;
+  double uz;
+  if (OFF32) {
+    const uint32_t o8 = j << 3, o16 = j << 4;
+    xy = *(const double2*)((const char*)d.XY + o16);
+    zv = *(const double2*)((const char*)d.ZV + o16);
+    uxy = *(const double2*)((const char*)d.UXY + o16);
+    uz = *(const double*)((const char*)d.UZ + o8);
+  } else {
+    xy = d.XY[j];
+    zv = d.ZV[j];
+    uxy = d.UXY[j];
+    uz = d.UZ[j];
+  }
+  x = make_double4(xy.x, xy.y, zv.x, zv.y);
+  u = make_double4(uxy.x, uxy.y, uz, 0.0);
 }
 
 // Neighbour records of a batch.  A dead entry reads the node itself
 // (MASK = false, branch-free) or issues no load at all (MASK = true: its
 // lanes are switched off for the loads and take the node's own record).
-template <int U, bool MASK>
-__device__ __forceinline__ void bond_gather(const Dev& d, uint64_t g, const uint32_t (&e)[U], const double4& xi,
+template <int U, bool MASK, bool OFF32>
+__device__ __forceinline__ void bond_gather(const Dev& d, uint32_t g, const uint32_t (&e)[U], const double4& xi,
                                             const double4& ui, double4 (&xj)[U], double4 (&uj)[U]) {
 #pragma unroll
   for (int u = 0; u < U; u++) {
     if (MASK) {
-      if (!(e[u] & kBroken)) {
-        xj[u] = d.X[e[u]];
-        uj[u] = d.U[e[u]];
+      if ((int32_t)e[u] >= 0) {
+        rec<OFF32>(d, e[u], xj[u], uj[u]);
       } else {
         xj[u] = xi;
         uj[u] = ui;
       }
     } else {
-      const uint32_t j = (e[u] & kBroken) ? (uint32_t)g : e[u];
-      xj[u] = d.X[j];
-      uj[u] = d.U[j];
+      rec<OFF32>(d, ((int32_t)e[u] < 0) ? g : e[u], xj[u], uj[u]);
     }
   }
 }
 
 // Terms of one batch, added to (f0, f1, f2) in row order; breaks recorded.
+//
+// Fast domain (every lane of the wave): a dead entry carries the node's own
+// record, so with finite operands its e and xi are exactly +0 and |e|^2 =
+// |xi|^2 = +0; giving both the high word of 1.0 makes them exactly 1.0, the
+// stretch exactly +0 and the coefficient +-0 without a select on it.  The
+// IEEE branch (safe mode, or a lane outside the domain) selects explicitly.
 template <int U>
-__device__ __forceinline__ void bond_terms(uint32_t* bp, uint32_t k, const uint32_t (&e)[U], const double4 (&xj)[U],
-                                           const double4 (&uj)[U], const double4& xi, const double4& ui, bool fast,
-                                           bool can_break, double cnot, double snot, double& f0, double& f1,
-                                           double& f2) {
+__device__ __forceinline__ void bond_terms(const SliceRows& R, uint32_t k, const uint32_t (&e)[U],
+                                           const double4 (&xj)[U], const double4 (&uj)[U], const double4& xi,
+                                           const double4& ui, bool fast, bool can_break, double cnot, double snot,
+                                           double& f0, double& f1, double& f2) {
   bool alive[U];
-  double ev[U][3], n2[U], l2[U];
-  bool ok = fast;
+  double ev[U][3], xv[U][3], n2[U], l2[U];
+  uint32_t over = 0;
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    alive[u] = !(e[u] & kBroken);
-    double x0 = xj[u].x - xi.x, x1 = xj[u].y - xi.y, x2 = xj[u].z - xi.z;  // xi (gen_xi)
-    ev[u][0] = (uj[u].x - ui.x) + x0;
-    ev[u][1] = (uj[u].y - ui.y) + x1;
-    ev[u][2] = (uj[u].z - ui.z) + x2;
-    n2[u] = alive[u] ? (ev[u][0] * ev[u][0] + ev[u][1] * ev[u][1]) + ev[u][2] * ev[u][2] : 1.0;
-    l2[u] = alive[u] ? (x0 * x0 + x1 * x1) + x2 * x2 : 1.0;
-    ok &= (n2[u] >= kFastLo) & (n2[u] <= kFastHi);
+    alive[u] = (int32_t)e[u] >= 0;
+    xv[u][0] = xj[u].x - xi.x;   // xi (gen_xi)
+    xv[u][1] = xj[u].y - xi.y;
+    xv[u][2] = xj[u].z - xi.z;
+    ev[u][0] = (uj[u].x - ui.x) + xv[u][0];
+    ev[u][1] = (uj[u].y - ui.y) + xv[u][1];
+    ev[u][2] = (uj[u].z - ui.z) + xv[u][2];
+    const double a2 = (ev[u][0] * ev[u][0] + ev[u][1] * ev[u][1]) + ev[u][2] * ev[u][2];
+    const double b2 = (xv[u][0] * xv[u][0] + xv[u][1] * xv[u][1]) + xv[u][2] * xv[u][2];
+    n2[u] = with_hi(a2, alive[u] ? hi_word(a2) : kOneHi);
+    l2[u] = with_hi(b2, alive[u] ? hi_word(b2) : kOneHi);
+    over = max(over, hi_word(n2[u]) - kN2Lo);
   }
   double str[U], t[U][3];
-  if (__all(ok)) {
+  if (__all(fast & (over < kN2Span))) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       double nn = sqrt_core(n2[u]);
       double L = sqrt_core(l2[u]);
       str[u] = div_core(nn - L, rcp_core(L));
-      double c = alive[u] ? (cnot * str[u]) * xj[u].w : 0.0;
+      double c = (cnot * str[u]) * xj[u].w;
       Rcp rn = rcp_core(nn);
 #pragma unroll
       for (int q = 0; q < 3; q++) t[u][q] = c * div_core(ev[u][q], rn);
@@ -470,12 +549,14 @@ __device__ __forceinline__ void bond_terms(uint32_t* bp, uint32_t k, const uint3
   } else {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      double nn = sqrt(n2[u]);
-      double L = sqrt(l2[u]);
+      const double a2 = (ev[u][0] * ev[u][0] + ev[u][1] * ev[u][1]) + ev[u][2] * ev[u][2];
+      const double b2 = (xv[u][0] * xv[u][0] + xv[u][1] * xv[u][1]) + xv[u][2] * xv[u][2];
+      double nn = sqrt(alive[u] ? a2 : 1.0);
+      double L = sqrt(alive[u] ? b2 : 1.0);
       str[u] = (nn - L) / L;
-      double c = alive[u] ? (cnot * str[u]) * xj[u].w : 0.0;
+      double c = (cnot * str[u]) * xj[u].w;
 #pragma unroll
-      for (int q = 0; q < 3; q++) t[u][q] = c * (ev[u][q] / nn);
+      for (int q = 0; q < 3; q++) t[u][q] = alive[u] ? c * (ev[u][q] / nn) : 0.0;
     }
   }
 #pragma unroll
@@ -483,15 +564,15 @@ __device__ __forceinline__ void bond_terms(uint32_t* bp, uint32_t k, const uint3
     f0 += t[u][0];
     f1 += t[u][1];
     f2 += t[u][2];
-    if (alive[u] & can_break & (str[u] > snot)) bp[(uint64_t)(k + u) * kSlice] = e[u] | kBroken;
+    if (alive[u] & can_break & (str[u] > snot)) *R.at(k + u) = e[u] | kBroken;
   }
 }
 
 // U: entries per batch.  MASK: see bond_gather.  PIPE: software pipelining -- the indices of batch
 // k+2 and the neighbour records of batch k+1 are in flight while batch k is
 // evaluated, so the index stream (HBM) and the gathers (L2) overlap the FP64
-// arithmetic instead of stalling every batch.
-template <int U, int PIPE, bool MASK>
+// arithmetic instead of stalling every batch.  OFF32: see rec.
+template <int U, int PIPE, bool MASK, bool OFF32>
 __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
   const uint64_t nbp = gridDim.x;   // a multiple of kXcds
   const uint64_t blk = (blockIdx.x % kXcds) * (nbp / kXcds) + blockIdx.x / kXcds;
@@ -513,34 +594,35 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b0 >> 32));
     const uint64_t base = ((uint64_t)bhi << 32) | (uint64_t)blo;
     const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)d.swid[s]);
-    uint32_t* bp = d.bond + base + (g & 63);
+    const SliceRows R{(const char*)(d.bond + base), (uint32_t)(g & 63) * 4u};
+    const uint32_t gi = (uint32_t)g;
     const double4 xi = d.X[g];
     const double4 ui = d.U[g];
     if (PIPE == 0) {
       for (uint32_t k = 0; k < W; k += U) {
         uint32_t e[U];
         double4 xj[U], uj[U];
-        bond_idx<U>(bp, k, W, e);
-        bond_gather<U, MASK>(d, g, e, xi, ui, xj, uj);
-        bond_terms<U>(bp, k, e, xj, uj, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+        bond_idx<U>(R, k, W, e);
+        bond_gather<U, MASK, OFF32>(d, gi, e, xi, ui, xj, uj);
+        bond_terms<U>(R, k, e, xj, uj, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
       }
     } else if (PIPE == 2 && W) {
       // the same pipeline unrolled twice: the neighbour records alternate
       // between two register sets instead of being copied every batch
       uint32_t ea[U], eb[U];
       double4 xa[U], ua[U], xb[U], ub[U];
-      bond_idx<U>(bp, 0, W, ea);
-      bond_idx<U>(bp, U, W, eb);
-      bond_gather<U, MASK>(d, g, ea, xi, ui, xa, ua);
+      bond_idx<U>(R, 0, W, ea);
+      bond_idx<U>(R, U, W, eb);
+      bond_gather<U, MASK, OFF32>(d, gi, ea, xi, ui, xa, ua);
       for (uint32_t k = 0; k < W; k += 2 * U) {
         uint32_t ec[U];
-        bond_idx<U>(bp, k + 2 * U, W, ec);
-        bond_gather<U, MASK>(d, g, eb, xi, ui, xb, ub);
-        bond_terms<U>(bp, k, ea, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+        bond_idx<U>(R, k + 2 * U, W, ec);
+        bond_gather<U, MASK, OFF32>(d, gi, eb, xi, ui, xb, ub);
+        bond_terms<U>(R, k, ea, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
         if (k + U >= W) break;
-        bond_idx<U>(bp, k + 3 * U, W, ea);
-        bond_gather<U, MASK>(d, g, ec, xi, ui, xa, ua);
-        bond_terms<U>(bp, k + U, eb, xb, ub, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+        bond_idx<U>(R, k + 3 * U, W, ea);
+        bond_gather<U, MASK, OFF32>(d, gi, ec, xi, ui, xa, ua);
+        bond_terms<U>(R, k + U, eb, xb, ub, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
 #pragma unroll
         for (int u = 0; u < U; u++) {
           eb[u] = ea[u];   // batch k + 3U
@@ -550,15 +632,15 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     } else if (W) {
       uint32_t e0[U], e1[U];
       double4 xa[U], ua[U];
-      bond_idx<U>(bp, 0, W, e0);
-      bond_idx<U>(bp, U, W, e1);
-      bond_gather<U, MASK>(d, g, e0, xi, ui, xa, ua);
+      bond_idx<U>(R, 0, W, e0);
+      bond_idx<U>(R, U, W, e1);
+      bond_gather<U, MASK, OFF32>(d, gi, e0, xi, ui, xa, ua);
       for (uint32_t k = 0; k < W; k += U) {
         uint32_t e2[U];
         double4 xb[U], ub[U];
-        bond_idx<U>(bp, k + 2 * U, W, e2);
-        bond_gather<U, MASK>(d, g, e1, xi, ui, xb, ub);
-        bond_terms<U>(bp, k, e0, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+        bond_idx<U>(R, k + 2 * U, W, e2);
+        bond_gather<U, MASK, OFF32>(d, gi, e1, xi, ui, xb, ub);
+        bond_terms<U>(R, k, e0, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
 #pragma unroll
         for (int u = 0; u < U; u++) {
           e0[u] = e1[u];
@@ -576,22 +658,27 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
 
 // Instantiated shapes of the bond kernel; the autotuner times them on the
 // resident pool and keeps the fastest.  Every shape sums each row in row order,
-// so all give the same bits.
+// so all give the same bits.  They address node records with 32-bit offsets;
+// pools of 2^27 node slots or more use kBondWide.
 struct BondShape {
   void (*fn)(Dev);
   int unroll, pipe, mask;   // pipe: 0 none, 1 pipelined, 2 pipelined and unrolled twice
 };
 constexpr int kBondShapes = 12;
+constexpr uint64_t kOff32MaxNodes = 1ull << 27;   // 16-byte records far below 4 GiB
 __host__ inline const BondShape* bond_shapes() {
   static const BondShape t[kBondShapes] = {
-      {k_bond_t<8, 0, false>, 8, 0, 0}, {k_bond_t<4, 0, false>, 4, 0, 0},
-      {k_bond_t<6, 0, false>, 6, 0, 0}, {k_bond_t<3, 1, false>, 3, 1, 0},
-      {k_bond_t<4, 1, false>, 4, 1, 0}, {k_bond_t<5, 1, false>, 5, 1, 0},
-      {k_bond_t<8, 0, true>, 8, 0, 1},  {k_bond_t<3, 1, true>, 3, 1, 1},
-      {k_bond_t<4, 1, true>, 4, 1, 1},  {k_bond_t<3, 2, false>, 3, 2, 0},
-      {k_bond_t<4, 2, false>, 4, 2, 0}, {k_bond_t<2, 2, false>, 2, 2, 0}};
+      {k_bond_t<8, 0, false, true>, 8, 0, 0}, {k_bond_t<4, 0, false, true>, 4, 0, 0},
+      {k_bond_t<6, 0, false, true>, 6, 0, 0}, {k_bond_t<3, 1, false, true>, 3, 1, 0},
+      {k_bond_t<4, 1, false, true>, 4, 1, 0}, {k_bond_t<5, 1, false, true>, 5, 1, 0},
+      {k_bond_t<8, 0, true, true>, 8, 0, 1},  {k_bond_t<3, 1, true, true>, 3, 1, 1},
+      {k_bond_t<4, 1, true, true>, 4, 1, 1},  {k_bond_t<3, 2, false, true>, 3, 2, 0},
+      {k_bond_t<4, 2, false, true>, 4, 2, 0}, {k_bond_t<2, 2, false, true>, 2, 2, 0}};
   return t;
 }
+// The default shape with 64-bit record addressing, for pools too large for
+// 32-bit offsets (never tuned).
+__host__ inline BondShape bond_wide() { return {k_bond_t<4, 2, false, false>, 4, 2, 0}; }
 
 // ---------------------------------------------------------------------------
 // Contact, wall and external force, added to the PD sum in the reference's
