@@ -17,6 +17,10 @@ roofline: the bond-force kernel's algorithmic bytes per launch
 with HIP events recorded on the library's stream around every phase of the
 timed steps themselves (profiled steps launch one by one; the same steps
 replayed from graphs without marks are timed beside them and logged).
+fp64_issue: FP64 lane-instructions per second of the same kernel launch
+(wave-instruction count per launch from the committed rocprofv3
+SQ_INSTS_VALU_*_F64 summary, times 64 lanes, over the live-measured average
+duration) against the FP64 vector issue peak.
 cpu_baseline: the CPU oracle (same algorithm, OpenMP threads emulating the
 reference's MPI ranks) on a bounded sample of the same workload.
 
@@ -68,10 +72,29 @@ def traffic_bytes(nodes, bonds, kernel):
     return t.get("per_kernel_bytes", {}).get(kernel)
 
 
-def shape_name(k):
+# FP64 vector issue peak: 256 CUs x 4 SIMDs x 16 lanes per clock x 2.4 GHz
+# (78.6 TFLOP/s counts each FMA lane-instruction as two flops)
+FP64_LANE_INSTR_PEAK = 256 * 64 * 2.4e9
+
+
+def fp64_instr(kernel):
+    """FP64 wave-instructions per launch of `kernel` from the committed
+    rocprofv3 SQ_INSTS_VALU_*_F64 summary (same workload and shape)."""
+    path = os.path.join(ROOT, "profiles", "k_bond_gravity_settling_fp64.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["per_kernel_fp64_wave_instr"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def shape_name(k, nodes):
+    """Kernel symbol of bond shape k (pools below 2^27 node slots address
+    node records with 32-bit offsets)."""
     from peridem_amd import abi
     u, pipe, mask = abi.bond_shapes()[k]
-    return f"pd::k_bond_t<{u}, {pipe}, {'true' if mask else 'false'}>"
+    off32 = "true" if nodes < (1 << 27) else "false"
+    return f"pd::k_bond_t<{u}, {pipe}, {'true' if mask else 'false'}, {off32}>"
 
 
 def cpu_baseline(s, nspheres, steps, threads):
@@ -253,7 +276,8 @@ def main():
     log(f"[bench] phases per step (ms): " +
         ", ".join(f"{k}={v['ms'] / max(v['launches'], 1):.4f}" for k, v in prof.items()))
     cnt = e.counters()
-    kname = shape_name(e.bond_shape())
+    kname = shape_name(e.bond_shape(), N)
+    fi = fp64_instr(kname)
     e.close()
 
     if probe is not None:
@@ -296,6 +320,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(N, Bm, kname),
                      "kernel": kname, "bytes_per_launch": force_bytes,
                      "avg_launch_ms": force_ms},
+        "fp64_issue": None if fi is None else {
+            "wave_instr_per_launch": fi, "achieved": fi * 64 / (force_ms * 1e-3) / 1e12,
+            "peak": FP64_LANE_INSTR_PEAK / 1e12, "unit": "T FP64 lane-instructions/s",
+            "frac": fi * 64 / (force_ms * 1e-3) / FP64_LANE_INSTR_PEAK},
         "cpu_baseline": cpu,
         "batched": bat,
         "serving": srv,

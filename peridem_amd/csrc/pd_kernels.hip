@@ -596,8 +596,10 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)d.swid[s]);
     const SliceRows R{(const char*)(d.bond + base), (uint32_t)(g & 63) * 4u};
     const uint32_t gi = (uint32_t)g;
-    const double4 xi = d.X[g];
-    const double4 ui = d.U[g];
+    // the node's own record from the same copies its neighbours read, so
+    // each node's data is fetched from HBM once per launch
+    double4 xi, ui;
+    rec<false>(d, gi, xi, ui);
     if (PIPE == 0) {
       for (uint32_t k = 0; k < W; k += U) {
         uint32_t e[U];

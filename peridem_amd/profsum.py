@@ -5,6 +5,9 @@
   python -m peridem_amd.profsum traffic <fetch_dir> <write_dir> <copy_mib> <nodes> <bonds>
         HBM bytes per launch of every kernel, FETCH_SIZE/WRITE_SIZE scaled by
         what the counters read for the plain device copy of copy_mib MiB
+  python -m peridem_amd.profsum fp64 <dir> [out.json]
+        FP64 wave-instructions per launch of every kernel (SQ_INSTS_VALU_ADD/
+        MUL/FMA/TRANS_F64), for the FP64 issue utilisation
 """
 import collections
 import csv
@@ -80,6 +83,27 @@ def traffic(fd, wd, copy_mib, nodes, bonds):
             "per_kernel_bytes": res}
 
 
+FP64_COUNTERS = ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64",
+                 "SQ_INSTS_VALU_TRANS_F64")
+
+
+def fp64(d):
+    a = _agg(d)
+    print("| kernel | ADD | MUL | FMA | TRANS | FP64 wave-instructions per launch | all VALU |")
+    print("|---|---|---|---|---|---|---|")
+    res = {}
+    for k, cs in a.items():
+        if not k.startswith("pd::") or FP64_COUNTERS[0] not in cs:
+            continue
+        m = {c: sum(cs[c]) / len(cs[c]) for c in FP64_COUNTERS}
+        tot = sum(m.values())
+        valu = sum(cs["SQ_INSTS_VALU"]) / len(cs["SQ_INSTS_VALU"]) if "SQ_INSTS_VALU" in cs else float("nan")
+        res[k] = tot
+        print(f"| `{k}` | {m[FP64_COUNTERS[0]]:.0f} | {m[FP64_COUNTERS[1]]:.0f} | {m[FP64_COUNTERS[2]]:.0f} | "
+              f"{m[FP64_COUNTERS[3]]:.0f} | {tot:.0f} | {valu:.0f} |")
+    return {"per_kernel_fp64_wave_instr": res}
+
+
 if __name__ == "__main__":
     cmd = sys.argv[1]
     if cmd == "trace":
@@ -90,4 +114,9 @@ if __name__ == "__main__":
         t = traffic(sys.argv[2], sys.argv[3], int(sys.argv[4]), int(sys.argv[5]), int(sys.argv[6]))
         if len(sys.argv) > 7:
             with open(sys.argv[7], "w") as fh:
+                json.dump(t, fh, indent=1)
+    elif cmd == "fp64":
+        t = fp64(sys.argv[2])
+        if len(sys.argv) > 3:
+            with open(sys.argv[3], "w") as fh:
                 json.dump(t, fh, indent=1)
