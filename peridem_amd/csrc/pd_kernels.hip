@@ -449,12 +449,14 @@ struct SliceRows {
 // W >= 1).  Every load is issued -- past the width it rereads entry W-1 and
 // the value is replaced -- so the loop body has no load under a branch and
 // the compiler can count outstanding loads exactly instead of draining them.
+// The entries are read once per launch: non-temporal loads, which leave the
+// caches to the neighbour records.
 template <int U>
 __device__ __forceinline__ void bond_idx(const SliceRows& R, uint32_t k, uint32_t W, uint32_t (&e)[U]) {
 #pragma unroll
   for (int u = 0; u < U; u++) {
     const bool in = k + u < W;
-    const uint32_t v = *R.at(in ? k + u : W - 1);
+    const uint32_t v = __builtin_nontemporal_load(R.at(in ? k + u : W - 1));
     e[u] = in ? v : kPad;
   }
 }

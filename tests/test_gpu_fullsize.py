@@ -1,0 +1,100 @@
+"""The headline workload at its full size: 125 spheres, 1,003,125 nodes,
+92,664,286 directed bonds (the bench's gravity-settling pool).
+
+The oracle cannot step the whole pool in seconds, so the full-size checks are
+properties that hold at any size, plus bitwise parity of a sample:
+
+* rigid motion: with one velocity per sphere and no gravity every bond keeps
+  exactly zero stretch, so every node's force is exactly zero;
+* linear momentum: bond forces come in antisymmetric pairs, so
+  sum_i V_i f_i = 0 per sphere up to rounding (no walls, no gravity, spheres
+  apart);
+* a sample: two of the 125 spheres, stepped inside the full pool, equal the
+  oracle stepping those two spheres alone, bit for bit (they are far from
+  the others and from the walls for these few steps).
+"""
+import numpy as np
+import pytest
+
+from helpers import FIELDS, assert_bitwise
+from peridem_amd import abi, gen
+
+pytestmark = pytest.mark.gpu
+
+NSPHERES = 125
+
+
+def full_pool(**kw):
+    s = gen.gravity_settling(nspheres=NSPHERES, **kw)
+    print(f"[fullsize] {len(s.particles)} spheres, {s.nodes} nodes, {s.bonds} directed bonds", flush=True)
+    assert s.nodes == 1003125 and s.bonds == 92664286
+    return s
+
+
+def without_gravity(s):
+    for p in s.particles:
+        p.ext = np.zeros((p.n, 3))
+    return s
+
+
+def test_rigid_motion_leaves_every_bond_unstretched():
+    s = without_gravity(full_pool(sphere_speed=0.5, node_speed=0.0))
+    s.allow_wall = 0
+    e = abi.gpu_engine(s)
+    try:
+        e.step(10)
+        c = e.counters()
+        assert c["alive_bonds"] == s.bonds
+        for p in range(NSPHERES):
+            st = e.state(p)
+            f = st["force"]
+            assert not np.any(f.view(np.uint64)), f"sphere {p}: force not exactly zero"
+            # every node of a sphere moved by the same displacement
+            d = st["disp"]
+            assert np.all(d == d[0]), f"sphere {p}: displacement not rigid"
+    finally:
+        e.close()
+
+
+def test_bond_forces_conserve_linear_momentum():
+    s = without_gravity(full_pool(sphere_speed=0.5, node_speed=0.01))
+    s.allow_wall = 0
+    e = abi.gpu_engine(s)
+    try:
+        e.step(20)
+        worst = 0.0
+        for p in range(NSPHERES):
+            f = e.state(p)["force"]
+            vol = s.particles[p].vol[:, None]
+            vf = vol * f
+            net = np.abs(vf.sum(axis=0)).max()
+            scale = np.abs(vf).sum(axis=0).max()
+            assert scale > 0.0
+            worst = max(worst, net / scale)
+        print(f"[fullsize] worst |sum V f| / sum |V f| = {worst:.3e}", flush=True)
+        assert worst < 1e-12
+    finally:
+        e.close()
+
+
+def test_two_sphere_sample_matches_oracle_bitwise(oracle_lib):
+    s = full_pool(sphere_speed=0.5, node_speed=0.01)
+    sample = gen.System(**{**s.__dict__, "particles": s.particles[:2]})
+    g = abi.gpu_engine(s)
+    o = oracle_lib.engine(sample)
+    try:
+        done = 0
+        for c in (1, 5, 12):
+            g.step(c - done)
+            o.step(c - done)
+            done = c
+            for p in range(2):
+                sg, so = g.state(p), o.state(p)
+                for k in FIELDS:
+                    assert_bitwise(sg[k], so[k], f"step {c} sphere {p} {k}")
+                cg, co = g.connectivity(p), o.connectivity(p)
+                assert np.array_equal(cg, co), f"step {c} sphere {p} connectivity"
+        assert_bitwise(g.wall()[0], o.wall()[0], "wall planes")
+    finally:
+        g.close()
+        o.close()
