@@ -92,9 +92,11 @@ def shape_name(k, nodes):
     """Kernel symbol of bond shape k (pools below 2^27 node slots address
     node records with 32-bit offsets)."""
     from peridem_amd import abi
-    u, pipe, mask = abi.bond_shapes()[k]
+    u, pipe, staged = abi.bond_shapes()[k]
+    if staged:
+        return f"pd::k_bond_lds<{u}, {pipe}>"
     off32 = "true" if nodes < (1 << 27) else "false"
-    return f"pd::k_bond_t<{u}, {pipe}, {'true' if mask else 'false'}, {off32}>"
+    return f"pd::k_bond_t<{u}, {pipe}, {off32}>"
 
 
 def cpu_baseline(s, nspheres, steps, threads):

@@ -227,17 +227,22 @@ int pd_set_graphs(pd_ctx* ctx, int on);
 
 /* Bond-kernel shapes and their autotuner.  No counterpart in the reference.
  * The library carries several instantiations of the bond kernel (entries
- * per batch, software pipelining of the index and neighbour loads, loads
- * switched off for dead entries); each
- * sums every row in row order, so all give the same bits.  By default the
+ * per batch, software pipelining of the index and neighbour loads, and
+ * "staged" shapes that first copy each 256-node block's table of referenced
+ * nodes and their positions and volumes into LDS); each sums every row in
+ * row order, so all give the same bits.
+ * Staged shapes are usable while every resident job's block tables hold at
+ * most 2240 nodes (pools of 2^27 node slots or more use one unstaged shape);
+ * a selected shape that is not usable falls back to shape 4.  By default the
  * first pd_step of a pool with at least 2^18 resident nodes times every
- * shape on the resident state (scratch copies of the force and bond arrays,
- * so nothing changes) and keeps the fastest; smaller pools use shape 4.
- * pd_autotune re-runs that with reps timed launches per shape, returning the
- * choice and (when ms != NULL) the mean time of each shape [count];
- * pd_set_bond_shape forces one (-1: tune at the next step). */
+ * usable shape on the resident state (scratch copies of the force and bond
+ * arrays, so nothing changes) and keeps the fastest; smaller pools use shape
+ * 10 (staged), or 4 while that is not usable.  pd_autotune re-runs that with reps timed launches per shape, returning
+ * the choice and (when ms != NULL) the mean time of each shape [count], -1
+ * for a shape not usable on this pool; pd_set_bond_shape forces one (-1: tune
+ * at the next step); pd_get_bond_shape reports the shape launches use. */
 int pd_bond_shape_count(void);
-int pd_bond_shape_info(int shape, int* unroll, int* pipelined, int* masked);
+int pd_bond_shape_info(int shape, int* unroll, int* pipelined, int* staged);
 int pd_set_bond_shape(pd_ctx* ctx, int shape);
 int pd_get_bond_shape(pd_ctx* ctx, int* shape);
 int pd_autotune(pd_ctx* ctx, uint32_t reps, int* best, double* ms);

@@ -507,14 +507,15 @@ class Pool:
 
 
 def bond_shapes():
-    """[(unroll, pipeline, masked)] of the bond-kernel shapes the library was built with
-    (pipeline: 0 none, 1 software-pipelined, 2 pipelined and unrolled twice)."""
+    """[(unroll, pipeline, staged)] of the bond-kernel shapes the library was
+    built with (pipeline: 0 none, 1 software-pipelined, 2 pipelined and unrolled
+    twice; staged: block node tables and static records in LDS)."""
     lib, _ = load_gpu_lib()
     out = []
     for k in range(lib.pd_bond_shape_count()):
-        u, p, m = C.c_int(), C.c_int(), C.c_int()
-        lib.pd_bond_shape_info(k, C.byref(u), C.byref(p), C.byref(m))
-        out.append((u.value, p.value, bool(m.value)))
+        u, p, st = C.c_int(), C.c_int(), C.c_int()
+        lib.pd_bond_shape_info(k, C.byref(u), C.byref(p), C.byref(st))
+        out.append((u.value, p.value, bool(st.value)))
     return out
 
 

@@ -35,15 +35,18 @@ namespace {
 
 constexpr uint64_t kUnlimited = ~0ull;
 constexpr uint64_t kTuneMinNodes = 1u << 18;  // autotune pools of at least this many nodes
-constexpr int kDefaultBondShape = 4;          // pipelined, 4 entries per batch
+constexpr int kDefaultBondShape = 4;          // unstaged, pipelined, 4 entries per batch (always usable)
+constexpr int kSmallPoolShape = 10;           // staged, pipelined and unrolled twice, 4 entries per batch
 constexpr double kSkinFrac = 0.3;             // candidate-list skin / R_c
 constexpr int kGraphSteps = 8;  // steps per captured graph
 constexpr int kProfRing = 16;   // event sets of profiled steps in flight
 constexpr int kProfLag = 8;     // a profiled step's marks are read this many steps later
 // device bytes per node slot (X, U, V, A, F, E, pid, nflag, deg, wmask, ckey,
 // csorted, ~2 hash buckets) and per bond entry
-constexpr uint64_t kBytesPerNode = 32 + 32 + 4 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16 + 56;
-constexpr uint64_t kBytesPerEntry = 4;
+// node arrays + the bond kernel's record copies + block tables (kLdsCap per
+// 256 nodes); per entry the u32 and u16 forms
+constexpr uint64_t kBytesPerNode = 32 + 32 + 4 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16 + 56 + 36;
+constexpr uint64_t kBytesPerEntry = 6;
 
 struct HostPart {
   uint32_t n = 0;
@@ -68,6 +71,7 @@ struct Job {
   bool reaction_valid = false;
   bool vc_on = false;        // a prescribed-velocity particle is set and switched on
   int64_t vc_part = -1;      // its particle index in the job
+  bool lds_ok = true;        // every block table fits kLdsCap
 };
 
 // first-fit allocator over [0, cap)
@@ -147,6 +151,7 @@ struct pd_ctx {
   bool use_graph = true, graph_dirty = true;
   uint64_t hwm_slices = 0;      // end of the highest node range in use
   int bond_shape = -1;          // index into bond_shapes(); -1: tune at the next step
+  uint32_t lds_bad = 0;         // resident jobs whose block tables exceed kLdsCap
   uint64_t tuned_nodes = 0;     // resident nodes when the shape was tuned
   double tune_ms[kBondShapes] = {};
   hipGraph_t graph = nullptr;
@@ -296,7 +301,8 @@ static void build_rows(const pd_particle_desc* P, uint32_t np, JobRows& R) {
     work += P[i].nconn;
   }
   R.nodes = nodes;
-  R.nslices = (nodes + kSlice - 1) / kSlice;
+  // whole node blocks (kBlockSlices slices)
+  R.nslices = ((nodes + kSlice - 1) / kSlice + kBlockSlices - 1) / kBlockSlices * kBlockSlices;
   const uint64_t Np = R.nslices * kSlice;
   R.deg.assign(Np, 0);
   parallel_for(np, work, [&](uint64_t i) {
@@ -476,7 +482,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   if (c->reserved) return fail(c, PD_E_STATE, "pool already reserved");
   if (!pool->max_nodes || !pool->max_jobs || !pool->max_particles)
     return fail(c, PD_E_ARG, "pool needs nodes, particles and jobs");
-  const uint64_t S = (pool->max_nodes + kSlice - 1) / kSlice, N = S * kSlice;
+  const uint64_t S = ((pool->max_nodes + kSlice - 1) / kSlice + kBlockSlices - 1) / kBlockSlices * kBlockSlices,
+                 N = S * kSlice;
   if (N >= 0x7FFFFFFFull) return fail(c, PD_E_OVERFLOW, "more than 2^31-1 node slots");
   if (pool->max_particles >= 0x7FFFFFFFull || pool->max_jobs >= 0x7FFFFFFFull)
     return fail(c, PD_E_OVERFLOW, "too many particle or job slots");
@@ -503,6 +510,9 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(ZV, N);
   ALLOC(UXY, N);
   ALLOC(UZ, N);
+  ALLOC(bloc, std::max<uint64_t>(c->cap_entries, 1));
+  ALLOC(btab, S / kBlockSlices * kLdsCap);
+  ALLOC(binfo, S / kBlockSlices);
   ALLOC(V, 3 * N);
   ALLOC(A, 3 * N);
   ALLOC(F, 3 * N);
@@ -558,6 +568,9 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipMemsetAsync(d.soff, 0, S * sizeof(uint64_t), st));
   HIPCHK(c, hipMemsetAsync(d.swid, 0, S * sizeof(uint32_t), st));
   if (c->cap_entries) HIPCHK(c, hipMemsetAsync(d.bond, 0xFF, c->cap_entries * sizeof(uint32_t), st));
+  if (c->cap_entries) HIPCHK(c, hipMemsetAsync(d.bloc, 0xFF, c->cap_entries * sizeof(uint16_t), st));
+  HIPCHK(c, hipMemsetAsync(d.btab, 0, S / kBlockSlices * kLdsCap * sizeof(uint32_t), st));
+  HIPCHK(c, hipMemsetAsync(d.binfo, 0, S / kBlockSlices * sizeof(uint2), st));
   HIPCHK(c, hipMemsetAsync(d.lrtb, 0, 6 * P * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.pactive, 0, P * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.counters, 0, 8 * sizeof(unsigned long long), st));
@@ -617,6 +630,10 @@ struct pd_job_blob {
   std::vector<HostPart> parts;    // off job-relative
   std::vector<uint64_t> soff;     // relative to the job's first bond entry
   std::vector<uint32_t> bond;     // job-relative node indices, kPad padding
+  std::vector<uint16_t> bloc;     // the entries as positions in their block's table, kPad16 padding
+  std::vector<uint32_t> btab;     // per block, kLdsCap slots: sorted job-relative node indices
+  std::vector<uint2> binfo;       // per block: table length, position of the block's first node
+  bool lds_ok = true;             // every block table fits kLdsCap
   uint32_t fl = 0;
   uint64_t movable_bonds = 0, total_bonds = 0;
 };
@@ -783,6 +800,38 @@ int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t n
       for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) col[(k - R.rstart[g]) * kSlice] = R.rows[k];
     }
   });
+  // node blocks: the sorted set of nodes each block's rows reference (its
+  // own nodes included) and every entry's position in it
+  const uint64_t nblk = R.nslices / kBlockSlices, bn = (uint64_t)kBlockSlices * kSlice;
+  B->bloc.assign(R.entries, (uint16_t)kPad16);
+  B->btab.assign(nblk * kLdsCap, 0);
+  B->binfo.assign(nblk, make_uint2(0, 0));
+  std::vector<uint8_t> over(nblk, 0);
+  parallel_for(nblk, work, [&](uint64_t b) {
+    const uint64_t n0 = b * bn, n1 = std::min<uint64_t>(R.nodes, n0 + bn);
+    if (n0 >= n1) return;
+    std::vector<uint32_t> t;
+    t.reserve((n1 - n0) + (R.rstart[n1] - R.rstart[n0]));
+    for (uint64_t g = n0; g < n1; g++) {
+      t.push_back((uint32_t)g);
+      for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) t.push_back(R.rows[k]);
+    }
+    std::sort(t.begin(), t.end());
+    t.erase(std::unique(t.begin(), t.end()), t.end());
+    if (t.size() > kLdsCap) {
+      over[b] = 1;
+      return;
+    }
+    auto pos = [&](uint32_t v) { return (uint32_t)(std::lower_bound(t.begin(), t.end(), v) - t.begin()); };
+    for (uint64_t g = n0; g < n1; g++) {
+      uint16_t* col = B->bloc.data() + B->soff[g / kSlice] + (g % kSlice);
+      for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++)
+        col[(k - R.rstart[g]) * kSlice] = (uint16_t)pos(R.rows[k]);
+    }
+    std::copy(t.begin(), t.end(), B->btab.begin() + b * kLdsCap);
+    B->binfo[b] = make_uint2((uint32_t)t.size(), pos((uint32_t)n0));
+  });
+  for (uint64_t b = 0; b < nblk; b++) B->lds_ok &= !over[b];
   B->deg = std::move(R.deg);
   B->width = std::move(R.width);
   *out = B;
@@ -830,6 +879,8 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
   J.nodes = B->nodes;
   J.movable_bonds = B->movable_bonds;
   J.total_bonds = B->total_bonds;
+  J.lds_ok = B->lds_ok;
+  if (!J.lds_ok) c->lds_bad++;
   const uint64_t g0 = J.node0, Np = B->nslices * kSlice;
   // place: particle slots, pool node indices, slice offsets
   std::vector<uint32_t> slot(np);
@@ -887,6 +938,17 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
     hipLaunchKernelGGL(k_rebase, dim3(grid_of(B->entries, 256)), dim3(256), 0, c->stream, d.bond + b0, B->entries,
                        (uint32_t)g0);
     HIPCHK(c, hipGetLastError());
+    UP(d.bloc + b0, B->bloc.data(), B->entries);
+  }
+  {
+    // block tables: job-relative -> pool node indices (the blocks start at
+    // s0 / kBlockSlices, since every job occupies whole blocks)
+    const uint64_t nb = B->nslices / kBlockSlices, t0 = s0 / kBlockSlices * (uint64_t)kLdsCap;
+    UP(d.btab + t0, B->btab.data(), nb * kLdsCap);
+    hipLaunchKernelGGL(k_rebase, dim3(grid_of(nb * kLdsCap, 256)), dim3(256), 0, c->stream, d.btab + t0,
+                       nb * (uint64_t)kLdsCap, (uint32_t)g0);
+    HIPCHK(c, hipGetLastError());
+    UP(d.binfo + s0 / kBlockSlices, B->binfo.data(), nb);
   }
   for (uint32_t i = 0; i < np; i++) {
     UP(d.part + slot[i], &dparts[i], 1);
@@ -938,6 +1000,9 @@ int pd_remove_job(pd_ctx* c, uint32_t job) {
     HIPCHK(c, hipMemsetAsync(d.swid + J.node0 / kSlice, 0, J.nslices * sizeof(uint32_t), st));
   }
   if (J.nentries) HIPCHK(c, hipMemsetAsync(d.bond + J.bond0, 0xFF, J.nentries * sizeof(uint32_t), st));
+  if (J.nentries) HIPCHK(c, hipMemsetAsync(d.bloc + J.bond0, 0xFF, J.nentries * sizeof(uint16_t), st));
+  HIPCHK(c, hipMemsetAsync(d.binfo + J.node0 / kSlice / kBlockSlices, 0, J.nslices / kBlockSlices * sizeof(uint2), st));
+  if (!J.lds_ok) c->lds_bad--;
   HIPCHK(c, hipStreamSynchronize(st));
   DevInst vi;
   void_inst(vi);
@@ -1021,6 +1086,17 @@ static Dev step_view(const pd_ctx* c) {
 // Node records of pools this large are addressed with 64-bit offsets.
 static bool wide_pool(const pd_ctx* c) { return c->cap_slices * kSlice > kOff32MaxNodes; }
 
+// The staged (LDS) shapes need every resident job's block tables to fit.
+static bool shape_usable(const pd_ctx* c, int v) {
+  return !bond_shapes()[v].lds || (c->lds_bad == 0 && !wide_pool(c));
+}
+
+// The shape a launch uses: the selected one while usable, else the default.
+static int effective_shape(const pd_ctx* c) {
+  const int v = c->bond_shape < 0 ? kDefaultBondShape : c->bond_shape;
+  return shape_usable(c, v) ? v : kDefaultBondShape;
+}
+
 // Phase times of one profiled step (waits for its last mark).
 static int accumulate_marks(pd_ctx* c, hipEvent_t* ev) {
   HIPCHK(c, hipEventSynchronize(ev[PD_NPHASE]));
@@ -1063,7 +1139,7 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
   }
   if ((rc = mark(2))) return rc;
   // grid padded to a multiple of the XCD count (the bond kernel remaps blocks)
-  hipLaunchKernelGGL(wide_pool(c) ? bond_wide().fn : bond_shapes()[c->bond_shape < 0 ? 0 : c->bond_shape].fn,
+  hipLaunchKernelGGL(wide_pool(c) ? bond_wide().fn : bond_shapes()[effective_shape(c)].fn,
                      dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
   if ((rc = mark(3))) return rc;
   hipLaunchKernelGGL(k_contact, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
@@ -1094,10 +1170,12 @@ static int autotune(pd_ctx* c, uint32_t reps) {
   int rc = PD_OK;
   double* F = nullptr;
   uint32_t* B = nullptr;
+  uint16_t* B16 = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if ((rc = dalloc(c, &F, 3 * Np)) || (rc = dalloc(c, &B, std::max<uint64_t>(c->cap_entries, 1)))) return rc;
+  const uint64_t ne = std::max<uint64_t>(c->cap_entries, 1);
+  if ((rc = dalloc(c, &F, 3 * Np)) || (rc = dalloc(c, &B, ne)) || (rc = dalloc(c, &B16, ne))) return rc;
   auto release = [&]() {
-    for (void* q : {(void*)F, (void*)B}) {
+    for (void* q : {(void*)F, (void*)B, (void*)B16}) {
       auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
       if (it != c->allocs.end()) c->allocs.erase(it);
       (void)hipFree(q);
@@ -1107,15 +1185,22 @@ static int autotune(pd_ctx* c, uint32_t reps) {
   };
   auto run = [&]() -> int {
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->cap_entries)
+    if (c->cap_entries) {
       HIPCHK(c, hipMemcpyAsync(B, c->d.bond, c->cap_entries * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(B16, c->d.bloc, c->cap_entries * sizeof(uint16_t), hipMemcpyDeviceToDevice, c->stream));
+    }
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     t.F = F;
     t.bond = B;
+    t.bloc = B16;
     const dim3 grid((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds);
-    int best = 0;
+    int best = kDefaultBondShape;
     for (int v = 0; v < kBondShapes; v++) {
+      if (!shape_usable(c, v)) {
+        c->tune_ms[v] = -1.0;   // not timed
+        continue;
+      }
       hipLaunchKernelGGL(bond_shapes()[v].fn, grid, dim3(256), 0, c->stream, t);   // warm-up
       HIPCHK(c, hipEventRecord(e0, c->stream));
       for (uint32_t r = 0; r < reps; r++) hipLaunchKernelGGL(bond_shapes()[v].fn, grid, dim3(256), 0, c->stream, t);
@@ -1190,7 +1275,7 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
     if (c->cnt.nodes >= kTuneMinNodes) {
       if ((rc = autotune(c, 3))) return rc;
     } else {
-      c->bond_shape = kDefaultBondShape;
+      c->bond_shape = kSmallPoolShape;   // kDefaultBondShape while it is not usable
     }
   }
   // steps_left drives reaction_step(): the reaction is formed on the last
@@ -1242,11 +1327,11 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
 
 int pd_bond_shape_count(void) { return kBondShapes; }
 
-int pd_bond_shape_info(int shape, int* unroll, int* pipelined, int* masked) {
+int pd_bond_shape_info(int shape, int* unroll, int* pipelined, int* staged) {
   if (shape < 0 || shape >= kBondShapes) return PD_E_ARG;
   if (unroll) *unroll = bond_shapes()[shape].unroll;
   if (pipelined) *pipelined = bond_shapes()[shape].pipe;
-  if (masked) *masked = bond_shapes()[shape].mask;
+  if (staged) *staged = bond_shapes()[shape].lds;
   return PD_OK;
 }
 
@@ -1271,7 +1356,7 @@ int pd_autotune(pd_ctx* c, uint32_t reps, int* best, double* ms) {
 
 int pd_get_bond_shape(pd_ctx* c, int* shape) {
   if (!c || !shape) return PD_E_ARG;
-  *shape = c->bond_shape;
+  *shape = c->bond_shape < 0 ? -1 : effective_shape(c);
   return PD_OK;
 }
 

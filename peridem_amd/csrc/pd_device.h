@@ -12,6 +12,14 @@ constexpr uint32_t kBroken = 0x80000000u;   // high bit of a bond entry: bond er
 constexpr uint32_t kPad = 0xFFFFFFFFu;      // slice padding entry
 constexpr uint32_t kIdxMask = 0x7FFFFFFFu;
 constexpr int kSlice = 64;                  // nodes per bond slice (= wavefront width)
+// Node blocks of the staged bond kernel: 4 slices = one 256-thread workgroup.
+// Jobs occupy whole blocks.  Each block keeps the sorted table of the nodes
+// its rows reference (its own nodes included), at most kLdsCap of them, and
+// each bond entry also exists as a u16 position in that table.
+constexpr int kBlockSlices = 4;
+constexpr uint32_t kLdsCap = 2240;          // 2240 * (16 + 16 + 4) B = 80 KB of LDS per workgroup
+constexpr uint32_t kDead16 = 0x8000u;       // high bit of a u16 entry: bond erased
+constexpr uint32_t kPad16 = 0xFFFFu;        // u16 slice padding entry
 constexpr int kMaxCand = 48;                // contact candidates kept per node (list capacity)
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 
@@ -79,6 +87,9 @@ struct Dev {
   double2* ZV;       // z, vol
   double2* UXY;      // disp x, y   (rewritten by the position kernel each step)
   double* UZ;        // disp z
+  uint16_t* bloc;    // [entries] the bond entries as positions in their block's node table
+  uint32_t* btab;    // [blocks * kLdsCap] per block: sorted pool indices of the nodes its rows reference
+  uint2* binfo;      // [blocks] (table length, position of the block's first node in it)
   double* V;         // vel [3N]
   double* A;         // acc [3N]
   double* F;         // force density [3N]

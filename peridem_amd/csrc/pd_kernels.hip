@@ -436,28 +436,31 @@ constexpr uint32_t kOneHi = 0x3FF00000u;                // high word of 1.0 (low
 
 // The slice's bond entries as one uniform pointer per batch entry plus the
 // lane's byte offset, so index loads and break stores take no vector address
-// arithmetic.
+// arithmetic.  T: uint32_t (pool node indices) or uint16_t (positions in the
+// block's node table).
+template <class T>
 struct SliceRows {
   const char* base;   // entry 0 of the slice (wave-uniform)
-  uint32_t lane4;     // 4 * lane
-  __device__ __forceinline__ uint32_t* at(uint32_t k) const {
-    return (uint32_t*)(base + (uint64_t)k * (kSlice * sizeof(uint32_t)) + lane4);
+  uint32_t lane_off;  // sizeof(T) * lane
+  __device__ __forceinline__ T* at(uint32_t k) const {
+    return (T*)(base + (uint64_t)k * (kSlice * sizeof(T)) + lane_off);
   }
 };
 
-// The k-th entries of the U rows-batches starting at k (kPad past the width,
+// The k-th entries of the U rows-batches starting at k (pad past the width,
 // W >= 1).  Every load is issued -- past the width it rereads entry W-1 and
 // the value is replaced -- so the loop body has no load under a branch and
 // the compiler can count outstanding loads exactly instead of draining them.
 // The entries are read once per launch: non-temporal loads, which leave the
 // caches to the neighbour records.
-template <int U>
-__device__ __forceinline__ void bond_idx(const SliceRows& R, uint32_t k, uint32_t W, uint32_t (&e)[U]) {
+template <int U, class T>
+__device__ __forceinline__ void bond_idx(const SliceRows<T>& R, uint32_t k, uint32_t W, uint32_t pad,
+                                         uint32_t (&e)[U]) {
 #pragma unroll
   for (int u = 0; u < U; u++) {
     const bool in = k + u < W;
     const uint32_t v = __builtin_nontemporal_load(R.at(in ? k + u : W - 1));
-    e[u] = in ? v : kPad;
+    e[u] = in ? v : pad;
   }
 }
 
@@ -466,45 +469,91 @@ __device__ __forceinline__ void bond_idx(const SliceRows& R, uint32_t k, uint32_
 // byte offset is a 32-bit shift and the loads take the array bases from
 // scalar registers.
 template <bool OFF32>
-__device__ __forceinline__ void rec(const Dev& d, uint32_t j, double4& x, double4& u) {
-  double2 xy, zv, uxy;
+__device__ __forceinline__ void rec_u(const Dev& d, uint32_t j, double4& u) {
+  double2 uxy;
   double uz;
   if (OFF32) {
-    const uint32_t o8 = j << 3, o16 = j << 4;
-    xy = *(const double2*)((const char*)d.XY + o16);
-    zv = *(const double2*)((const char*)d.ZV + o16);
-    uxy = *(const double2*)((const char*)d.UXY + o16);
-    uz = *(const double*)((const char*)d.UZ + o8);
+    uxy = *(const double2*)((const char*)d.UXY + (j << 4));
+    uz = *(const double*)((const char*)d.UZ + (j << 3));
   } else {
-    xy = d.XY[j];
-    zv = d.ZV[j];
     uxy = d.UXY[j];
     uz = d.UZ[j];
   }
-  x = make_double4(xy.x, xy.y, zv.x, zv.y);
   u = make_double4(uxy.x, uxy.y, uz, 0.0);
 }
+template <bool OFF32>
+__device__ __forceinline__ void rec(const Dev& d, uint32_t j, double4& x, double4& u) {
+  double2 xy, zv;
+  if (OFF32) {
+    xy = *(const double2*)((const char*)d.XY + (j << 4));
+    zv = *(const double2*)((const char*)d.ZV + (j << 4));
+  } else {
+    xy = d.XY[j];
+    zv = d.ZV[j];
+  }
+  x = make_double4(xy.x, xy.y, zv.x, zv.y);
+  rec_u<OFF32>(d, j, u);
+}
 
-// Neighbour records of a batch.  A dead entry reads the node itself
-// (MASK = false, branch-free) or issues no load at all (MASK = true: its
-// lanes are switched off for the loads and take the node's own record).
-template <int U, bool MASK, bool OFF32>
-__device__ __forceinline__ void bond_gather(const Dev& d, uint32_t g, const uint32_t (&e)[U], const double4& xi,
-                                            const double4& ui, double4 (&xj)[U], double4 (&uj)[U]) {
+// Where a bond kernel reads its entries and neighbour records from.
+//
+// GlobalRows: the u32 entries (pool node index, bit 31 = broken); records
+// gathered from the split copies in HBM/L2.  A dead entry reads the node
+// itself (branch-free; switching its lanes off for the loads measured slower).
+template <bool OFF32>
+struct GlobalRows {
+  const Dev& d;
+  SliceRows<uint32_t> R;
+  SliceRows<uint16_t> R16;
+  uint32_t g;
+  static constexpr uint32_t kPadE = kPad;
+  __device__ __forceinline__ const SliceRows<uint32_t>& rows() const { return R; }
+  __device__ __forceinline__ bool alive(uint32_t e) const { return (int32_t)e >= 0; }
+  template <int U>
+  __device__ __forceinline__ void gather(const uint32_t (&e)[U], const double4& xi, const double4& ui,
+                                         double4 (&xj)[U], double4 (&uj)[U]) const {
 #pragma unroll
-  for (int u = 0; u < U; u++) {
-    if (MASK) {
-      if ((int32_t)e[u] >= 0) {
-        rec<OFF32>(d, e[u], xj[u], uj[u]);
-      } else {
-        xj[u] = xi;
-        uj[u] = ui;
-      }
-    } else {
-      rec<OFF32>(d, ((int32_t)e[u] < 0) ? g : e[u], xj[u], uj[u]);
+    for (int u = 0; u < U; u++) rec<OFF32>(d, alive(e[u]) ? e[u] : g, xj[u], uj[u]);
+  }
+  // both forms of the entry carry the broken flag
+  __device__ __forceinline__ void broke(uint32_t k, uint32_t e) const {
+    *R.at(k) = e | kBroken;
+    uint16_t* q = R16.at(k);
+    *q = (uint16_t)(*q | kDead16);
+  }
+};
+
+// LdsRows: the u16 entries (position in the 256-node block's table of the
+// nodes its rows reference, bit 15 = broken); the block's table and the
+// static part of those nodes' records (x, y, z, vol) staged in LDS once per
+// launch, the displacement gathered from the split copies through the
+// table.  A dead entry uses the node's own position in the table.
+__shared__ double2 s_xy[kLdsCap], s_zv[kLdsCap];
+__shared__ uint32_t s_id[kLdsCap];
+struct LdsRows {
+  const Dev& d;
+  SliceRows<uint16_t> R16;
+  SliceRows<uint32_t> R;
+  uint32_t self;   // the node's position in the table
+  static constexpr uint32_t kPadE = kPad16;
+  __device__ __forceinline__ const SliceRows<uint16_t>& rows() const { return R16; }
+  __device__ __forceinline__ bool alive(uint32_t e) const { return (e & kDead16) == 0; }
+  template <int U>
+  __device__ __forceinline__ void gather(const uint32_t (&e)[U], const double4&, const double4&, double4 (&xj)[U],
+                                         double4 (&uj)[U]) const {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t t = alive(e[u]) ? e[u] : self;
+      const double2 xy = s_xy[t], zv = s_zv[t];
+      xj[u] = make_double4(xy.x, xy.y, zv.x, zv.y);
+      rec_u<true>(d, s_id[t], uj[u]);
     }
   }
-}
+  __device__ __forceinline__ void broke(uint32_t k, uint32_t e) const {
+    *R.at(k) = s_id[e] | kBroken;
+    *R16.at(k) = (uint16_t)(e | kDead16);
+  }
+};
 
 // Terms of one batch, added to (f0, f1, f2) in row order; breaks recorded.
 //
@@ -513,17 +562,17 @@ __device__ __forceinline__ void bond_gather(const Dev& d, uint32_t g, const uint
 // |xi|^2 = +0; giving both the high word of 1.0 makes them exactly 1.0, the
 // stretch exactly +0 and the coefficient +-0 without a select on it.  The
 // IEEE branch (safe mode, or a lane outside the domain) selects explicitly.
-template <int U>
-__device__ __forceinline__ void bond_terms(const SliceRows& R, uint32_t k, const uint32_t (&e)[U],
-                                           const double4 (&xj)[U], const double4 (&uj)[U], const double4& xi,
-                                           const double4& ui, bool fast, bool can_break, double cnot, double snot,
-                                           double& f0, double& f1, double& f2) {
+template <int U, class Src>
+__device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint32_t (&e)[U], const double4 (&xj)[U],
+                                           const double4 (&uj)[U], const double4& xi, const double4& ui, bool fast,
+                                           bool can_break, double cnot, double snot, double& f0, double& f1,
+                                           double& f2) {
   bool alive[U];
   double ev[U][3], xv[U][3], n2[U], l2[U];
   uint32_t over = 0;
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    alive[u] = (int32_t)e[u] >= 0;
+    alive[u] = S.alive(e[u]);
     xv[u][0] = xj[u].x - xi.x;   // xi (gen_xi)
     xv[u][1] = xj[u].y - xi.y;
     xv[u][2] = xj[u].z - xi.z;
@@ -566,19 +615,97 @@ __device__ __forceinline__ void bond_terms(const SliceRows& R, uint32_t k, const
     f0 += t[u][0];
     f1 += t[u][1];
     f2 += t[u][2];
-    if (alive[u] & can_break & (str[u] > snot)) *R.at(k + u) = e[u] | kBroken;
+    if (alive[u] & can_break & (str[u] > snot)) S.broke(k + u, e[u]);
   }
 }
 
-// U: entries per batch.  MASK: see bond_gather.  PIPE: software pipelining -- the indices of batch
+// The rows of a wavefront's slice, W entries each, summed in row order.
+// U: entries per batch.  PIPE: software pipelining -- the indices of batch
 // k+2 and the neighbour records of batch k+1 are in flight while batch k is
-// evaluated, so the index stream (HBM) and the gathers (L2) overlap the FP64
-// arithmetic instead of stalling every batch.  OFF32: see rec.
-template <int U, int PIPE, bool MASK, bool OFF32>
-__global__ __launch_bounds__(256) void k_bond_t(Dev d) {
+// evaluated, so the index stream (HBM) and the gathers (L2 or LDS) overlap the
+// FP64 arithmetic instead of stalling every batch; 2: the same, unrolled
+// twice so the records alternate between two register sets.
+template <int U, int PIPE, class Src>
+__device__ __forceinline__ void bond_rows(const Src& S, uint32_t W, const double4& xi, const double4& ui, bool fast,
+                                          bool can_break, double cnot, double snot, double& f0, double& f1,
+                                          double& f2) {
+  const auto& R = S.rows();
+  const uint32_t pad = Src::kPadE;
+  if (PIPE == 0) {
+    for (uint32_t k = 0; k < W; k += U) {
+      uint32_t e[U];
+      double4 xj[U], uj[U];
+      bond_idx<U>(R, k, W, pad, e);
+      S.template gather<U>(e, xi, ui, xj, uj);
+      bond_terms<U>(S, k, e, xj, uj, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+    }
+  } else if (PIPE == 2 && W) {
+    uint32_t ea[U], eb[U];
+    double4 xa[U], ua[U], xb[U], ub[U];
+    bond_idx<U>(R, 0, W, pad, ea);
+    bond_idx<U>(R, U, W, pad, eb);
+    S.template gather<U>(ea, xi, ui, xa, ua);
+    for (uint32_t k = 0; k < W; k += 2 * U) {
+      uint32_t ec[U];
+      bond_idx<U>(R, k + 2 * U, W, pad, ec);
+      S.template gather<U>(eb, xi, ui, xb, ub);
+      bond_terms<U>(S, k, ea, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+      if (k + U >= W) break;
+      bond_idx<U>(R, k + 3 * U, W, pad, ea);
+      S.template gather<U>(ec, xi, ui, xa, ua);
+      bond_terms<U>(S, k + U, eb, xb, ub, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        eb[u] = ea[u];   // batch k + 3U
+        ea[u] = ec[u];   // batch k + 2U, its records already in xa/ua
+      }
+    }
+  } else if (W) {
+    uint32_t e0[U], e1[U];
+    double4 xa[U], ua[U];
+    bond_idx<U>(R, 0, W, pad, e0);
+    bond_idx<U>(R, U, W, pad, e1);
+    S.template gather<U>(e0, xi, ui, xa, ua);
+    for (uint32_t k = 0; k < W; k += U) {
+      uint32_t e2[U];
+      double4 xb[U], ub[U];
+      bond_idx<U>(R, k + 2 * U, W, pad, e2);
+      S.template gather<U>(e1, xi, ui, xb, ub);
+      bond_terms<U>(S, k, e0, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        e0[u] = e1[u];
+        e1[u] = e2[u];
+        xa[u] = xb[u];
+        ua[u] = ub[u];
+      }
+    }
+  }
+}
+
+// Workgroup -> node block: blocks are dealt round-robin over the XCDs, so
+// each XCD runs one contiguous range of nodes and neighbouring tiles, which
+// gather the same records, share that XCD's L2.
+__device__ __forceinline__ uint64_t xcd_block() {
   const uint64_t nbp = gridDim.x;   // a multiple of kXcds
-  const uint64_t blk = (blockIdx.x % kXcds) * (nbp / kXcds) + blockIdx.x / kXcds;
-  const uint64_t g = blk * blockDim.x + threadIdx.x;
+  return (blockIdx.x % kXcds) * (nbp / kXcds) + blockIdx.x / kXcds;
+}
+
+// One slice's offset and width, wave-uniform.
+__device__ __forceinline__ void slice_of(const Dev& d, uint64_t g, uint64_t& base, uint32_t& W) {
+  const uint64_t s = g >> 6;
+  const uint64_t b0 = d.soff[s];
+  const uint32_t blo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b0);
+  const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b0 >> 32));
+  base = ((uint64_t)bhi << 32) | (uint64_t)blo;
+  W = (uint32_t)__builtin_amdgcn_readfirstlane((int)d.swid[s]);
+}
+
+// Bond kernel over the u32 entries, records gathered from HBM/L2.
+// OFF32: see rec.
+template <int U, int PIPE, bool OFF32>
+__global__ __launch_bounds__(256) void k_bond_t(Dev d) {
+  const uint64_t g = xcd_block() * blockDim.x + threadIdx.x;
   // padded node range: S slices of 64; whole wavefronts exit together
   if (g >= d.S * kSlice) return;
   const DevPart& P = d.part[d.pid[g]];
@@ -586,74 +713,65 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
   if (!live(I)) return;            // frozen job or free slot: F keeps its value
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   if (P.movable) {
-    const bool can_break = I.enable_fracture && P.breakable;
-    const double cnot = P.cnot, snot = P.snot;
-    const bool fast = (*d.flags & kFlagSafeMath) == 0;
-    // one wavefront = one slice: offset and width are wave-uniform
-    const uint64_t s = g >> 6;
-    const uint64_t b0 = d.soff[s];
-    const uint32_t blo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b0);
-    const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b0 >> 32));
-    const uint64_t base = ((uint64_t)bhi << 32) | (uint64_t)blo;
-    const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)d.swid[s]);
-    const SliceRows R{(const char*)(d.bond + base), (uint32_t)(g & 63) * 4u};
-    const uint32_t gi = (uint32_t)g;
+    uint64_t base;
+    uint32_t W;
+    slice_of(d, g, base, W);
+    const uint32_t lane = (uint32_t)(g & 63);
+    const GlobalRows<OFF32> S{d,
+                              {(const char*)(d.bond + base), lane * 4u},
+                              {(const char*)(d.bloc + base), lane * 2u},
+                              (uint32_t)g};
     // the node's own record from the same copies its neighbours read, so
     // each node's data is fetched from HBM once per launch
     double4 xi, ui;
-    rec<false>(d, gi, xi, ui);
-    if (PIPE == 0) {
-      for (uint32_t k = 0; k < W; k += U) {
-        uint32_t e[U];
-        double4 xj[U], uj[U];
-        bond_idx<U>(R, k, W, e);
-        bond_gather<U, MASK, OFF32>(d, gi, e, xi, ui, xj, uj);
-        bond_terms<U>(R, k, e, xj, uj, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
-      }
-    } else if (PIPE == 2 && W) {
-      // the same pipeline unrolled twice: the neighbour records alternate
-      // between two register sets instead of being copied every batch
-      uint32_t ea[U], eb[U];
-      double4 xa[U], ua[U], xb[U], ub[U];
-      bond_idx<U>(R, 0, W, ea);
-      bond_idx<U>(R, U, W, eb);
-      bond_gather<U, MASK, OFF32>(d, gi, ea, xi, ui, xa, ua);
-      for (uint32_t k = 0; k < W; k += 2 * U) {
-        uint32_t ec[U];
-        bond_idx<U>(R, k + 2 * U, W, ec);
-        bond_gather<U, MASK, OFF32>(d, gi, eb, xi, ui, xb, ub);
-        bond_terms<U>(R, k, ea, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
-        if (k + U >= W) break;
-        bond_idx<U>(R, k + 3 * U, W, ea);
-        bond_gather<U, MASK, OFF32>(d, gi, ec, xi, ui, xa, ua);
-        bond_terms<U>(R, k + U, eb, xb, ub, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          eb[u] = ea[u];   // batch k + 3U
-          ea[u] = ec[u];   // batch k + 2U, its records already in xa/ua
-        }
-      }
-    } else if (W) {
-      uint32_t e0[U], e1[U];
-      double4 xa[U], ua[U];
-      bond_idx<U>(R, 0, W, e0);
-      bond_idx<U>(R, U, W, e1);
-      bond_gather<U, MASK, OFF32>(d, gi, e0, xi, ui, xa, ua);
-      for (uint32_t k = 0; k < W; k += U) {
-        uint32_t e2[U];
-        double4 xb[U], ub[U];
-        bond_idx<U>(R, k + 2 * U, W, e2);
-        bond_gather<U, MASK, OFF32>(d, gi, e1, xi, ui, xb, ub);
-        bond_terms<U>(R, k, e0, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          e0[u] = e1[u];
-          e1[u] = e2[u];
-          xa[u] = xb[u];
-          ua[u] = ub[u];
-        }
-      }
+    rec<false>(d, (uint32_t)g, xi, ui);
+    bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
+                       P.snot, f0, f1, f2);
+  }
+  d.F[3 * g] = f0;
+  d.F[3 * g + 1] = f1;
+  d.F[3 * g + 2] = f2;
+}
+
+// Bond kernel over the u16 entries: the workgroup's 256-node block first
+// stages its node table and those nodes' static records in LDS (80 KB: two
+// workgroups per CU), then every gather of x, y, z, vol is an LDS read and
+// only the displacement is gathered from L2.  Used when every job of the
+// pool has block tables of at most kLdsCap nodes.
+template <int U, int PIPE>
+__global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
+  const uint64_t blk = xcd_block();
+  const uint64_t g = blk * blockDim.x + threadIdx.x;
+  {
+    uint32_t cnt = 0;
+    if (blk < d.S / kBlockSlices) cnt = min(d.binfo[blk].x, kLdsCap);
+    const uint32_t* tab = d.btab + blk * (uint64_t)kLdsCap;
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const uint32_t id = tab[i];
+      s_id[i] = id;
+      s_xy[i] = d.XY[id];
+      s_zv[i] = d.ZV[id];
     }
+    __syncthreads();
+  }
+  if (g >= d.S * kSlice) return;
+  const DevPart& P = d.part[d.pid[g]];
+  const DevInst& I = d.inst[P.inst];
+  if (!live(I)) return;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  if (P.movable) {
+    uint64_t base;
+    uint32_t W;
+    slice_of(d, g, base, W);
+    const uint32_t lane = (uint32_t)(g & 63);
+    const uint32_t self = d.binfo[blk].y + (uint32_t)(g - blk * blockDim.x);
+    const LdsRows S{d, {(const char*)(d.bloc + base), lane * 2u}, {(const char*)(d.bond + base), lane * 4u}, self};
+    const double2 xy = s_xy[self], zv = s_zv[self];
+    const double4 xi = make_double4(xy.x, xy.y, zv.x, zv.y);
+    double4 ui;
+    rec_u<true>(d, (uint32_t)g, ui);
+    bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
+                       P.snot, f0, f1, f2);
   }
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
@@ -663,26 +781,26 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
 // Instantiated shapes of the bond kernel; the autotuner times them on the
 // resident pool and keeps the fastest.  Every shape sums each row in row order,
 // so all give the same bits.  They address node records with 32-bit offsets;
-// pools of 2^27 node slots or more use kBondWide.
+// pools of 2^27 node slots or more use bond_wide.  lds: the staged kernel,
+// usable while every job's block tables fit kLdsCap.
 struct BondShape {
   void (*fn)(Dev);
-  int unroll, pipe, mask;   // pipe: 0 none, 1 pipelined, 2 pipelined and unrolled twice
+  int unroll, pipe, lds;   // pipe: 0 none, 1 pipelined, 2 pipelined and unrolled twice
 };
-constexpr int kBondShapes = 12;
+constexpr int kBondShapes = 13;
 constexpr uint64_t kOff32MaxNodes = 1ull << 27;   // 16-byte records far below 4 GiB
 __host__ inline const BondShape* bond_shapes() {
   static const BondShape t[kBondShapes] = {
-      {k_bond_t<8, 0, false, true>, 8, 0, 0}, {k_bond_t<4, 0, false, true>, 4, 0, 0},
-      {k_bond_t<6, 0, false, true>, 6, 0, 0}, {k_bond_t<3, 1, false, true>, 3, 1, 0},
-      {k_bond_t<4, 1, false, true>, 4, 1, 0}, {k_bond_t<5, 1, false, true>, 5, 1, 0},
-      {k_bond_t<8, 0, true, true>, 8, 0, 1},  {k_bond_t<3, 1, true, true>, 3, 1, 1},
-      {k_bond_t<4, 1, true, true>, 4, 1, 1},  {k_bond_t<3, 2, false, true>, 3, 2, 0},
-      {k_bond_t<4, 2, false, true>, 4, 2, 0}, {k_bond_t<2, 2, false, true>, 2, 2, 0}};
+      {k_bond_t<8, 0, true>, 8, 0, 0}, {k_bond_t<4, 0, true>, 4, 0, 0}, {k_bond_t<6, 0, true>, 6, 0, 0},
+      {k_bond_t<3, 1, true>, 3, 1, 0}, {k_bond_t<4, 1, true>, 4, 1, 0}, {k_bond_t<5, 1, true>, 5, 1, 0},
+      {k_bond_t<3, 2, true>, 3, 2, 0}, {k_bond_t<4, 2, true>, 4, 2, 0}, {k_bond_t<2, 2, true>, 2, 2, 0},
+      {k_bond_lds<3, 2>, 3, 2, 1},     {k_bond_lds<4, 2>, 4, 2, 1},     {k_bond_lds<2, 2>, 2, 2, 1},
+      {k_bond_lds<4, 1>, 4, 1, 1}};
   return t;
 }
 // The default shape with 64-bit record addressing, for pools too large for
 // 32-bit offsets (never tuned).
-__host__ inline BondShape bond_wide() { return {k_bond_t<4, 2, false, false>, 4, 2, 0}; }
+__host__ inline BondShape bond_wide() { return {k_bond_t<4, 2, false>, 4, 2, 0}; }
 
 // ---------------------------------------------------------------------------
 // Contact, wall and external force, added to the PD sum in the reference's

@@ -137,6 +137,65 @@ def test_every_bond_shape_matches_oracle(oracle_lib):
             g.close()
 
 
+def test_switching_between_staged_and_global_shapes_keeps_breaks(oracle_lib):
+    """Both forms of a bond entry (pool index and position in the block's
+    node table) carry the broken flag: alternating staged and unstaged shapes
+    through a run that breaks bonds still gives the oracle's bits."""
+    s = small_impact(seed=21)
+    shapes = abi.bond_shapes()
+    staged = [k for k, sh in enumerate(shapes) if sh[2]]
+    plain = [k for k, sh in enumerate(shapes) if not sh[2]]
+    assert staged and plain
+    order = [staged[0], plain[0], staged[-1], plain[3], staged[1], plain[-1]]
+    print(f"[parity] shape order {order}", flush=True)
+    g = abi.gpu_engine(s)
+    o = oracle_lib.engine(s)
+    try:
+        done = 0
+        for k in order:
+            g.set_bond_shape(k)
+            g.step(12)
+            o.step(12)
+            done += 12
+            assert g.bond_shape() == k
+            compare_engines(g, o, s, f"shape {k} to step {done}")
+        assert g.counters()["alive_bonds"] < g.counters()["total_bonds"], "no bond broke"
+    finally:
+        g.close()
+        o.close()
+
+
+def test_staged_shapes_fall_back_when_block_tables_overflow(oracle_lib):
+    """A horizon of 5h on a 17^3 cube makes 256-node blocks reference more
+    nodes than the staged kernel's LDS table holds: a staged shape then falls
+    back to the default unstaged one, the tuner skips staged shapes, and the
+    result is still the oracle's."""
+    rng = np.random.default_rng(5)
+    p = gen.make_particle(gen.lattice_cube(8), 1e-4, np.zeros(3), rng, horizon=5.0)
+    p.vel = rng.uniform(-2.0, 2.0, size=(p.n, 3))
+    s = gen.System(particles=[p], dt=1e-8)
+    print(f"[parity] overflowing block tables: nodes={s.nodes} bonds={s.bonds}", flush=True)
+    shapes = abi.bond_shapes()
+    staged = [k for k, sh in enumerate(shapes) if sh[2]]
+    g = abi.gpu_engine(s)
+    o = oracle_lib.engine(s)
+    try:
+        g.set_bond_shape(staged[0])
+        g.step(6)
+        o.step(6)
+        assert not shapes[g.bond_shape()][2]
+        compare_engines(g, o, s, "overflowing block tables")
+        best, ms = g.autotune(1)
+        assert not shapes[best][2]
+        assert all(ms[k] == -1 for k in staged) and all(ms[k] > 0 for k in range(len(shapes)) if k not in staged)
+        g.step(4)
+        o.step(4)
+        compare_engines(g, o, s, "after autotune")
+    finally:
+        g.close()
+        o.close()
+
+
 def test_autotune_leaves_state_untouched(oracle_lib):
     s = small_impact(seed=12)
     g = abi.gpu_engine(s)

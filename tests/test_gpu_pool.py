@@ -49,7 +49,8 @@ def pool_for(systems, jobs_cap=None, slack=1.0):
 def test_footprint_counts_slices():
     s = small_jobs(1, seed=3)[0]
     ns, ne, nb = abi.footprint(s)
-    assert ns % 64 == 0 and ns >= s.nodes and ns < s.nodes + 64
+    # jobs occupy whole 256-node blocks (4 slices of 64)
+    assert ns % 256 == 0 and ns >= s.nodes and ns < s.nodes + 256
     assert ne % 64 == 0 and ne >= s.bonds
     assert nb > 0
 
