@@ -45,7 +45,7 @@ constexpr int kProfLag = 8;     // a profiled step's marks are read this many st
 // csorted, ~2 hash buckets) and per bond entry
 // node arrays + the bond kernel's record copies + block tables (kLdsCap per
 // 256 nodes); per entry the u32 and u16 forms
-constexpr uint64_t kBytesPerNode = 32 + 32 + 4 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16 + 56 + 36;
+constexpr uint64_t kBytesPerNode = 32 + 32 + 4 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16 + 56 + 36 + 2;
 constexpr uint64_t kBytesPerEntry = 6;
 
 struct HostPart {
@@ -122,6 +122,12 @@ struct JobRows {
   std::vector<uint32_t> rows;
   std::vector<uint32_t> width;   // per slice
   std::vector<uint64_t> poff;    // per particle: first node (job-relative)
+  // lanes of a 256-node block hold its nodes by decreasing row length (in
+  // classes of 8), so a slice's width (its longest row) wastes few entries on
+  // shorter rows
+  std::vector<uint8_t> lane_node;  // per slot: offset in the block of the node it holds
+  std::vector<uint8_t> node_lane;  // per node: offset in the block of its slot
+  uint64_t node_of(uint64_t t) const { return (t & ~(uint64_t)255) + lane_node[t]; }
 };
 
 }  // namespace
@@ -321,10 +327,24 @@ static void build_rows(const pd_particle_desc* P, uint32_t np, JobRows& R) {
       R.rows[fp[b]++] = (uint32_t)(o + a);
     }
   });
+  const uint64_t bn = (uint64_t)kBlockSlices * kSlice;
+  R.lane_node.assign(Np, 0);
+  R.node_lane.assign(Np, 0);
+  for (uint64_t b0 = 0; b0 < Np; b0 += bn) {
+    uint32_t ord[(uint64_t)kBlockSlices * kSlice];
+    for (uint32_t i = 0; i < bn; i++) ord[i] = i;
+    // row lengths in classes of 8: lanes with rows of similar length, still
+    // mostly in node order (neighbouring lanes gather neighbouring records)
+    std::stable_sort(ord, ord + bn, [&](uint32_t a, uint32_t b) { return R.deg[b0 + a] / 8 > R.deg[b0 + b] / 8; });
+    for (uint32_t t = 0; t < bn; t++) {
+      R.lane_node[b0 + t] = (uint8_t)ord[t];
+      R.node_lane[b0 + ord[t]] = (uint8_t)t;
+    }
+  }
   R.width.assign(R.nslices, 0);
   for (uint64_t s = 0; s < R.nslices; s++) {
     uint32_t W = 0;
-    for (int l = 0; l < kSlice; l++) W = std::max(W, R.deg[s * kSlice + l]);
+    for (int l = 0; l < kSlice; l++) W = std::max(W, R.deg[R.node_of(s * kSlice + l)]);
     R.width[s] = W;
   }
   R.entries = 0;
@@ -513,6 +533,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(bloc, std::max<uint64_t>(c->cap_entries, 1));
   ALLOC(btab, S / kBlockSlices * kLdsCap);
   ALLOC(binfo, S / kBlockSlices);
+  ALLOC(lane_node, N);
+  ALLOC(node_lane, N);
   ALLOC(V, 3 * N);
   ALLOC(A, 3 * N);
   ALLOC(F, 3 * N);
@@ -571,6 +593,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   if (c->cap_entries) HIPCHK(c, hipMemsetAsync(d.bloc, 0xFF, c->cap_entries * sizeof(uint16_t), st));
   HIPCHK(c, hipMemsetAsync(d.btab, 0, S / kBlockSlices * kLdsCap * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.binfo, 0, S / kBlockSlices * sizeof(uint2), st));
+  HIPCHK(c, hipMemsetAsync(d.lane_node, 0, N, st));
+  HIPCHK(c, hipMemsetAsync(d.node_lane, 0, N, st));
   HIPCHK(c, hipMemsetAsync(d.lrtb, 0, 6 * P * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.pactive, 0, P * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.counters, 0, 8 * sizeof(unsigned long long), st));
@@ -633,6 +657,7 @@ struct pd_job_blob {
   std::vector<uint16_t> bloc;     // the entries as positions in their block's table, kPad16 padding
   std::vector<uint32_t> btab;     // per block, kLdsCap slots: sorted job-relative node indices
   std::vector<uint2> binfo;       // per block: table length, position of the block's first node
+  std::vector<uint8_t> lane_node, node_lane;   // see JobRows
   bool lds_ok = true;             // every block table fits kLdsCap
   uint32_t fl = 0;
   uint64_t movable_bonds = 0, total_bonds = 0;
@@ -795,8 +820,9 @@ int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t n
     at += (uint64_t)R.width[s] * kSlice;
   }
   parallel_for(R.nslices, work, [&](uint64_t s) {
-    for (uint64_t g = s * kSlice; g < std::min<uint64_t>(R.nodes, (s + 1) * kSlice); g++) {
-      uint32_t* col = B->bond.data() + B->soff[s] + (g % kSlice);
+    for (uint64_t t = s * kSlice; t < (s + 1) * kSlice; t++) {
+      const uint64_t g = R.node_of(t);
+      uint32_t* col = B->bond.data() + B->soff[s] + (t % kSlice);
       for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) col[(k - R.rstart[g]) * kSlice] = R.rows[k];
     }
   });
@@ -824,7 +850,8 @@ int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t n
     }
     auto pos = [&](uint32_t v) { return (uint32_t)(std::lower_bound(t.begin(), t.end(), v) - t.begin()); };
     for (uint64_t g = n0; g < n1; g++) {
-      uint16_t* col = B->bloc.data() + B->soff[g / kSlice] + (g % kSlice);
+      const uint64_t t = (g & ~(uint64_t)255) + R.node_lane[g];
+      uint16_t* col = B->bloc.data() + B->soff[t / kSlice] + (t % kSlice);
       for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++)
         col[(k - R.rstart[g]) * kSlice] = (uint16_t)pos(R.rows[k]);
     }
@@ -832,6 +859,8 @@ int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t n
     B->binfo[b] = make_uint2((uint32_t)t.size(), pos((uint32_t)n0));
   });
   for (uint64_t b = 0; b < nblk; b++) B->lds_ok &= !over[b];
+  B->lane_node = std::move(R.lane_node);
+  B->node_lane = std::move(R.node_lane);
   B->deg = std::move(R.deg);
   B->width = std::move(R.width);
   *out = B;
@@ -931,6 +960,8 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
   UP(d.brank + g0, B->brank.data(), Np);
   if (!blist.empty()) UP(d.blist + g0, blist.data(), blist.size());
   UP(d.deg + g0, B->deg.data(), Np);
+  UP(d.lane_node + g0, B->lane_node.data(), Np);
+  UP(d.node_lane + g0, B->node_lane.data(), Np);
   UP(d.soff + s0, soff.data(), B->nslices);
   UP(d.swid + s0, B->width.data(), B->nslices);
   if (B->entries) {
@@ -1668,14 +1699,18 @@ int pd_job_get_connectivity(pd_ctx* c, uint32_t job, uint32_t pi, uint32_t* pair
   int rc = locate(c, job, pi, &Hp);
   if (rc) return rc;
   const HostPart& p = *Hp;
-  // the particle's slices and their bond entries
-  const uint64_t s0 = p.off / kSlice, s1 = (p.off + p.n + kSlice - 1) / kSlice;
+  // the slices of the particle's node blocks and their bond entries (a node's
+  // row sits at the slot node_lane gives it in its block)
+  const uint64_t bn = (uint64_t)kBlockSlices * kSlice;
+  const uint64_t s0 = p.off / bn * kBlockSlices, s1 = (p.off + p.n + bn - 1) / bn * kBlockSlices;
   std::vector<uint64_t> soff(s1 - s0);
   std::vector<uint32_t> swid(s1 - s0), deg(p.n);
+  std::vector<uint8_t> nl(p.n);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(soff.data(), c->d.soff + s0, soff.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(swid.data(), c->d.swid + s0, swid.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(deg.data(), c->d.deg + p.off, p.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(nl.data(), c->d.node_lane + p.off, p.n, hipMemcpyDeviceToHost));
   const uint64_t lo = soff.front(), hi = soff.back() + (uint64_t)swid.back() * kSlice;
   std::vector<uint32_t> ent(hi - lo);
   if (!ent.empty())
@@ -1683,7 +1718,7 @@ int pd_job_get_connectivity(pd_ctx* c, uint32_t job, uint32_t pi, uint32_t* pair
   // NbdArr2conn (timeloop.h:20-44): pairs (i, q) with i < q, in row order
   std::vector<uint32_t> out;
   for (uint32_t i = 0; i < p.n; i++) {
-    const uint64_t g = p.off + i, s = g / kSlice, l = g % kSlice;
+    const uint64_t g = p.off + i, t = (g & ~(uint64_t)255) + nl[i], s = t / kSlice, l = t % kSlice;
     for (uint32_t k = 0; k < deg[i]; k++) {
       const uint32_t e = ent[soff[s - s0] - lo + (uint64_t)k * kSlice + l];
       if (e & kBroken) continue;

@@ -90,6 +90,11 @@ struct Dev {
   uint16_t* bloc;    // [entries] the bond entries as positions in their block's node table
   uint32_t* btab;    // [blocks * kLdsCap] per block: sorted pool indices of the nodes its rows reference
   uint2* binfo;      // [blocks] (table length, position of the block's first node in it)
+  // the lanes of a 256-node block hold its nodes by decreasing row length:
+  // slot t holds node (t & ~255) + lane_node[t]; node g sits at slot
+  // (g & ~255) + node_lane[g].  Bond slices are laid out by slot.
+  uint8_t* lane_node;
+  uint8_t* node_lane;
   double* V;         // vel [3N]
   double* A;         // acc [3N]
   double* F;         // force density [3N]

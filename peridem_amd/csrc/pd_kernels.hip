@@ -369,8 +369,8 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
 // this step (the force loop above already erased this step's breaks), as
 // std::find over NbdArr[i] sees it (contact.h:783, timeloop.h:552-557).
 __device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) {
-  uint64_t s = g >> 6;
-  const uint32_t* row = d.bond + d.soff[s] + (g & 63);
+  const uint64_t t = (g & ~255ull) + d.node_lane[g];   // the node's slot
+  const uint32_t* row = d.bond + d.soff[t >> 6] + (t & 63);
   uint32_t n = d.deg[g];
   if (sorted) {
     uint32_t lo = 0, hi = n;
@@ -400,16 +400,18 @@ __device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) 
 // particle2.h:107-193; remove_bonds, particle2.h:196-237).
 //
 // One thread per node sums its row in row order, exactly like the reference:
-// no atomics, no reordering.  The 64 nodes of a wavefront share one bond
+// no atomics, no reordering.  The 64 slots of a wavefront share one bond
 // slice (entry k of lane l at soff + k*64 + l), so the k-th entries of their
-// rows are one coalesced 256-byte load.  The stretch is tested against snot
-// right after the term is summed; a broken entry stops contributing from the
-// next step on, as the reference's erase after the force pass does.
+// rows are one coalesced load; within each 256-node block the slots hold the
+// nodes by decreasing row length, so a slice's width (its longest row)
+// leaves few padding entries.  The stretch is tested against snot right after
+// the term is summed; a broken entry stops contributing from the next step
+// on, as the reference's erase after the force pass does.
 //
 // Per term (particle2.h:139-153):
 //   e = (u_j - u_i) + xi,  s = (|e| - |xi|) / |xi|,  t = (cnot*s*V_j) * (e/|e|)
-// Eight entries are processed per iteration as straight-line code, so their
-// dependency chains interleave.  In the fast domain sqrt and division are the
+// Two to eight entries are processed per iteration as straight-line code, so
+// their dependency chains interleave.  In the fast domain sqrt and division are the
 // backend's own sequences without their range wrappers (bitwise equal);
 // whenever a lane leaves it, the whole batch uses the IEEE operators.  A dead
 // (broken or padding) entry reads the node itself, gets unit lengths and a
@@ -691,9 +693,9 @@ __device__ __forceinline__ uint64_t xcd_block() {
   return (blockIdx.x % kXcds) * (nbp / kXcds) + blockIdx.x / kXcds;
 }
 
-// One slice's offset and width, wave-uniform.
-__device__ __forceinline__ void slice_of(const Dev& d, uint64_t g, uint64_t& base, uint32_t& W) {
-  const uint64_t s = g >> 6;
+// The offset and width of slot t's slice, wave-uniform.
+__device__ __forceinline__ void slice_of(const Dev& d, uint64_t t, uint64_t& base, uint32_t& W) {
+  const uint64_t s = t >> 6;
   const uint64_t b0 = d.soff[s];
   const uint32_t blo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b0);
   const uint32_t bhi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b0 >> 32));
@@ -705,9 +707,10 @@ __device__ __forceinline__ void slice_of(const Dev& d, uint64_t g, uint64_t& bas
 // OFF32: see rec.
 template <int U, int PIPE, bool OFF32>
 __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
-  const uint64_t g = xcd_block() * blockDim.x + threadIdx.x;
+  const uint64_t t = xcd_block() * blockDim.x + threadIdx.x;   // slot
   // padded node range: S slices of 64; whole wavefronts exit together
-  if (g >= d.S * kSlice) return;
+  if (t >= d.S * kSlice) return;
+  const uint64_t g = (t & ~255ull) + d.lane_node[t];           // node
   const DevPart& P = d.part[d.pid[g]];
   const DevInst& I = d.inst[P.inst];
   if (!live(I)) return;            // frozen job or free slot: F keeps its value
@@ -715,8 +718,8 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
   if (P.movable) {
     uint64_t base;
     uint32_t W;
-    slice_of(d, g, base, W);
-    const uint32_t lane = (uint32_t)(g & 63);
+    slice_of(d, t, base, W);
+    const uint32_t lane = (uint32_t)(t & 63);
     const GlobalRows<OFF32> S{d,
                               {(const char*)(d.bond + base), lane * 4u},
                               {(const char*)(d.bloc + base), lane * 2u},
@@ -741,7 +744,7 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
 template <int U, int PIPE>
 __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
   const uint64_t blk = xcd_block();
-  const uint64_t g = blk * blockDim.x + threadIdx.x;
+  const uint64_t t = blk * blockDim.x + threadIdx.x;   // slot
   {
     uint32_t cnt = 0;
     if (blk < d.S / kBlockSlices) cnt = min(d.binfo[blk].x, kLdsCap);
@@ -754,7 +757,8 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
     }
     __syncthreads();
   }
-  if (g >= d.S * kSlice) return;
+  if (t >= d.S * kSlice) return;
+  const uint64_t g = (t & ~255ull) + d.lane_node[t];   // node
   const DevPart& P = d.part[d.pid[g]];
   const DevInst& I = d.inst[P.inst];
   if (!live(I)) return;
@@ -762,8 +766,8 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
   if (P.movable) {
     uint64_t base;
     uint32_t W;
-    slice_of(d, g, base, W);
-    const uint32_t lane = (uint32_t)(g & 63);
+    slice_of(d, t, base, W);
+    const uint32_t lane = (uint32_t)(t & 63);
     const uint32_t self = d.binfo[blk].y + (uint32_t)(g - blk * blockDim.x);
     const LdsRows S{d, {(const char*)(d.bloc + base), lane * 2u}, {(const char*)(d.bond + base), lane * 4u}, self};
     const double2 xy = s_xy[self], zv = s_zv[self];
