@@ -203,6 +203,34 @@ __device__ __forceinline__ double div_core(double a, Rcp R) {
   return __builtin_fma(rem, R.r, q);
 }
 
+// sqrt(x) and the refined reciprocal of that root.  The reciprocal starts
+// from the reciprocal square root that sqrt_core refines anyway (2h, already
+// at least as close to 1/sqrt(x) as v_rcp_f64 is to 1/s) and takes the same
+// two Newton-Raphson steps against the rounded root that rcp_core takes, so it
+// ends in the same error class -- without the transcendental.  div_core with
+// it is checked bit for bit against IEEE division by pd_selftest_math.
+struct SqrtRcp {
+  double s;   // correctly rounded sqrt(x)
+  Rcp r;      // refined reciprocal of s
+};
+__device__ __forceinline__ SqrtRcp sqrt_rcp_core(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = y * 0.5;
+  double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  const double r0 = h + h;
+  const double e0 = __builtin_fma(-g, r0, 1.0);
+  const double r1 = __builtin_fma(r0, e0, r0);
+  const double e1 = __builtin_fma(-g, r1, 1.0);
+  return {g, {-g, __builtin_fma(r1, e1, r1)}};
+}
+
 // Eigen 3.3 fixed-size 1x3 norm/dot order.
 __device__ __forceinline__ double nrm3(double a, double b, double c) {
   return sqrt((a * a + b * b) + c * c);

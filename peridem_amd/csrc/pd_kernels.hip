@@ -412,8 +412,10 @@ __device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) 
 //   e = (u_j - u_i) + xi,  s = (|e| - |xi|) / |xi|,  t = (cnot*s*V_j) * (e/|e|)
 // Two to eight entries are processed per iteration as straight-line code, so
 // their dependency chains interleave.  In the fast domain sqrt and division are the
-// backend's own sequences without their range wrappers (bitwise equal);
-// whenever a lane leaves it, the whole batch uses the IEEE operators.  A dead
+// backend's own sequences without their range wrappers, the divisors'
+// reciprocals refined from the square roots' own reciprocal estimates
+// (bitwise equal to IEEE sqrt and division); whenever a lane leaves the
+// domain, the whole batch uses the IEEE operators.  A dead
 // (broken or padding) entry reads the node itself, gets unit lengths and a
 // zero coefficient, and adds +-0 to a sum that is never -0: no effect.
 //
@@ -591,13 +593,12 @@ __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint3
   if (__all(fast & (over < kN2Span))) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      double nn = sqrt_core(n2[u]);
-      double L = sqrt_core(l2[u]);
-      str[u] = div_core(nn - L, rcp_core(L));
+      const SqrtRcp sn = sqrt_rcp_core(n2[u]);
+      const SqrtRcp sl = sqrt_rcp_core(l2[u]);
+      str[u] = div_core(sn.s - sl.s, sl.r);
       double c = (cnot * str[u]) * xj[u].w;
-      Rcp rn = rcp_core(nn);
 #pragma unroll
-      for (int q = 0; q < 3; q++) t[u][q] = c * div_core(ev[u][q], rn);
+      for (int q = 0; q < 3; q++) t[u][q] = c * div_core(ev[u][q], sn.r);
     }
   } else {
 #pragma unroll
@@ -1202,8 +1203,24 @@ __global__ __launch_bounds__(256) void k_selftest_math(uint64_t n, uint64_t seed
                          : (__double_as_longlong(q) == __double_as_longlong(qi));
     bad_div += eq ? 0u : 1u;
     // sqrt argument in [2^-600, 2^601)
-    double x = make_double(r2 >> 3, (int)(r3 % 1201) - 600, 0);
-    bad_sqrt += (__double_as_longlong(sqrt_core(x)) == __double_as_longlong(sqrt(x))) ? 0u : 1u;
+    const int ex = (int)(r3 % 1201) - 600;
+    double x = make_double(r2 >> 3, ex, 0);
+    const double sx = sqrt(x);
+    bad_sqrt += (__double_as_longlong(sqrt_core(x)) == __double_as_longlong(sx)) ? 0u : 1u;
+    // division by that root through sqrt_rcp_core's reciprocal: numerators
+    // as above relative to the root (components of e, differences of lengths)
+    const SqrtRcp sr = sqrt_rcp_core(x);
+    bad_sqrt += (__double_as_longlong(sr.s) == __double_as_longlong(sx)) ? 0u : 1u;
+    const int es = ex >> 1;
+    const uint64_t r4 = mix64(seed ^ ~(i * 4 + 1));
+    const int ea2 = es + 1 - (int)((r4 >> 32) % 653);
+    double a2 = make_double(r4, ea2, (int)(r3 >> 63));
+    if ((r4 & 31) == 0) a2 = ((r4 >> 8) & 1) ? -0.0 : 0.0;
+    const double q2 = div_core(a2, sr.r);
+    const double q2i = a2 / sx;
+    const bool eq2 = (a2 == 0.0) ? (q2 == 0.0 && q2i == 0.0)
+                                 : (__double_as_longlong(q2) == __double_as_longlong(q2i));
+    bad_div += eq2 ? 0u : 1u;
   }
   if (bad_div) atomicAdd(&mism[0], (unsigned long long)bad_div);
   if (bad_sqrt) atomicAdd(&mism[1], (unsigned long long)bad_sqrt);
