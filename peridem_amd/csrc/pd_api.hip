@@ -1154,9 +1154,8 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
   if ((rc = mark(0))) return rc;
   hipLaunchKernelGGL(k_position, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if ((rc = mark(1))) return rc;
-  hipLaunchKernelGGL(k_decide, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
+  hipLaunchKernelGGL(k_decide_pairs, dim3(grid_of(std::max<uint64_t>(d.ninst, d.npairs), 256)), dim3(256), 0, st, d);
   if (any_torque(c)) hipLaunchKernelGGL(k_centroid, dim3(grid_of(d.P, 64)), dim3(64), 0, st, d);
-  if (d.npairs) hipLaunchKernelGGL(k_pairs, dim3(grid_of(d.npairs, 256)), dim3(256), 0, st, d);
   if (any_contact_possible(c)) {
     // every kernel of the list rebuild exits at once unless a job rebuilds
     hipLaunchKernelGGL(k_clear_cells, dim3(grid_of((uint64_t)d.T + 1, 256)), dim3(256), 0, st, d);

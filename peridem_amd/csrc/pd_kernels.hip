@@ -134,9 +134,7 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
 // so the exact reference tests applied to the list members find exactly the
 // reference's contact pairs.  One thread per job decides, from the largest
 // displacement since the build, whether its lists are rebuilt this step.
-__global__ void k_decide(Dev d) {
-  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= d.ninst) return;
+__device__ __forceinline__ void decide_job(Dev& d, uint64_t q) {
   DevInst& I = d.inst[q];
   uint32_t r = 0;
   if (live(I)) {
@@ -178,9 +176,7 @@ __global__ __launch_bounds__(64) void k_centroid(Dev d) {
 // ---------------------------------------------------------------------------
 // Pair box test on the lagged boxes, pair list in the reference's pair order
 // restricted to one instance (timeloop.h:243-255, 636-653).
-__global__ __launch_bounds__(256) void k_pairs(Dev d) {
-  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= d.npairs) return;
+__device__ __forceinline__ void pair_test(Dev& d, uint64_t q) {
   uint32_t i = d.pairs[2 * q], j = d.pairs[2 * q + 1];
   const DevPart& Pi = d.part[i];
   const DevPart& Pj = d.part[j];
@@ -192,6 +188,13 @@ __global__ __launch_bounds__(256) void k_pairs(Dev d) {
   // bit 0: the particle receives contact terms (only stoppable ones do)
   atomicOr(&d.pactive[i], 2u | (Pi.stoppable ? 1u : 0u));
   atomicOr(&d.pactive[j], 2u | (Pj.stoppable ? 1u : 0u));
+}
+
+// The two are independent; one launch does both (thread q: job q, pair q).
+__global__ __launch_bounds__(256) void k_decide_pairs(Dev d) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < d.ninst) decide_job(d, q);
+  if (q < d.npairs) pair_test(d, q);
 }
 
 __device__ __forceinline__ bool self_active(const DevPart& P, const DevInst& I) {
