@@ -836,19 +836,36 @@ int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t n
   parallel_for(nblk, work, [&](uint64_t b) {
     const uint64_t n0 = b * bn, n1 = std::min<uint64_t>(R.nodes, n0 + bn);
     if (n0 >= n1) return;
-    std::vector<uint32_t> t;
-    t.reserve((n1 - n0) + (R.rstart[n1] - R.rstart[n0]));
-    for (uint64_t g = n0; g < n1; g++) {
-      t.push_back((uint32_t)g);
-      for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) t.push_back(R.rows[k]);
+    // per worker thread: a mark per job node (the block being built) and the
+    // table position of each marked node
+    thread_local std::vector<uint32_t> mark;
+    thread_local std::vector<uint32_t> where;
+    thread_local uint32_t epoch = 0;
+    if (mark.size() < R.nodes || epoch == ~0u) {
+      mark.assign(std::max<uint64_t>(R.nodes, mark.size()), 0);
+      where.resize(mark.size());
+      epoch = 0;
     }
-    std::sort(t.begin(), t.end());
-    t.erase(std::unique(t.begin(), t.end()), t.end());
+    const uint32_t tag = ++epoch;
+    std::vector<uint32_t> t;
+    t.reserve(kLdsCap + bn);
+    auto add = [&](uint32_t v) {
+      if (mark[v] != tag) {
+        mark[v] = tag;
+        t.push_back(v);
+      }
+    };
+    for (uint64_t g = n0; g < n1; g++) {
+      add((uint32_t)g);
+      for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++) add(R.rows[k]);
+    }
     if (t.size() > kLdsCap) {
       over[b] = 1;
       return;
     }
-    auto pos = [&](uint32_t v) { return (uint32_t)(std::lower_bound(t.begin(), t.end(), v) - t.begin()); };
+    std::sort(t.begin(), t.end());
+    for (uint32_t i = 0; i < t.size(); i++) where[t[i]] = i;
+    auto pos = [&](uint32_t v) { return where[v]; };
     for (uint64_t g = n0; g < n1; g++) {
       const uint64_t t = (g & ~(uint64_t)255) + R.node_lane[g];
       uint16_t* col = B->bloc.data() + B->soff[t / kSlice] + (t % kSlice);

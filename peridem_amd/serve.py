@@ -30,6 +30,10 @@ import numpy as np
 from . import abi
 
 
+# host threads laying out jobs (each call is single-threaded for small jobs)
+PREPARE_WORKERS = 16
+
+
 def digest_state(read_state, read_conn, read_wall, nparticles):
     """SHA-1 over every particle's disp, CurrPos, vel, acc, force, its intact
     bonds and the wall planes and reaction: equal digests = bitwise-equal results."""
@@ -79,7 +83,17 @@ def run_batch(systems, nsteps, digests=False):
     pool = abi.Pool(cap["max_nodes"], cap["max_bond_entries"], cap["max_particles"], cap["max_jobs"])
     try:
         budgets = nsteps if isinstance(nsteps, (list, tuple)) else [nsteps] * len(systems)
-        jobs = [pool.add_job(s, n) for s, n in zip(systems, budgets)]
+        # the host layout of every job (pd_job_prepare, thread-safe) on worker
+        # threads, then placement in arrival order
+        with ThreadPoolExecutor(max_workers=PREPARE_WORKERS) as ex:
+            blobs = list(ex.map(abi.prepare_job, systems))
+        jobs = []
+        for b, s, n in zip(blobs, systems, budgets):
+            j = pool.add_prepared(b, s, n)
+            b.free()
+            if j is None:
+                raise RuntimeError("job does not fit the pool sized for it")
+            jobs.append(j)
         t1 = time.perf_counter()
         dev_ms = 0.0
         left = max(budgets)
@@ -160,7 +174,8 @@ class Server:
         self.steps_run = 0
         self.device_ms = 0.0
         self.peak_live = 0
-        self.workers = ThreadPoolExecutor(max_workers=1) if self.prepare_ahead else None
+        self.workers = ThreadPoolExecutor(max_workers=min(self.prepare_ahead, PREPARE_WORKERS)) \
+            if self.prepare_ahead else None
 
     def close(self):
         if self.workers:
