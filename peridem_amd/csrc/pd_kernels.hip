@@ -517,8 +517,10 @@ struct GlobalRows {
   __device__ __forceinline__ const SliceRows<uint32_t>& rows() const { return R; }
   __device__ __forceinline__ bool alive(uint32_t e) const { return (int32_t)e >= 0; }
   template <int U>
-  __device__ __forceinline__ void gather(const uint32_t (&e)[U], const double4& xi, const double4& ui,
-                                         double4 (&xj)[U], double4 (&uj)[U]) const {
+  __device__ __forceinline__ void resolve(const uint32_t (&)[U], uint32_t (&)[U]) const {}
+  template <int U>
+  __device__ __forceinline__ void gather(const uint32_t (&e)[U], const uint32_t (&)[U], double4 (&xj)[U],
+                                         double4 (&uj)[U]) const {
 #pragma unroll
     for (int u = 0; u < U; u++) rec<OFF32>(d, alive(e[u]) ? e[u] : g, xj[u], uj[u]);
   }
@@ -534,9 +536,11 @@ struct GlobalRows {
 // nodes its rows reference, bit 15 = broken); the block's table and the
 // static part of those nodes' records (x, y, z, vol) staged in LDS once per
 // launch, the displacement gathered from the split copies through the
-// table.  A dead entry uses the node's own position in the table.
+// table.  A dead entry uses the node's own position in the table.  The
+// table lookup (resolve) runs one pipeline stage before the gather, so the
+// displacement loads issue without waiting on LDS.
 __shared__ double2 s_xy[kLdsCap], s_zv[kLdsCap];
-__shared__ uint32_t s_id[kLdsCap];
+__shared__ uint32_t s_id[kLdsCap];   // 8 * pool node index: the byte offset of its UZ record
 struct LdsRows {
   const Dev& d;
   SliceRows<uint16_t> R16;
@@ -545,19 +549,27 @@ struct LdsRows {
   static constexpr uint32_t kPadE = kPad16;
   __device__ __forceinline__ const SliceRows<uint16_t>& rows() const { return R16; }
   __device__ __forceinline__ bool alive(uint32_t e) const { return (e & kDead16) == 0; }
+  __device__ __forceinline__ uint32_t pos(uint32_t e) const { return e < kDead16 ? e : self; }
   template <int U>
-  __device__ __forceinline__ void gather(const uint32_t (&e)[U], const double4&, const double4&, double4 (&xj)[U],
+  __device__ __forceinline__ void resolve(const uint32_t (&e)[U], uint32_t (&id8)[U]) const {
+#pragma unroll
+    for (int u = 0; u < U; u++) id8[u] = s_id[pos(e[u])];
+  }
+  template <int U>
+  __device__ __forceinline__ void gather(const uint32_t (&e)[U], const uint32_t (&id8)[U], double4 (&xj)[U],
                                          double4 (&uj)[U]) const {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const uint32_t t = alive(e[u]) ? e[u] : self;
+      const double2 uxy = *(const double2*)((const char*)d.UXY + (id8[u] << 1));
+      const double uz = *(const double*)((const char*)d.UZ + id8[u]);
+      uj[u] = make_double4(uxy.x, uxy.y, uz, 0.0);
+      const uint32_t t = pos(e[u]);
       const double2 xy = s_xy[t], zv = s_zv[t];
       xj[u] = make_double4(xy.x, xy.y, zv.x, zv.y);
-      rec_u<true>(d, s_id[t], uj[u]);
     }
   }
   __device__ __forceinline__ void broke(uint32_t k, uint32_t e) const {
-    *R.at(k) = s_id[e] | kBroken;
+    *R.at(k) = (s_id[e] >> 3) | kBroken;
     *R16.at(k) = (uint16_t)(e | kDead16);
   }
 };
@@ -639,45 +651,55 @@ __device__ __forceinline__ void bond_rows(const Src& S, uint32_t W, const double
   const uint32_t pad = Src::kPadE;
   if (PIPE == 0) {
     for (uint32_t k = 0; k < W; k += U) {
-      uint32_t e[U];
+      uint32_t e[U], a[U];
       double4 xj[U], uj[U];
       bond_idx<U>(R, k, W, pad, e);
-      S.template gather<U>(e, xi, ui, xj, uj);
+      S.template resolve<U>(e, a);
+      S.template gather<U>(e, a, xj, uj);
       bond_terms<U>(S, k, e, xj, uj, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
     }
   } else if (PIPE == 2 && W) {
-    uint32_t ea[U], eb[U];
+    // batch k: records in (xa, ua); k+U: entries resolved, records loading
+    // into (xb, ub); k+2U: entries loading, resolved after batch k's terms
+    uint32_t ea[U], eb[U], aa[U], ab[U];
     double4 xa[U], ua[U], xb[U], ub[U];
     bond_idx<U>(R, 0, W, pad, ea);
     bond_idx<U>(R, U, W, pad, eb);
-    S.template gather<U>(ea, xi, ui, xa, ua);
+    S.template resolve<U>(ea, aa);
+    S.template gather<U>(ea, aa, xa, ua);
+    S.template resolve<U>(eb, ab);
     for (uint32_t k = 0; k < W; k += 2 * U) {
-      uint32_t ec[U];
+      uint32_t ec[U], ac[U];
       bond_idx<U>(R, k + 2 * U, W, pad, ec);
-      S.template gather<U>(eb, xi, ui, xb, ub);
+      S.template gather<U>(eb, ab, xb, ub);
       bond_terms<U>(S, k, ea, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
       if (k + U >= W) break;
+      S.template resolve<U>(ec, ac);
       bond_idx<U>(R, k + 3 * U, W, pad, ea);
-      S.template gather<U>(ec, xi, ui, xa, ua);
+      S.template gather<U>(ec, ac, xa, ua);
       bond_terms<U>(S, k + U, eb, xb, ub, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+      S.template resolve<U>(ea, ab);   // batch k + 3U
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        eb[u] = ea[u];   // batch k + 3U
+        eb[u] = ea[u];   // batch k + 3U, resolved
         ea[u] = ec[u];   // batch k + 2U, its records already in xa/ua
       }
     }
   } else if (W) {
-    uint32_t e0[U], e1[U];
+    uint32_t e0[U], e1[U], a1[U];
     double4 xa[U], ua[U];
     bond_idx<U>(R, 0, W, pad, e0);
     bond_idx<U>(R, U, W, pad, e1);
-    S.template gather<U>(e0, xi, ui, xa, ua);
+    S.template resolve<U>(e0, a1);
+    S.template gather<U>(e0, a1, xa, ua);
+    S.template resolve<U>(e1, a1);
     for (uint32_t k = 0; k < W; k += U) {
       uint32_t e2[U];
       double4 xb[U], ub[U];
       bond_idx<U>(R, k + 2 * U, W, pad, e2);
-      S.template gather<U>(e1, xi, ui, xb, ub);
+      S.template gather<U>(e1, a1, xb, ub);
       bond_terms<U>(S, k, e0, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+      S.template resolve<U>(e2, a1);
 #pragma unroll
       for (int u = 0; u < U; u++) {
         e0[u] = e1[u];
@@ -755,7 +777,7 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
     const uint32_t* tab = d.btab + blk * (uint64_t)kLdsCap;
     for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
       const uint32_t id = tab[i];
-      s_id[i] = id;
+      s_id[i] = id << 3;
       s_xy[i] = d.XY[id];
       s_zv[i] = d.ZV[id];
     }
