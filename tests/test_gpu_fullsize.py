@@ -98,3 +98,44 @@ def test_two_sphere_sample_matches_oracle_bitwise(oracle_lib):
     finally:
         g.close()
         o.close()
+
+
+def test_crushing_aggregate_full_size_sample_matches_oracle(oracle_lib):
+    """The crushing-aggregate configuration at full size (512 cubes, spheres
+    and convex polyhedra of 2k-12k nodes, 2.85M nodes, 256M directed bonds,
+    self-contact, walls with the top one moving, wall reaction on), every
+    node with a seeded velocity so every bond carries stretch.  The particles
+    sit in separate grid cells (gaps of at least 4h > R_c, 2h to the walls),
+    so for these few steps no contact exists and two of them, stepped inside
+    the full pool, must equal the oracle stepping those two alone with the
+    same walls, bit for bit."""
+    s = gen.crushing_aggregate(512)
+    rng = np.random.default_rng(17)
+    for p in s.particles:
+        p.vel = rng.uniform(-1.0, 1.0, size=(p.n, 3))
+    print(f"[fullsize] crushing aggregate: {len(s.particles)} particles, {s.nodes} nodes, "
+          f"{s.bonds} directed bonds", flush=True)
+    assert s.nodes == 2852640 and s.bonds == 256487986
+    sample = gen.System(**{**s.__dict__, "particles": s.particles[:2]})
+    g = abi.gpu_engine(s)
+    o = oracle_lib.engine(sample)
+    try:
+        done = 0
+        for c in (1, 4, 10):
+            g.step(c - done)
+            o.step(c - done)
+            done = c
+            for p in range(2):
+                sg, so = g.state(p), o.state(p)
+                for k in FIELDS:
+                    assert_bitwise(sg[k], so[k], f"step {c} particle {p} {k}")
+                assert np.array_equal(g.connectivity(p), o.connectivity(p)), f"step {c} particle {p} bonds"
+            lg, rg = g.wall()
+            lo, ro = o.wall()
+            assert_bitwise(lg, lo, f"step {c} wall planes")
+            assert not np.any(rg), "no node touches a wall yet"
+        cnt = g.counters()
+        assert cnt["alive_bonds"] == s.bonds and cnt["pair_computations"] == 0
+    finally:
+        g.close()
+        o.close()
