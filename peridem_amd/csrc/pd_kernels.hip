@@ -2,11 +2,13 @@
 //
 // One step (timeloop.h:395-797) is the launch sequence
 //   k_position   Verlet position half + per-particle box of CurrPos
-//   k_decide     per job: rebuild the contact candidate lists this step?
-//   k_pairs      particle-pair box test on the lagged boxes (timeloop.h:636-653)
+//   k_decide_pairs  per job: rebuild the contact candidate lists this step?
+//                and the particle-pair box test on the lagged boxes
+//                (timeloop.h:636-653)
 //   cell list    k_clear_cells -> k_cellkey -> scan -> k_scatter -> k_build
 //                (only in steps where some job rebuilds its candidate lists)
-//   k_bond       PD bond force + fused irreversible break, per-node row sums
+//   bond kernel  PD bond force + fused irreversible break, per-node row sums
+//                (k_bond_lds: block tables staged in LDS; k_bond_t: unstaged)
 //   k_contact    self/wall/ext/pair contact added in the reference's order
 //   k_update     Verlet velocity half, clamped nodes
 //   k_finalize   lagged boxes, wall motion
