@@ -22,7 +22,8 @@ fp64_issue: FP64 lane-instructions per second of the same kernel launch
 SQ_INSTS_VALU_*_F64 summary, times 64 lanes, over the live-measured average
 duration) against the FP64 vector issue peak.
 cpu_baseline: the CPU oracle (same algorithm, OpenMP threads emulating the
-reference's MPI ranks) on a bounded sample of the same workload.
+reference's MPI ranks) on a bounded sample of the same workload, all host
+threads up to 16 (value) and one thread (single_core), with the host CPU model.
 
 batched: 256 independent two-sphere impacts (node counts per sphere
 log-uniform on [500, 20k], speed U(5, 50) m/s, E U(20, 100) GPa with s0 and
@@ -99,23 +100,45 @@ def shape_name(k, nodes):
     return f"pd::k_bond_t<{u}, {pipe}, {off32}>"
 
 
-def cpu_baseline(s, nspheres, steps, threads):
-    """The oracle on the first `nspheres` spheres of the workload `s`."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(s, nspheres, steps, threads, steps_1core):
+    """The oracle on the first `nspheres` spheres of the workload `s`, on
+    `threads` host threads and (a shorter window) on one."""
     from oracle import oracle
     from peridem_amd import gen
     if nspheres < len(s.particles):
         s = gen.System(**{**s.__dict__, "particles": s.particles[:nspheres]})
-    o = oracle.engine(s, ranks=threads)
-    o.step(2)
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
-    bu = s.movable_bonds() * steps
-    o.close()
-    return {"value": bu / dt, "unit": "bond-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps (after 2 untimed) of {len(s.particles)} of the 125 spheres "
-                      f"({s.nodes} nodes, {s.bonds} directed bonds), the oracle with {threads} "
-                      f"OpenMP threads standing in for the reference's MPI ranks, {dt:.1f} s"}
+
+    def timed(ranks, n):
+        o = oracle.engine(s, ranks=ranks)
+        o.step(2)
+        t0 = time.perf_counter()
+        o.step(n)
+        dt = time.perf_counter() - t0
+        o.close()
+        return s.movable_bonds() * n / dt, dt
+
+    v, dt = timed(threads, steps)
+    out = {"value": v, "unit": "bond-updates/s", "cores": threads, "kind": "port",
+           "sample": f"{steps} steps (after 2 untimed) of {len(s.particles)} of the 125 spheres "
+                     f"({s.nodes} nodes, {s.bonds} directed bonds), the oracle with {threads} "
+                     f"OpenMP threads standing in for the reference's MPI ranks, {dt:.1f} s",
+           "host": f"{cpu_model()}, {os.cpu_count()} logical CPUs visible"}
+    if steps_1core:
+        v1, dt1 = timed(1, steps_1core)
+        out["single_core"] = {"value": v1, "sample": f"{steps_1core} steps of the same spheres on one "
+                                                     f"thread, {dt1:.1f} s"}
+    return out
 
 
 def batched(args):
@@ -213,6 +236,8 @@ def main():
     ap.add_argument("--cpu-spheres", type=int, default=125)
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-steps-1core", type=int, default=8,
+                    help="steps of the single-thread CPU sample (0: skip it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256)
@@ -296,7 +321,7 @@ def main():
     if not args.no_cpu:
         log(f"[bench] CPU baseline: {args.cpu_spheres} spheres x {args.cpu_steps} steps, "
             f"{args.cpu_threads} threads")
-        cpu = cpu_baseline(s, args.cpu_spheres, args.cpu_steps, args.cpu_threads)
+        cpu = cpu_baseline(s, args.cpu_spheres, args.cpu_steps, args.cpu_threads, args.cpu_steps_1core)
 
     out = {
         "metric": "bond-updates/s & % HBM peak; batched sims/hr + p99 latency vs plain GPU",
