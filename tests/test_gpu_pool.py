@@ -212,3 +212,37 @@ def test_profiled_graph_replay_keeps_results_and_times_every_step():
     finally:
         a.close()
         b.close()
+
+
+def test_admitting_a_job_with_overflowing_block_tables_switches_shape():
+    """A pool steps a job with the staged bond kernel; a job whose 256-node
+    blocks reference more nodes than the staged kernel's tables hold (a
+    horizon of 5h) is admitted mid-run, so launches fall back to the
+    unstaged kernel until it leaves, then return to the staged one.  Both
+    jobs equal their solo runs bit for bit."""
+    from helpers import small_impact
+    a = small_impact(seed=23, R_h=3.0)
+    rng = np.random.default_rng(29)
+    wide = gen.make_particle(gen.lattice_cube(8), 1e-4, np.zeros(3), rng, horizon=5.0)
+    wide.vel = rng.uniform(-2.0, 2.0, size=(wide.n, 3))
+    b = gen.System(particles=[wide], dt=1e-8)
+    shapes = abi.bond_shapes()
+    staged = next(k for k, sh in enumerate(shapes) if sh[2])
+    ref_a = alone(a, 60)
+    ref_b = alone(b, 20)
+    pool = pool_for([a, b], slack=1.2)
+    try:
+        ja = pool.add_job(a, 60)
+        pool.set_bond_shape(staged)
+        pool.step(20)
+        assert pool.bond_shape() == staged
+        jb = pool.add_job(b, 20)
+        pool.step(20)                       # both jobs: the unstaged default runs
+        assert not shapes[pool.bond_shape()][2]
+        compare_job(pool, jb, ref_b, "wide job")
+        pool.remove_job(jb)
+        assert pool.bond_shape() == staged  # the staged shape is usable again
+        pool.step(20)
+        compare_job(pool, ja, ref_a, "job across the switches")
+    finally:
+        pool.close()
