@@ -249,9 +249,14 @@ int pd_autotune(pd_ctx* ctx, uint32_t reps, int* best, double* ms);
 
 /* Per-phase device time, measured with HIP events on the context's stream
  * while profiling is on: profiled steps are launched one by one (not replayed
- * from a graph) with a mark before each phase, and each step's marks are read
- * eight steps later, so the stream stays busy.  No counterpart in the reference, which
- * only times whole modulo blocks (timeloop.h:909-914). */
+ * from a graph) with a mark before each phase (PD_PROFILE_PHASES) or only
+ * around the bond kernel (PD_PROFILE_BOND: fewer marks in the stream, only
+ * the bond phase is reported), and each step's marks are read eight steps
+ * later, so the stream stays busy.  pd_set_profiling(ctx, 0) switches it off.
+ * No counterpart in the reference, which only times whole modulo blocks
+ * (timeloop.h:909-914). */
+#define PD_PROFILE_PHASES 1
+#define PD_PROFILE_BOND 2
 #define PD_PHASE_POSITION 0 /* Verlet position half + boxes */
 #define PD_PHASE_CONTACT_LIST 1 /* pair box test + cell-list rebuild */
 #define PD_PHASE_BOND 2     /* bond force + fused break (the hot kernel) */

@@ -27,6 +27,8 @@ PD_FLAG_ALLOW_FRICTION = 4
 PD_FLAG_ENABLE_TORQUE = 5
 PD_FLAG_COMPUTE_WALL_REACTION = 6
 PD_FLAG_VELOCITY_CONSTRAINT = 7
+PD_PROFILE_PHASES = 1   # a mark around every phase of a profiled step
+PD_PROFILE_BOND = 2     # marks around the bond kernel only
 PD_PFLAG_MOVABLE = 1
 PD_PFLAG_STOPPABLE = 2
 PD_PFLAG_BREAKABLE = 3

@@ -151,7 +151,7 @@ struct pd_ctx {
   std::vector<uint32_t> part_free;
   pd_counters cnt{};
   double last_ms = 0;
-  bool prof = false;
+  int prof = 0;                 // PD_PROFILE_PHASES or PD_PROFILE_BOND while profiling
   hipEvent_t pev[kProfRing][PD_NPHASE + 1] = {};    // phase marks of profiled steps
   pd_profile acc{};
   bool use_graph = true, graph_dirty = true;
@@ -1147,6 +1147,14 @@ static int effective_shape(const pd_ctx* c) {
 
 // Phase times of one profiled step (waits for its last mark).
 static int accumulate_marks(pd_ctx* c, hipEvent_t* ev) {
+  if (c->prof == PD_PROFILE_BOND) {
+    HIPCHK(c, hipEventSynchronize(ev[PD_PHASE_BOND + 1]));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[PD_PHASE_BOND], ev[PD_PHASE_BOND + 1]));
+    c->acc.ms[PD_PHASE_BOND] += ms;
+    c->acc.launches[PD_PHASE_BOND] += 1;
+    return PD_OK;
+  }
   HIPCHK(c, hipEventSynchronize(ev[PD_NPHASE]));
   for (int k = 0; k < PD_NPHASE; k++) {
     float ms = 0;
@@ -1158,13 +1166,14 @@ static int accumulate_marks(pd_ctx* c, hipEvent_t* ev) {
 }
 
 // One step's launches; with `ev` set, a mark is recorded before each phase
-// and after the last.
+// and after the last (PD_PROFILE_BOND: only around the bond kernel).
 static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
   const Dev d = step_view(c);
   hipStream_t st = c->stream;
   const uint64_t Np = d.S * kSlice;
   auto mark = [&](int k) -> int {
-    if (ev) HIPCHK(c, hipEventRecord(ev[k], st));
+    if (ev && (c->prof == PD_PROFILE_PHASES || k == PD_PHASE_BOND || k == PD_PHASE_BOND + 1))
+      HIPCHK(c, hipEventRecord(ev[k], st));
     return PD_OK;
   };
   int rc;
@@ -1809,11 +1818,11 @@ int pd_get_counters(pd_ctx* c, pd_counters* out) {
 }
 
 int pd_set_profiling(pd_ctx* c, int on) {
-  if (!c) return PD_E_ARG;
+  if (!c || on < 0 || on > PD_PROFILE_BOND) return PD_E_ARG;
   if (on && !c->pev[0][0])
     for (int s = 0; s < kProfRing; s++)
       for (int k = 0; k <= PD_NPHASE; k++) HIPCHK(c, hipEventCreate(&c->pev[s][k]));
-  c->prof = on != 0;
+  c->prof = on;
   c->acc = pd_profile{};
   return PD_OK;
 }

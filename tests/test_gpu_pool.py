@@ -246,3 +246,23 @@ def test_admitting_a_job_with_overflowing_block_tables_switches_shape():
         compare_job(pool, ja, ref_a, "job across the switches")
     finally:
         pool.close()
+
+
+def test_bond_only_profiling_counts_the_bond_phase_only():
+    s = small_jobs(4, seed=41)[0]
+    a, b = abi.gpu_engine(s), abi.gpu_engine(s)
+    try:
+        a.set_profiling(abi.PD_PROFILE_BOND)
+        for n in (3, 20):
+            a.step(n)
+            b.step(n)
+        pr = a.profile()
+        assert pr["bond"]["launches"] == 23 and pr["bond"]["ms"] > 0.0
+        for name in ("position", "contact_list", "contact", "update"):
+            assert pr[name]["launches"] == 0, (name, pr[name])
+        a.set_profiling(0)
+        for p in range(len(s.particles)):
+            assert_bitwise(a.state(p)["force"], b.state(p)["force"], f"particle {p} force")
+    finally:
+        a.close()
+        b.close()
