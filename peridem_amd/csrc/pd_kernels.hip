@@ -819,7 +819,7 @@ struct BondShape {
   void (*fn)(Dev);
   int unroll, pipe, lds;   // pipe: 0 none, 1 pipelined, 2 pipelined and unrolled twice
 };
-constexpr int kBondShapes = 13;
+constexpr int kBondShapes = 14;
 constexpr uint64_t kOff32MaxNodes = 1ull << 27;   // 16-byte records far below 4 GiB
 __host__ inline const BondShape* bond_shapes() {
   static const BondShape t[kBondShapes] = {
@@ -827,7 +827,7 @@ __host__ inline const BondShape* bond_shapes() {
       {k_bond_t<3, 1, true>, 3, 1, 0}, {k_bond_t<4, 1, true>, 4, 1, 0}, {k_bond_t<5, 1, true>, 5, 1, 0},
       {k_bond_t<3, 2, true>, 3, 2, 0}, {k_bond_t<4, 2, true>, 4, 2, 0}, {k_bond_t<2, 2, true>, 2, 2, 0},
       {k_bond_lds<3, 2>, 3, 2, 1},     {k_bond_lds<4, 2>, 4, 2, 1},     {k_bond_lds<2, 2>, 2, 2, 1},
-      {k_bond_lds<4, 1>, 4, 1, 1}};
+      {k_bond_lds<4, 1>, 4, 1, 1},     {k_bond_lds<5, 2>, 5, 2, 1}};
   return t;
 }
 // The default shape with 64-bit record addressing, for pools too large for
