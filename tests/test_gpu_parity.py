@@ -300,3 +300,18 @@ def test_crushing_aggregate_force_displacement_trace(oracle_lib):
     broken = 1 - cg["alive_bonds"] / cg["total_bonds"]
     print(f"[parity] crushing: broken fraction {broken:.3f} after 800 steps", flush=True)
     assert 0.05 < broken < 0.9
+
+
+def test_pair_activity_lags_one_step_like_the_oracle(oracle_lib):
+    """A node crossing into R_c within one step: the pair test uses the
+    previous step's boxes (timeloop.h:389-393,779), so contact starts one
+    step late -- the library and the oracle agree bit for bit."""
+    Rc, V = 9e-5, 1000.0
+    A = gen.Particle(pos=np.zeros((1, 3)), vol=np.array([2e-12]), conn=np.zeros((0, 2), np.uint32),
+                     delta=3e-4, rho=2000.0, cnot=3e25, snot=2e-3)
+    A.vel = np.array([[V, 0.0, 0.0]])
+    B = gen.Particle(pos=np.array([[Rc + 0.5 * V * 1e-8, 0.0, 0.0]]), vol=np.array([5e-12]),
+                     conn=np.zeros((0, 2), np.uint32), delta=3e-4, rho=2440.0, cnot=3e25, snot=2e-3)
+    s = gen.System(particles=[A, B], dt=1e-8, contact_rad=Rc, normal_stiffness=4.6e31,
+                   friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=0)
+    run_pair(s, (1, 2, 3), oracle_lib, "lagged pair boxes")

@@ -7,7 +7,8 @@ sign or a swapped operand fails at least one check:
                momentum balance, strict s > s0 threshold, irreversibility
   conn2NArr    row order for a hand-made unsorted Connectivity
   contact      two-node normal spring, reaction scaling, damping, friction
-               with its small-velocity cap, self-contact bond exclusion
+               with its small-velocity cap, self-contact bond exclusion,
+               pair activity on the previous step's boxes
   wall         segment_mass at d = 0 (hemisphere centroid 3R/8) and its
                closed form, force sign per plane, reaction = -sum V f
   Verlet       free flight, constant-force parabola, clamped nodes
@@ -162,6 +163,29 @@ def test_pair_contact_normal_and_reaction(oracle_lib):
     assert fA < 0
     assert st[1]["force"][0, 0] == pytest.approx(-fA / 5e-12 * 2e-12, rel=1e-14)
     assert np.all(st[0]["force"][0, 1:] == 0)
+
+
+def test_pair_activity_uses_the_previous_steps_boxes(oracle_lib):
+    """timeloop.h:389-393,779: the pair box test of a step uses the boxes
+    refreshed at the end of the previous step.  A node closing in fast enough
+    to cross into R_c during step 1 gets no contact force in step 1 (the boxes
+    from before it moved are half a step beyond R_c apart) and gets it in
+    step 2; a box test on the current positions would act in step 1."""
+    Rc, dt, V = 9e-5, 1e-8, 1000.0          # V dt = 1e-5
+    d0 = Rc + 0.5 * V * dt
+    _, _, s = two_nodes(d0, vA=(V, 0, 0))
+    e = oracle_lib.engine(s)
+    e.step(1)
+    st = [e.state(p) for p in range(2)]
+    d1 = st[1]["currpos"][0, 0] - st[0]["currpos"][0, 0]
+    assert d1 < Rc                           # within R_c already ...
+    assert np.all(st[0]["force"] == 0) and np.all(st[1]["force"] == 0)   # ... but not yet active
+    assert e.counters()["pair_computations"] == 0
+    e.step(1)
+    st = [e.state(p) for p in range(2)]
+    assert st[0]["force"][0, 0] < 0 and st[1]["force"][0, 0] > 0       # repulsion from step 2
+    assert e.counters()["pair_computations"] == 1
+    e.close()
 
 
 def test_pair_contact_outside_radius_and_tension(oracle_lib):
