@@ -259,8 +259,9 @@ int pd_autotune(pd_ctx* ctx, uint32_t reps, int* best, double* ms);
 #define PD_PROFILE_BOND 2
 #define PD_PHASE_POSITION 0 /* Verlet position half + boxes */
 #define PD_PHASE_CONTACT_LIST 1 /* pair box test + cell-list rebuild */
-#define PD_PHASE_BOND 2     /* bond force + fused break (the hot kernel) */
-#define PD_PHASE_CONTACT 3  /* self/wall/ext/pair contact */
+#define PD_PHASE_BOND 2     /* bond force + fused break + the node's contact,
+                               wall, external and torque terms (one kernel) */
+#define PD_PHASE_CONTACT 3  /* wall reaction sum (steps that report it) */
 #define PD_PHASE_UPDATE 4   /* Verlet velocity half + lagged boxes + walls */
 #define PD_NPHASE 5
 typedef struct pd_profile {

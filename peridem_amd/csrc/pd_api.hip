@@ -1125,6 +1125,7 @@ static Dev step_view(const pd_ctx* c) {
   Dev d = c->d;
   d.S = c->hwm_slices;
   d.N = d.S * kSlice;
+  d.contact = 1;
   uint32_t T = 1024;
   while (T < 2 * d.N && T < c->d.T) T <<= 1;
   d.T = T;
@@ -1194,13 +1195,14 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
     hipLaunchKernelGGL(k_build, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
   }
   if ((rc = mark(2))) return rc;
-  // grid padded to a multiple of the XCD count (the bond kernel remaps blocks)
+  // the bond kernel sums each node's row, then adds the node's contact,
+  // wall, external and torque terms; grid padded to a multiple of the XCD
+  // count (the bond kernel remaps blocks)
   hipLaunchKernelGGL(wide_pool(c) ? bond_wide().fn : bond_shapes()[effective_shape(c)].fn,
                      dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
   if ((rc = mark(3))) return rc;
-  hipLaunchKernelGGL(k_contact, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
-  if ((rc = mark(4))) return rc;
   if (any_reaction(c)) hipLaunchKernelGGL(k_reaction, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
+  if ((rc = mark(4))) return rc;
   hipLaunchKernelGGL(k_update, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if (any_vc(c)) {
     hipLaunchKernelGGL(k_pv_mean, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
@@ -1250,6 +1252,7 @@ static int autotune(pd_ctx* c, uint32_t reps) {
     t.F = F;
     t.bond = B;
     t.bloc = B16;
+    t.contact = 0;   // the contact terms are the same code in every shape and write state
     const dim3 grid((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds);
     int best = kDefaultBondShape;
     for (int v = 0; v < kBondShapes; v++) {

@@ -139,7 +139,7 @@ struct Dev {
   uint32_t* scan_tmp;
   uint64_t N, S, P, npairs, ninst;   // N = S*64 node slots; P, ninst incl. the void slot
   uint32_t T;        // hash table size (power of two)
-  uint32_t pad;
+  uint32_t contact;  // the bond kernel adds the contact terms (0: the force sums only)
 };
 
 // ---------------------------------------------------------------------------

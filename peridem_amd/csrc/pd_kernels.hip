@@ -8,8 +8,10 @@
 //   cell list    k_clear_cells -> k_cellkey -> scan -> k_scatter -> k_build
 //                (only in steps where some job rebuilds its candidate lists)
 //   bond kernel  PD bond force + fused irreversible break, per-node row sums
-//                (k_bond_lds: block tables staged in LDS; k_bond_t: unstaged)
-//   k_contact    self/wall/ext/pair contact added in the reference's order
+//                (k_bond_lds: block tables staged in LDS; k_bond_t: unstaged),
+//                then the node's self/wall/ext/torque/pair contact terms
+//                added in the reference's order (add_contact)
+//   k_reaction   wall reaction sum (jobs that report it, on its steps)
 //   k_update     Verlet velocity half, clamped nodes
 //   k_finalize   lagged boxes, wall motion
 // All floating point is FP64 with the reference's operation order; the build
@@ -401,6 +403,154 @@ __device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) 
 }
 
 // ---------------------------------------------------------------------------
+// Contact, wall and external force of node g, added to its PD sum f in the
+// reference's order (timeloop.h:544-651): self-contact (q ascending), wall
+// planes, external force, torque, then pair contact in pair order.
+//
+// The bond kernel calls it once the node's row is summed.  Everything it
+// reads is final before the bond kernel starts (positions, velocities,
+// candidate lists, lagged boxes, active pairs, centroids) except the bond
+// state, and of that it reads only node g's own row, which the same thread
+// has just updated -- the post-break row std::find sees (contact.h:783).
+//
+// The candidate list of g holds every partner within rb, same particle
+// first (q ascending), then the other particles by global node index (=
+// particle order, then node order); the exact reference tests pick the
+// pairs within R_c.
+__device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p, const DevPart& P,
+                                            const DevInst& I, double& f0, double& f1, double& f2) {
+  const double dt = I.dt;
+  double f[3] = {f0, f1, f2};
+  const bool selfc = self_active(P, I);
+  const bool pairc = (d.pactive[p] & 1u) != 0;
+  const bool wall = P.movable && P.stoppable && I.allow_wall && (!I.nl_bdry_only || d.brank[g] != kNoKey) &&
+                    !within_interior_fext(d.lrtb + 6 * p, I.wall, I.Rc);
+  const uint32_t n = (selfc || pairc) ? (uint32_t)d.ccnt[g] : 0u;
+  const uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
+  double4 xi = make_double4(0.0, 0.0, 0.0, 0.0), ui = xi;
+  double vi[3] = {0.0, 0.0, 0.0};
+  if (n || wall || I.enable_torque) {
+    xi = d.X[g];
+    ui = d.U[g];
+  }
+  if (n || wall) {
+    vi[0] = d.V[3 * g];
+    vi[1] = d.V[3 * g + 1];
+    vi[2] = d.V[3 * g + 2];
+  }
+  const double cpi[3] = {xi.x + ui.x, xi.y + ui.y, xi.z + ui.z};
+
+  // (b) self-contact, action only (contact.h:779-796 with is_self_contact=1),
+  // over the leading same-particle entries
+  uint32_t k = 0;
+  for (; k < n; k++) {
+    const uint32_t j = cl[(uint64_t)k * kSlice];
+    if (j & kBroken) break;   // the other particles' entries start here
+    if (!selfc) continue;
+    const double4 xj = d.X[j];
+    const double4 uj = d.U[j];
+    const double xx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+    const double dd[3] = {(uj.x - ui.x) + xx[0], (uj.y - ui.y) + xx[1], (uj.z - ui.z) + xx[2]};
+    const double dn = nrm3(dd[0], dd[1], dd[2]);
+    if (!(dn <= I.Rc)) continue;   // the reference's distance test
+    if (bonded_alive(d, g, j, P.rows_sorted)) continue;
+    const double xin = nrm3(xx[0], xx[1], xx[2]);
+    double len = I.Rc, K = I.Kn * I.Rc;
+    if (xin < I.Rc) {
+      len = xin;
+      K = P.cnot;
+    }
+    const double rv[3] = {vi[0] - d.V[3 * j], vi[1] - d.V[3 * j + 1], vi[2] - d.V[3 * j + 2]};
+    contact_terms(I, dt, kSelf, dd, dn, len, K, xj.w, xi.w, P.rho, rv, f);
+  }
+
+  // (c) wall contact (timeloop.h:561-570, contact.h:1060-1124)
+  if (wall) {
+    const uint32_t m = wall_node(I, dt, cpi, vi, xi.w, P.rho, f, nullptr);
+    if (reaction_step(d, I)) {
+      d.wmask[g] = (uint8_t)m;
+      if (m) d.bflag[g >> 8] = 1;
+    }
+  }
+
+  // (d) external force density (timeloop.h:581-597)
+  if (P.movable) {
+    if (I.gradient_extforce) {
+      // ramp min(t / extforce_maxstep, 1) (timeloop.h:582-587)
+      double sc = (double)I.t / (double)I.extforce_maxstep;
+      if (sc > 1) sc = 1;
+      f[0] += sc * d.E[3 * g];
+      f[1] += sc * d.E[3 * g + 1];
+      f[2] += sc * d.E[3 * g + 2];
+    } else {
+      f[0] += d.E[3 * g];
+      f[1] += d.E[3 * g + 1];
+      f[2] += d.E[3 * g + 2];
+    }
+  }
+
+  // (d') external torque about the centroid, every particle (timeloop.h:601-633)
+  if (I.enable_torque) {
+    const uint32_t ta = P.taxis;
+    double r[3] = {cpi[0] - d.cent[3 * p], cpi[1] - d.cent[3 * p + 1], cpi[2] - d.cent[3 * p + 2]};
+    const double rn = nrm3(r[0], r[1], r[2]);
+    const double rinv = rn != 0 ? 1 / rn : 0;
+    r[ta] = 0;
+    double rp[3];
+    rp[ta] = 0;
+    rp[(ta + 1) % 3] = -r[(ta + 2) % 3];
+    rp[(ta + 2) % 3] = r[(ta + 1) % 3];
+    const double pn = nrm3(rp[0], rp[1], rp[2]);
+    if (pn > 0) {
+      for (int c = 0; c < 3; c++) rp[c] = rp[c] / pn;
+    } else {
+      rp[(ta + 1) % 3] = 0;
+      rp[(ta + 2) % 3] = 0;
+    }
+    const double sc = rinv * P.tval;
+    for (int c = 0; c < 3; c++) f[c] += sc * rp[c];
+  }
+
+  // (e) inter-particle contact in pair order (timeloop.h:636-651) over the
+  // remaining entries
+  for (; pairc && k < n; k++) {
+    const uint32_t j = cl[(uint64_t)k * kSlice] & kIdxMask;
+    const uint32_t pj = d.pid[j];
+    const DevPart& Q = d.part[pj];
+    // pair (lo, hi) in the job's particle order, which is node order
+    const bool below = Q.node_off < P.node_off;
+    const uint32_t lo = below ? pj : p, hi = below ? p : pj;
+    if (!(d.part[lo].movable || d.part[hi].movable)) continue;
+    if (!intersects_box_fext(d.lrtb + 6 * lo, d.lrtb + 6 * hi, I.Rc)) continue;
+    const double4 xj = d.X[j];
+    const double4 uj = d.U[j];
+    double dd[3], rv[3];
+    if (below) {   // the partner is P_C's node i, we are P_N's node j
+      dd[0] = (ui.x - uj.x) + (xi.x - xj.x);
+      dd[1] = (ui.y - uj.y) + (xi.y - xj.y);
+      dd[2] = (ui.z - uj.z) + (xi.z - xj.z);
+    } else {       // we are P_C's node i
+      dd[0] = (uj.x - ui.x) + (xj.x - xi.x);
+      dd[1] = (uj.y - ui.y) + (xj.y - xi.y);
+      dd[2] = (uj.z - ui.z) + (xj.z - xi.z);
+    }
+    const double dn = nrm3(dd[0], dd[1], dd[2]);
+    if (!(dn <= I.Rc)) continue;   // the reference's distance test
+    const double vj[3] = {d.V[3 * j], d.V[3 * j + 1], d.V[3 * j + 2]};
+    if (below) {   // reaction terms
+      for (int c = 0; c < 3; c++) rv[c] = vj[c] - vi[c];
+      contact_terms(I, dt, kReaction, dd, dn, I.Rc, I.Kn * I.Rc, xi.w, xj.w, Q.rho, rv, f);
+    } else {
+      for (int c = 0; c < 3; c++) rv[c] = vi[c] - vj[c];
+      contact_terms(I, dt, kAction, dd, dn, I.Rc, I.Kn * I.Rc, xj.w, xi.w, P.rho, rv, f);
+    }
+  }
+  f0 = f[0];
+  f1 = f[1];
+  f2 = f[2];
+}
+
+// ---------------------------------------------------------------------------
 // Bond-based PD force + critical-stretch break (get_peridynamic_force,
 // particle2.h:107-193; remove_bonds, particle2.h:196-237).
 //
@@ -739,7 +889,8 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
   // padded node range: S slices of 64; whole wavefronts exit together
   if (t >= d.S * kSlice) return;
   const uint64_t g = (t & ~255ull) + d.lane_node[t];           // node
-  const DevPart& P = d.part[d.pid[g]];
+  const uint32_t p = d.pid[g];
+  const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
   if (!live(I)) return;            // frozen job or free slot: F keeps its value
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
@@ -759,6 +910,7 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
                        P.snot, f0, f1, f2);
   }
+  if (d.contact) add_contact(d, g, p, P, I, f0, f1, f2);
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
@@ -787,7 +939,8 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
   }
   if (t >= d.S * kSlice) return;
   const uint64_t g = (t & ~255ull) + d.lane_node[t];   // node
-  const DevPart& P = d.part[d.pid[g]];
+  const uint32_t p = d.pid[g];
+  const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
   if (!live(I)) return;
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
@@ -805,6 +958,7 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
     bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
                        P.snot, f0, f1, f2);
   }
+  if (d.contact) add_contact(d, g, p, P, I, f0, f1, f2);
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
@@ -833,171 +987,6 @@ __host__ inline const BondShape* bond_shapes() {
 // The default shape with 64-bit record addressing, for pools too large for
 // 32-bit offsets (never tuned).
 __host__ inline BondShape bond_wide() { return {k_bond_t<4, 2, false>, 4, 2, 0}; }
-
-// ---------------------------------------------------------------------------
-// Contact, wall and external force, added to the PD sum in the reference's
-// order (timeloop.h:544-651): self-contact (q ascending), wall planes,
-// external force, then pair contact in pair order.
-__global__ __launch_bounds__(256) void k_contact(Dev d) {
-  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= d.N) return;
-  const uint32_t p = d.pid[g];
-  const DevPart P = d.part[p];
-  const DevInst& I = d.inst[P.inst];
-  if (!live(I)) return;
-  const double dt = I.dt;
-  const double4 xi = d.X[g];
-  const double4 ui = d.U[g];
-  double f[3] = {d.F[3 * g], d.F[3 * g + 1], d.F[3 * g + 2]};
-
-  const bool selfc = self_active(P, I);
-  const bool pairc = (d.pactive[p] & 1u) != 0;
-  const double cpi[3] = {xi.x + ui.x, xi.y + ui.y, xi.z + ui.z};
-  const double vi[3] = {d.V[3 * g], d.V[3 * g + 1], d.V[3 * g + 2]};
-
-  // candidate partners within R_c, kept sorted by the reference's
-  // accumulation order: same particle first (q ascending), then other
-  // particles by global node index (= particle order, then node order).
-  uint32_t cand[kMaxCand];
-  uint32_t ncand = 0;
-  if (selfc || pairc) {
-    const uint32_t n = d.ccnt[g];
-    const uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
-    for (uint32_t k = 0; k < n; k++) {
-      const uint32_t keyj = cl[(uint64_t)k * kSlice];
-      const uint32_t j = keyj & kIdxMask;
-      const uint32_t pj = d.pid[j];
-      const DevPart& Q = d.part[pj];
-      if (!(keyj & kBroken)) {
-        if (!selfc) continue;
-      } else {
-        if (!pairc) continue;
-        // pair (lo, hi) in the job's particle order, which is node order
-        const bool below = Q.node_off < P.node_off;
-        const uint32_t lo = below ? pj : p, hi = below ? p : pj;
-        if (!(d.part[lo].movable || d.part[hi].movable)) continue;
-        if (!intersects_box_fext(d.lrtb + 6 * lo, d.lrtb + 6 * hi, I.Rc)) continue;
-      }
-      // exact reference distance test, in the reference orientation
-      const double4 xj = d.X[j];
-      const double4 uj = d.U[j];
-      double dd[3];
-      if (Q.node_off < P.node_off) {   // we are P_N's node j
-        dd[0] = (ui.x - uj.x) + (xi.x - xj.x);
-        dd[1] = (ui.y - uj.y) + (xi.y - xj.y);
-        dd[2] = (ui.z - uj.z) + (xi.z - xj.z);
-      } else {                         // we are P_C's node i
-        dd[0] = (uj.x - ui.x) + (xj.x - xi.x);
-        dd[1] = (uj.y - ui.y) + (xj.y - xi.y);
-        dd[2] = (uj.z - ui.z) + (xj.z - xi.z);
-      }
-      if (!(nrm3(dd[0], dd[1], dd[2]) <= I.Rc)) continue;
-      cand[ncand++] = keyj;   // the list is sorted, so is its subsequence
-    }
-  }
-
-  uint32_t ci = 0;
-  // (b) self-contact, action only (contact.h:779-796 with is_self_contact=1)
-  if (selfc) {
-    for (; ci < ncand && !(cand[ci] & kBroken); ci++) {
-      uint32_t j = cand[ci];
-      if (bonded_alive(d, g, j, P.rows_sorted)) continue;
-      const double4 xj = d.X[j];
-      const double4 uj = d.U[j];
-      double xx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
-      double dd[3] = {(uj.x - ui.x) + xx[0], (uj.y - ui.y) + xx[1], (uj.z - ui.z) + xx[2]};
-      double dn = nrm3(dd[0], dd[1], dd[2]);
-      double xin = nrm3(xx[0], xx[1], xx[2]);
-      double len = I.Rc, K = I.Kn * I.Rc;
-      if (xin < I.Rc) {
-        len = xin;
-        K = P.cnot;
-      }
-      double rv[3] = {vi[0] - d.V[3 * j], vi[1] - d.V[3 * j + 1], vi[2] - d.V[3 * j + 2]};
-      contact_terms(I, dt, kSelf, dd, dn, len, K, xj.w, xi.w, P.rho, rv, f);
-    }
-  }
-  while (ci < ncand && !(cand[ci] & kBroken)) ci++;
-
-  // (c) wall contact (timeloop.h:561-570, contact.h:1060-1124)
-  if (P.movable && P.stoppable && I.allow_wall && (!I.nl_bdry_only || d.brank[g] != kNoKey) &&
-      !within_interior_fext(d.lrtb + 6 * p, I.wall, I.Rc)) {
-    uint32_t m = wall_node(I, dt, cpi, vi, xi.w, P.rho, f, nullptr);
-    if (reaction_step(d, I)) {
-      d.wmask[g] = (uint8_t)m;
-      if (m) d.bflag[g >> 8] = 1;
-    }
-  }
-
-  // (d) external force density (timeloop.h:581-597)
-  if (P.movable) {
-    if (I.gradient_extforce) {
-      // ramp min(t / extforce_maxstep, 1) (timeloop.h:582-587)
-      double sc = (double)I.t / (double)I.extforce_maxstep;
-      if (sc > 1) sc = 1;
-      f[0] += sc * d.E[3 * g];
-      f[1] += sc * d.E[3 * g + 1];
-      f[2] += sc * d.E[3 * g + 2];
-    } else {
-      f[0] += d.E[3 * g];
-      f[1] += d.E[3 * g + 1];
-      f[2] += d.E[3 * g + 2];
-    }
-  }
-
-  // (d') external torque about the centroid, every particle (timeloop.h:601-633)
-  if (I.enable_torque) {
-    const uint32_t ta = P.taxis;
-    double r[3] = {cpi[0] - d.cent[3 * p], cpi[1] - d.cent[3 * p + 1], cpi[2] - d.cent[3 * p + 2]};
-    const double rn = nrm3(r[0], r[1], r[2]);
-    const double rinv = rn != 0 ? 1 / rn : 0;
-    r[ta] = 0;
-    double rp[3];
-    rp[ta] = 0;
-    rp[(ta + 1) % 3] = -r[(ta + 2) % 3];
-    rp[(ta + 2) % 3] = r[(ta + 1) % 3];
-    const double pn = nrm3(rp[0], rp[1], rp[2]);
-    if (pn > 0) {
-      for (int k = 0; k < 3; k++) rp[k] = rp[k] / pn;
-    } else {
-      rp[(ta + 1) % 3] = 0;
-      rp[(ta + 2) % 3] = 0;
-    }
-    const double sc = rinv * P.tval;
-    for (int k = 0; k < 3; k++) f[k] += sc * rp[k];
-  }
-
-  // (e) inter-particle contact in pair order (timeloop.h:636-651)
-  for (; ci < ncand; ci++) {
-    uint32_t j = cand[ci] & kIdxMask;
-    uint32_t pj = d.pid[j];
-    const DevPart& Q = d.part[pj];
-    const double4 xj = d.X[j];
-    const double4 uj = d.U[j];
-    const double vj[3] = {d.V[3 * j], d.V[3 * j + 1], d.V[3 * j + 2]};
-    double dd[3], rv[3];
-    if (Q.node_off < P.node_off) {
-      // partner is P_C (node i), we are P_N (node j): reaction terms
-      for (int k = 0; k < 3; k++) rv[k] = vj[k] - vi[k];
-      dd[0] = (ui.x - uj.x) + (xi.x - xj.x);
-      dd[1] = (ui.y - uj.y) + (xi.y - xj.y);
-      dd[2] = (ui.z - uj.z) + (xi.z - xj.z);
-      double dn = nrm3(dd[0], dd[1], dd[2]);
-      contact_terms(I, dt, kReaction, dd, dn, I.Rc, I.Kn * I.Rc, xi.w, xj.w, Q.rho, rv, f);
-    } else {
-      for (int k = 0; k < 3; k++) rv[k] = vi[k] - vj[k];
-      dd[0] = (uj.x - ui.x) + (xj.x - xi.x);
-      dd[1] = (uj.y - ui.y) + (xj.y - xi.y);
-      dd[2] = (uj.z - ui.z) + (xj.z - xi.z);
-      double dn = nrm3(dd[0], dd[1], dd[2]);
-      contact_terms(I, dt, kAction, dd, dn, I.Rc, I.Kn * I.Rc, xj.w, xi.w, P.rho, rv, f);
-    }
-  }
-
-  d.F[3 * g] = f[0];
-  d.F[3 * g + 1] = f[1];
-  d.F[3 * g + 2] = f[2];
-}
 
 // ---------------------------------------------------------------------------
 // Verlet velocity half (timeloop.h:662-682) and clamped nodes (:767-774).
