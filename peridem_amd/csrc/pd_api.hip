@@ -1181,7 +1181,7 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
   if ((rc = mark(0))) return rc;
   hipLaunchKernelGGL(k_position, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if ((rc = mark(1))) return rc;
-  hipLaunchKernelGGL(k_decide_pairs, dim3(grid_of(std::max<uint64_t>(d.ninst, d.npairs), 256)), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_decide_pairs, dim3(grid_of(std::max({d.ninst, d.npairs, d.P}), 256)), dim3(256), 0, st, d);
   if (any_torque(c)) hipLaunchKernelGGL(k_centroid, dim3(grid_of(d.P, 64)), dim3(64), 0, st, d);
   if (any_contact_possible(c)) {
     // every kernel of the list rebuild exits at once unless a job rebuilds

@@ -194,11 +194,22 @@ __device__ __forceinline__ void pair_test(Dev& d, uint64_t q) {
   atomicOr(&d.pactive[j], 2u | (Pj.stoppable ? 1u : 0u));
 }
 
-// The two are independent; one launch does both (thread q: job q, pair q).
+// Wall prefilter of particle q on its lagged box (timeloop.h:561-570): bit 2
+// of pactive marks the particles whose nodes test the wall planes this step.
+__device__ __forceinline__ void wall_test(Dev& d, uint64_t q) {
+  const DevPart& P = d.part[q];
+  const DevInst& I = d.inst[P.inst];
+  if (!live(I) || !(P.movable && P.stoppable && I.allow_wall)) return;
+  if (!within_interior_fext(d.lrtb + 6 * q, I.wall, I.Rc)) atomicOr(&d.pactive[q], 4u);
+}
+
+// The three are independent; one launch does all (thread q: job q, pair q,
+// particle q).
 __global__ __launch_bounds__(256) void k_decide_pairs(Dev d) {
   const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < d.ninst) decide_job(d, q);
   if (q < d.npairs) pair_test(d, q);
+  if (q < d.P) wall_test(d, q);
 }
 
 __device__ __forceinline__ bool self_active(const DevPart& P, const DevInst& I) {
@@ -417,14 +428,35 @@ __device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) 
 // first (q ascending), then the other particles by global node index (=
 // particle order, then node order); the exact reference tests pick the
 // pairs within R_c.
+//
+// The node's external force and its particle's contact flags are loaded
+// before the row (ContactIn), so their latency hides behind the bond sum.
+struct ContactIn {
+  double e0, e1, e2;   // external force density
+  uint32_t pa;         // pactive of the particle
+};
+
+__device__ __forceinline__ ContactIn contact_in(const Dev& d, uint64_t g, uint32_t p, const DevPart& P) {
+  ContactIn c{0.0, 0.0, 0.0, 0u};
+  if (d.contact) {
+    c.pa = d.pactive[p];
+    if (P.movable) {
+      c.e0 = d.E[3 * g];
+      c.e1 = d.E[3 * g + 1];
+      c.e2 = d.E[3 * g + 2];
+    }
+  }
+  return c;
+}
+
 __device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p, const DevPart& P,
-                                            const DevInst& I, double& f0, double& f1, double& f2) {
+                                            const DevInst& I, const ContactIn& in, double& f0, double& f1,
+                                            double& f2) {
   const double dt = I.dt;
   double f[3] = {f0, f1, f2};
   const bool selfc = self_active(P, I);
-  const bool pairc = (d.pactive[p] & 1u) != 0;
-  const bool wall = P.movable && P.stoppable && I.allow_wall && (!I.nl_bdry_only || d.brank[g] != kNoKey) &&
-                    !within_interior_fext(d.lrtb + 6 * p, I.wall, I.Rc);
+  const bool pairc = (in.pa & 1u) != 0;
+  const bool wall = (in.pa & 4u) && (!I.nl_bdry_only || d.brank[g] != kNoKey);
   const uint32_t n = (selfc || pairc) ? (uint32_t)d.ccnt[g] : 0u;
   const uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
   double4 xi = make_double4(0.0, 0.0, 0.0, 0.0), ui = xi;
@@ -479,13 +511,13 @@ __device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p
       // ramp min(t / extforce_maxstep, 1) (timeloop.h:582-587)
       double sc = (double)I.t / (double)I.extforce_maxstep;
       if (sc > 1) sc = 1;
-      f[0] += sc * d.E[3 * g];
-      f[1] += sc * d.E[3 * g + 1];
-      f[2] += sc * d.E[3 * g + 2];
+      f[0] += sc * in.e0;
+      f[1] += sc * in.e1;
+      f[2] += sc * in.e2;
     } else {
-      f[0] += d.E[3 * g];
-      f[1] += d.E[3 * g + 1];
-      f[2] += d.E[3 * g + 2];
+      f[0] += in.e0;
+      f[1] += in.e1;
+      f[2] += in.e2;
     }
   }
 
@@ -894,6 +926,7 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
   const DevInst& I = d.inst[P.inst];
   if (!live(I)) return;            // frozen job or free slot: F keeps its value
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  const ContactIn cin = contact_in(d, g, p, P);
   if (P.movable) {
     uint64_t base;
     uint32_t W;
@@ -910,7 +943,7 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
                        P.snot, f0, f1, f2);
   }
-  if (d.contact) add_contact(d, g, p, P, I, f0, f1, f2);
+  if (d.contact) add_contact(d, g, p, P, I, cin, f0, f1, f2);
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
@@ -944,6 +977,7 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
   const DevInst& I = d.inst[P.inst];
   if (!live(I)) return;
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  const ContactIn cin = contact_in(d, g, p, P);
   if (P.movable) {
     uint64_t base;
     uint32_t W;
@@ -958,7 +992,7 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
     bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
                        P.snot, f0, f1, f2);
   }
-  if (d.contact) add_contact(d, g, p, P, I, f0, f1, f2);
+  if (d.contact) add_contact(d, g, p, P, I, cin, f0, f1, f2);
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
