@@ -8,15 +8,16 @@ seeded rigid velocity (+-0.5 m/s per axis) plus a seeded per-node velocity
 (+-0.01 m/s): without it every node of a sphere shares one displacement and
 every bond has exactly zero stretch, which is not a representative load.  One "step" is one full
 explicit step of the library (Verlet position half, contact list, bond force
-+ break + contact + wall + gravity, Verlet velocity half).
++ break + contact + wall + gravity + Verlet velocity half in one kernel).
 
 value: bond-updates/s over the timed steps (directed bonds of movable
 particles per step, broken ones included) with all state resident in HBM.
 roofline: the bond-force kernel's algorithmic bytes per launch
-(5 B per directed bond + 108 B per node: positions and volume 32, displacement
-24, force 24, particle id 4, external force 24 -- the kernel adds each node's
-contact, wall, external and torque terms after its bond sum) over its
-average duration, measured
+(5 B per directed bond + 205 B per node: positions and volume 32, displacement
+24, force 24, particle id 4, external force 24, acceleration and velocity
+read and written 96, clamp flag 1 -- after its bond sum the kernel adds each
+node's contact, wall, external and torque terms and takes the Verlet
+velocity half) over its average duration, measured
 with HIP events recorded on the library's stream around the bond kernel in
 the timed steps themselves (profiled steps launch one by one; the same steps
 replayed from graphs without marks are timed beside them and logged).  A
@@ -307,7 +308,7 @@ def main():
     log(f"[bench] {args.steps} steps: {elapsed * 1e3:.2f} ms wall, {dev_ms:.2f} ms device "
         f"(unmarked graph replay of as many steps: {graph_s * 1e3:.2f} ms wall)")
     force_ms = prof["bond"]["ms"] / max(prof["bond"]["launches"], 1)
-    force_bytes = 5 * Bm + 108 * N
+    force_bytes = 5 * Bm + 205 * N
     achieved = force_bytes / (force_ms * 1e-3) / 1e9
     step_ms = elapsed * 1e3 / args.steps
     step_bytes = 5 * Bm + 252 * N

@@ -260,9 +260,11 @@ int pd_autotune(pd_ctx* ctx, uint32_t reps, int* best, double* ms);
 #define PD_PHASE_POSITION 0 /* Verlet position half + boxes */
 #define PD_PHASE_CONTACT_LIST 1 /* pair box test + cell-list rebuild */
 #define PD_PHASE_BOND 2     /* bond force + fused break + the node's contact,
-                               wall, external and torque terms (one kernel) */
+                               wall, external and torque terms + Verlet
+                               velocity half (one kernel) */
 #define PD_PHASE_CONTACT 3  /* wall reaction sum (steps that report it) */
-#define PD_PHASE_UPDATE 4   /* Verlet velocity half + lagged boxes + walls */
+#define PD_PHASE_UPDATE 4   /* clamped displacements, prescribed velocity,
+                               lagged boxes, walls */
 #define PD_NPHASE 5
 typedef struct pd_profile {
   double ms[PD_NPHASE];

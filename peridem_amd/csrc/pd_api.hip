@@ -41,11 +41,11 @@ constexpr double kSkinFrac = 0.3;             // candidate-list skin / R_c
 constexpr int kGraphSteps = 8;  // steps per captured graph
 constexpr int kProfRing = 16;   // event sets of profiled steps in flight
 constexpr int kProfLag = 8;     // a profiled step's marks are read this many steps later
-// device bytes per node slot (X, U, V, A, F, E, pid, nflag, deg, wmask, ckey,
+// device bytes per node slot (X, U, V, Vold, A, F, E, pid, nflag, deg, wmask, ckey,
 // csorted, ~2 hash buckets) and per bond entry
 // node arrays + the bond kernel's record copies + block tables (kLdsCap per
 // 256 nodes); per entry the u32 and u16 forms
-constexpr uint64_t kBytesPerNode = 32 + 32 + 4 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16 + 56 + 36 + 2;
+constexpr uint64_t kBytesPerNode = 32 + 32 + 5 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16 + 56 + 36 + 2;
 constexpr uint64_t kBytesPerEntry = 6;
 
 struct HostPart {
@@ -377,6 +377,14 @@ static bool any_vc(pd_ctx* c) {
   return false;
 }
 
+static bool any_clamped(pd_ctx* c) {
+  for (auto& J : c->jobs)
+    if (J.used)
+      for (auto& H : J.parts)
+        if (H.movable && !H.clamped.empty()) return true;
+  return false;
+}
+
 static bool any_reaction(pd_ctx* c) {
   for (auto& J : c->jobs)
     if (J.used && J.run.compute_wall_reaction && J.run.allow_wall) return true;
@@ -536,6 +544,7 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(lane_node, N);
   ALLOC(node_lane, N);
   ALLOC(V, 3 * N);
+  ALLOC(Vold, 3 * N);
   ALLOC(A, 3 * N);
   ALLOC(F, 3 * N);
   ALLOC(E, 3 * N);
@@ -1125,7 +1134,7 @@ static Dev step_view(const pd_ctx* c) {
   Dev d = c->d;
   d.S = c->hwm_slices;
   d.N = d.S * kSlice;
-  d.contact = 1;
+  d.finish = 1;
   uint32_t T = 1024;
   while (T < 2 * d.N && T < c->d.T) T <<= 1;
   d.T = T;
@@ -1196,14 +1205,15 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
   }
   if ((rc = mark(2))) return rc;
   // the bond kernel sums each node's row, then adds the node's contact,
-  // wall, external and torque terms; grid padded to a multiple of the XCD
-  // count (the bond kernel remaps blocks)
+  // wall, external and torque terms and takes the Verlet velocity half;
+  // grid padded to a multiple of the XCD count (the bond kernel remaps
+  // blocks)
   hipLaunchKernelGGL(wide_pool(c) ? bond_wide().fn : bond_shapes()[effective_shape(c)].fn,
                      dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
   if ((rc = mark(3))) return rc;
   if (any_reaction(c)) hipLaunchKernelGGL(k_reaction, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
   if ((rc = mark(4))) return rc;
-  hipLaunchKernelGGL(k_update, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
+  if (any_clamped(c)) hipLaunchKernelGGL(k_clamp, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if (any_vc(c)) {
     hipLaunchKernelGGL(k_pv_mean, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
     hipLaunchKernelGGL(k_pv_apply, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
@@ -1252,7 +1262,7 @@ static int autotune(pd_ctx* c, uint32_t reps) {
     t.F = F;
     t.bond = B;
     t.bloc = B16;
-    t.contact = 0;   // the contact terms are the same code in every shape and write state
+    t.finish = 0;   // the contact terms and velocity half are the same code in every shape and write state
     const dim3 grid((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds);
     int best = kDefaultBondShape;
     for (int v = 0; v < kBondShapes; v++) {

@@ -96,6 +96,8 @@ struct Dev {
   uint8_t* lane_node;
   uint8_t* node_lane;
   double* V;         // vel [3N]
+  double* Vold;      // vel [3N] as the step's force phase reads it: copied by the
+                     // position kernel, since the bond kernel writes V node by node
   double* A;         // acc [3N]
   double* F;         // force density [3N]
   double* AFD;       // applied_force_density of prescribed-velocity particles [3N]
@@ -139,7 +141,8 @@ struct Dev {
   uint32_t* scan_tmp;
   uint64_t N, S, P, npairs, ninst;   // N = S*64 node slots; P, ninst incl. the void slot
   uint32_t T;        // hash table size (power of two)
-  uint32_t contact;  // the bond kernel adds the contact terms (0: the force sums only)
+  uint32_t finish;   // the bond kernel finishes each node's step: contact terms and
+                     // the velocity half (0: the force sums only)
 };
 
 // ---------------------------------------------------------------------------

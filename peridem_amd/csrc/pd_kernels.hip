@@ -11,8 +11,9 @@
 //                (k_bond_lds: block tables staged in LDS; k_bond_t: unstaged),
 //                then the node's self/wall/ext/torque/pair contact terms
 //                added in the reference's order (add_contact)
+//                and the Verlet velocity half (velocity_half)
 //   k_reaction   wall reaction sum (jobs that report it, on its steps)
-//   k_update     Verlet velocity half, clamped nodes
+//   k_clamp      clamped nodes' displacement (pools with clamped nodes)
 //   k_finalize   lagged boxes, wall motion
 // All floating point is FP64 with the reference's operation order; the build
 // uses -ffp-contract=off so no multiply-add is fused.
@@ -66,6 +67,9 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
     const double h = (dt * dt) * 0.5;
     double v0 = d.V[3 * g], v1 = d.V[3 * g + 1], v2 = d.V[3 * g + 2];
     double a0 = d.A[3 * g], a1 = d.A[3 * g + 1], a2 = d.A[3 * g + 2];
+    d.Vold[3 * g] = v0;
+    d.Vold[3 * g + 1] = v1;
+    d.Vold[3 * g + 2] = v2;
     u.x = u.x + ((dt * v0) + (h * a0));
     u.y = u.y + ((dt * v1) + (h * a1));
     u.z = u.z + ((dt * v2) + (h * a2));
@@ -438,7 +442,7 @@ struct ContactIn {
 
 __device__ __forceinline__ ContactIn contact_in(const Dev& d, uint64_t g, uint32_t p, const DevPart& P) {
   ContactIn c{0.0, 0.0, 0.0, 0u};
-  if (d.contact) {
+  if (d.finish) {
     c.pa = d.pactive[p];
     if (P.movable) {
       c.e0 = d.E[3 * g];
@@ -466,9 +470,9 @@ __device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p
     ui = d.U[g];
   }
   if (n || wall) {
-    vi[0] = d.V[3 * g];
-    vi[1] = d.V[3 * g + 1];
-    vi[2] = d.V[3 * g + 2];
+    vi[0] = d.Vold[3 * g];
+    vi[1] = d.Vold[3 * g + 1];
+    vi[2] = d.Vold[3 * g + 2];
   }
   const double cpi[3] = {xi.x + ui.x, xi.y + ui.y, xi.z + ui.z};
 
@@ -492,7 +496,7 @@ __device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p
       len = xin;
       K = P.cnot;
     }
-    const double rv[3] = {vi[0] - d.V[3 * j], vi[1] - d.V[3 * j + 1], vi[2] - d.V[3 * j + 2]};
+    const double rv[3] = {vi[0] - d.Vold[3 * j], vi[1] - d.Vold[3 * j + 1], vi[2] - d.Vold[3 * j + 2]};
     contact_terms(I, dt, kSelf, dd, dn, len, K, xj.w, xi.w, P.rho, rv, f);
   }
 
@@ -568,7 +572,7 @@ __device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p
     }
     const double dn = nrm3(dd[0], dd[1], dd[2]);
     if (!(dn <= I.Rc)) continue;   // the reference's distance test
-    const double vj[3] = {d.V[3 * j], d.V[3 * j + 1], d.V[3 * j + 2]};
+    const double vj[3] = {d.Vold[3 * j], d.Vold[3 * j + 1], d.Vold[3 * j + 2]};
     if (below) {   // reaction terms
       for (int c = 0; c < 3; c++) rv[c] = vj[c] - vi[c];
       contact_terms(I, dt, kReaction, dd, dn, I.Rc, I.Kn * I.Rc, xi.w, xj.w, Q.rho, rv, f);
@@ -580,6 +584,32 @@ __device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p
   f0 = f[0];
   f1 = f[1];
   f2 = f[2];
+}
+
+// Verlet velocity half (timeloop.h:662-682) of node g from its final force,
+// after its contact terms; V and A of g are read and written by this thread
+// only (the contact terms of every node read Vold).  A clamped node's
+// velocity and acceleration are zeroed here and its displacement by k_clamp
+// after the launch, since other nodes' contact terms still read U
+// (timeloop.h:767-774).  The prescribed-velocity particle is updated in full
+// and clamped after its prescription (k_pv_apply).
+__device__ __forceinline__ void velocity_half(const Dev& d, uint64_t g, uint32_t p, const DevPart& P,
+                                              const DevInst& I, double f0, double f1, double f2) {
+  if (!P.movable) return;
+  const bool vc = I.vc_on && p == I.vc_slot;
+  if ((d.nflag[g] & 1u) && !vc) {
+    for (int k = 0; k < 3; k++) d.V[3 * g + k] = d.A[3 * g + k] = 0.0;
+    return;
+  }
+  const double hdt = 0.5 * I.dt;
+  const double f[3] = {f0, f1, f2};
+  for (int k = 0; k < 3; k++) {
+    const double aold = d.A[3 * g + k];
+    const double anew = P.rinv * f[k];
+    if (vc) d.AFD[3 * g + k] = aold;
+    d.A[3 * g + k] = anew;
+    d.V[3 * g + k] = d.V[3 * g + k] + ((aold + anew) * hdt);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -943,7 +973,10 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
                        P.snot, f0, f1, f2);
   }
-  if (d.contact) add_contact(d, g, p, P, I, cin, f0, f1, f2);
+  if (d.finish) {
+    add_contact(d, g, p, P, I, cin, f0, f1, f2);
+    velocity_half(d, g, p, P, I, f0, f1, f2);
+  }
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
@@ -992,7 +1025,10 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
     bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
                        P.snot, f0, f1, f2);
   }
-  if (d.contact) add_contact(d, g, p, P, I, cin, f0, f1, f2);
+  if (d.finish) {
+    add_contact(d, g, p, P, I, cin, f0, f1, f2);
+    velocity_half(d, g, p, P, I, f0, f1, f2);
+  }
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
@@ -1023,29 +1059,17 @@ __host__ inline const BondShape* bond_shapes() {
 __host__ inline BondShape bond_wide() { return {k_bond_t<4, 2, false>, 4, 2, 0}; }
 
 // ---------------------------------------------------------------------------
-// Verlet velocity half (timeloop.h:662-682) and clamped nodes (:767-774).
-__global__ __launch_bounds__(256) void k_update(Dev d) {
+// Displacement of the clamped nodes (timeloop.h:767-774), after the bond
+// kernel has zeroed their velocity and acceleration.  Launched only for pools
+// with clamped nodes.
+__global__ __launch_bounds__(256) void k_clamp(Dev d) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= d.N) return;
-  const DevPart& P = d.part[d.pid[g]];
+  if (g >= d.N || !(d.nflag[g] & 1u)) return;
+  const uint32_t p = d.pid[g];
+  const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
-  if (!live(I) || !P.movable) return;
-  const double hdt = 0.5 * I.dt;
-  // the prescribed-velocity particle is updated in full here and clamped
-  // after its prescription (k_pv_apply)
-  const bool vc = I.vc_on && d.pid[g] == I.vc_slot;
-  if ((d.nflag[g] & 1u) && !vc) {
-    d.U[g] = make_double4(0.0, 0.0, 0.0, 0.0);
-    for (int k = 0; k < 3; k++) d.V[3 * g + k] = d.A[3 * g + k] = 0.0;
-    return;
-  }
-  for (int k = 0; k < 3; k++) {
-    double aold = d.A[3 * g + k];
-    double anew = P.rinv * d.F[3 * g + k];
-    if (vc) d.AFD[3 * g + k] = aold;
-    d.A[3 * g + k] = anew;
-    d.V[3 * g + k] = d.V[3 * g + k] + ((aold + anew) * hdt);
-  }
+  if (!live(I) || !P.movable || (I.vc_on && p == I.vc_slot)) return;
+  d.U[g] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
 // Prescribed velocity (timeloop.h:685-764), one thread per job: the mean
@@ -1163,7 +1187,7 @@ __global__ void k_reaction(Dev d) {
         const double4 x = d.X[h];
         const double4 u = d.U[h];
         const double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
-        const double v[3] = {d.V[3 * h], d.V[3 * h + 1], d.V[3 * h + 2]};
+        const double v[3] = {d.Vold[3 * h], d.Vold[3 * h + 1], d.Vold[3 * h + 2]};
         double fdummy[3] = {0, 0, 0};
         wall_node(I, I.dt, cp, v, x.w, P.rho, fdummy, reac);
         d.wmask[h] = 0;
@@ -1186,7 +1210,7 @@ __global__ void k_reaction(Dev d) {
       double4 x = d.X[g];
       double4 u = d.U[g];
       double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
-      double v[3] = {d.V[3 * g], d.V[3 * g + 1], d.V[3 * g + 2]};
+      double v[3] = {d.Vold[3 * g], d.Vold[3 * g + 1], d.Vold[3 * g + 2]};
       double fdummy[3] = {0, 0, 0};
       wall_node(I, I.dt, cp, v, x.w, P.rho, fdummy, reac);
       d.wmask[g] = 0;
