@@ -612,6 +612,40 @@ __device__ __forceinline__ void velocity_half(const Dev& d, uint64_t g, uint32_t
   }
 }
 
+// The end of node g's step in the bond kernel: contact terms, then the
+// velocity half.  In a wavefront over one particle (all but the few that
+// straddle two) the particle's and job's records are wave-uniform and are
+// read through the scalar cache -- they are not written during the launch --
+// instead of as a chain of per-lane loads.
+template <class T>
+using kptr = const T __attribute__((address_space(4)))*;
+
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T* base, uint32_t i) {
+  static_assert(sizeof(T) % 4 == 0, "records of whole words");
+  T v;
+  uint32_t* w = reinterpret_cast<uint32_t*>(&v);
+  const kptr<uint32_t> s = (kptr<uint32_t>)((kptr<T>)base + i);
+#pragma unroll
+  for (unsigned k = 0; k < sizeof(T) / 4; k++) w[k] = s[k];
+  return v;
+}
+
+__device__ __forceinline__ void finish_node(const Dev& d, uint64_t g, uint32_t p, const DevPart& P,
+                                            const DevInst& I, const ContactIn& cin, double& f0, double& f1,
+                                            double& f2) {
+  const uint32_t pu = __builtin_amdgcn_readfirstlane(p);
+  if (__builtin_amdgcn_ballot_w64(p != pu) == 0) {
+    const DevPart Pu = ld_uniform(d.part, pu);
+    const DevInst Iu = ld_uniform(d.inst, Pu.inst);
+    add_contact(d, g, pu, Pu, Iu, cin, f0, f1, f2);
+    velocity_half(d, g, pu, Pu, Iu, f0, f1, f2);
+  } else {
+    add_contact(d, g, p, P, I, cin, f0, f1, f2);
+    velocity_half(d, g, p, P, I, f0, f1, f2);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Bond-based PD force + critical-stretch break (get_peridynamic_force,
 // particle2.h:107-193; remove_bonds, particle2.h:196-237).
@@ -973,10 +1007,7 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
                        P.snot, f0, f1, f2);
   }
-  if (d.finish) {
-    add_contact(d, g, p, P, I, cin, f0, f1, f2);
-    velocity_half(d, g, p, P, I, f0, f1, f2);
-  }
+  if (d.finish) finish_node(d, g, p, P, I, cin, f0, f1, f2);
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
@@ -1025,10 +1056,7 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
     bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
                        P.snot, f0, f1, f2);
   }
-  if (d.finish) {
-    add_contact(d, g, p, P, I, cin, f0, f1, f2);
-    velocity_half(d, g, p, P, I, f0, f1, f2);
-  }
+  if (d.finish) finish_node(d, g, p, P, I, cin, f0, f1, f2);
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
