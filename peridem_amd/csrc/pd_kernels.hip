@@ -440,11 +440,11 @@ struct ContactIn {
   uint32_t pa;         // pactive of the particle
 };
 
-__device__ __forceinline__ ContactIn contact_in(const Dev& d, uint64_t g, uint32_t p, const DevPart& P) {
+__device__ __forceinline__ ContactIn contact_in(const Dev& d, uint64_t g, uint32_t p, bool movable) {
   ContactIn c{0.0, 0.0, 0.0, 0u};
   if (d.finish) {
     c.pa = d.pactive[p];
-    if (P.movable) {
+    if (movable) {
       c.e0 = d.E[3 * g];
       c.e1 = d.E[3 * g + 1];
       c.e2 = d.E[3 * g + 2];
@@ -631,9 +631,8 @@ __device__ __forceinline__ T ld_uniform(const T* base, uint32_t i) {
   return v;
 }
 
-__device__ __forceinline__ void finish_node(const Dev& d, uint64_t g, uint32_t p, const DevPart& P,
-                                            const DevInst& I, const ContactIn& cin, double& f0, double& f1,
-                                            double& f2) {
+__device__ __forceinline__ void finish_node(const Dev& d, uint64_t g, uint32_t p, const ContactIn& cin,
+                                            double& f0, double& f1, double& f2) {
   const uint32_t pu = __builtin_amdgcn_readfirstlane(p);
   if (__builtin_amdgcn_ballot_w64(p != pu) == 0) {
     const DevPart Pu = ld_uniform(d.part, pu);
@@ -641,9 +640,30 @@ __device__ __forceinline__ void finish_node(const Dev& d, uint64_t g, uint32_t p
     add_contact(d, g, pu, Pu, Iu, cin, f0, f1, f2);
     velocity_half(d, g, pu, Pu, Iu, f0, f1, f2);
   } else {
+    const DevPart& P = d.part[p];
+    const DevInst& I = d.inst[P.inst];
     add_contact(d, g, p, P, I, cin, f0, f1, f2);
     velocity_half(d, g, p, P, I, f0, f1, f2);
   }
+}
+
+// What the row loop needs of the node's particle and job, through the
+// scalar cache in single-particle wavefronts.
+struct NodeHead {
+  bool live, movable, frac;   // frac: breaks enabled for this particle
+  double cnot, snot;
+};
+
+__device__ __forceinline__ NodeHead node_head(const Dev& d, uint32_t p) {
+  const uint32_t pu = __builtin_amdgcn_readfirstlane(p);
+  if (__builtin_amdgcn_ballot_w64(p != pu) == 0) {
+    const DevPart P = ld_uniform(d.part, pu);
+    const DevInst I = ld_uniform(d.inst, P.inst);
+    return {live(I), P.movable != 0, I.enable_fracture && P.breakable, P.cnot, P.snot};
+  }
+  const DevPart& P = d.part[p];
+  const DevInst& I = d.inst[P.inst];
+  return {live(I), P.movable != 0, I.enable_fracture && P.breakable, P.cnot, P.snot};
 }
 
 // ---------------------------------------------------------------------------
@@ -986,12 +1006,11 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
   if (t >= d.S * kSlice) return;
   const uint64_t g = (t & ~255ull) + d.lane_node[t];           // node
   const uint32_t p = d.pid[g];
-  const DevPart& P = d.part[p];
-  const DevInst& I = d.inst[P.inst];
-  if (!live(I)) return;            // frozen job or free slot: F keeps its value
+  const NodeHead H = node_head(d, p);
+  if (!H.live) return;             // frozen job or free slot: F keeps its value
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  const ContactIn cin = contact_in(d, g, p, P);
-  if (P.movable) {
+  const ContactIn cin = contact_in(d, g, p, H.movable);
+  if (H.movable) {
     uint64_t base;
     uint32_t W;
     slice_of(d, t, base, W);
@@ -1004,10 +1023,9 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     // each node's data is fetched from HBM once per launch
     double4 xi, ui;
     rec<false>(d, (uint32_t)g, xi, ui);
-    bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
-                       P.snot, f0, f1, f2);
+    bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, H.frac, H.cnot, H.snot, f0, f1, f2);
   }
-  if (d.finish) finish_node(d, g, p, P, I, cin, f0, f1, f2);
+  if (d.finish) finish_node(d, g, p, cin, f0, f1, f2);
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
@@ -1037,12 +1055,11 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
   if (t >= d.S * kSlice) return;
   const uint64_t g = (t & ~255ull) + d.lane_node[t];   // node
   const uint32_t p = d.pid[g];
-  const DevPart& P = d.part[p];
-  const DevInst& I = d.inst[P.inst];
-  if (!live(I)) return;
+  const NodeHead H = node_head(d, p);
+  if (!H.live) return;
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  const ContactIn cin = contact_in(d, g, p, P);
-  if (P.movable) {
+  const ContactIn cin = contact_in(d, g, p, H.movable);
+  if (H.movable) {
     uint64_t base;
     uint32_t W;
     slice_of(d, t, base, W);
@@ -1053,10 +1070,9 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
     const double4 xi = make_double4(xy.x, xy.y, zv.x, zv.y);
     double4 ui;
     rec_u<true>(d, (uint32_t)g, ui);
-    bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, I.enable_fracture && P.breakable, P.cnot,
-                       P.snot, f0, f1, f2);
+    bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, H.frac, H.cnot, H.snot, f0, f1, f2);
   }
-  if (d.finish) finish_node(d, g, p, P, I, cin, f0, f1, f2);
+  if (d.finish) finish_node(d, g, p, cin, f0, f1, f2);
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
