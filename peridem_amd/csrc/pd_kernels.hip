@@ -1044,11 +1044,31 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
     uint32_t cnt = 0;
     if (blk < d.S / kBlockSlices) cnt = min(d.binfo[blk].x, kLdsCap);
     const uint32_t* tab = d.btab + blk * (uint64_t)kLdsCap;
-    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-      const uint32_t id = tab[i];
-      s_id[i] = id << 3;
-      s_xy[i] = d.XY[id];
-      s_zv[i] = d.ZV[id];
+    // kStage table entries per thread in flight: their index loads go out
+    // together, then their gathers (a clamped index rereads entry i0)
+    constexpr uint32_t kStage = 4;
+    for (uint32_t i0 = threadIdx.x; i0 < cnt; i0 += kStage * blockDim.x) {
+      uint32_t id[kStage];
+      double2 xy[kStage], zv[kStage];
+#pragma unroll
+      for (uint32_t k = 0; k < kStage; k++) {
+        const uint32_t i = i0 + k * blockDim.x;
+        id[k] = tab[i < cnt ? i : i0];
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kStage; k++) {
+        xy[k] = d.XY[id[k]];
+        zv[k] = d.ZV[id[k]];
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kStage; k++) {
+        const uint32_t i = i0 + k * blockDim.x;
+        if (i < cnt) {
+          s_id[i] = id[k] << 3;
+          s_xy[i] = xy[k];
+          s_zv[i] = zv[k];
+        }
+      }
     }
     __syncthreads();
   }
