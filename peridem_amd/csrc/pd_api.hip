@@ -1657,6 +1657,11 @@ int pd_job_get_state(pd_ctx* c, uint32_t job, uint32_t pi, double* disp, double*
   if (rc) return rc;
   const HostPart& p = *Hp;
   const Job& J = c->jobs[job];
+  if (disp || currpos) {
+    // the displacement lives in the split copies; U is its transfer form
+    hipLaunchKernelGGL(k_join, dim3(grid_of(p.n, 256)), dim3(256), 0, c->stream, c->d, p.off, (uint64_t)p.n);
+    HIPCHK(c, hipGetLastError());
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (disp || currpos) {
     std::vector<double4> U(p.n), X(p.n);

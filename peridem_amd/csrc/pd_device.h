@@ -79,13 +79,14 @@ struct DevCtl {
 struct Dev {
   // node state, global node index g
   double4* X;        // x, y, z, vol
-  double4* U;        // disp x, y, z, (unused)
+  double4* U;        // disp x, y, z, (unused): transfer form only (uploads, read-backs);
+                     // the resident displacement is UXY, UZ
   // the bond kernel's neighbour copies of X and U in 16- and 8-byte records:
   // a wavefront's gather of one field then spans fewer cache lines than the
   // same field read out of 32-byte records
   double2* XY;       // x, y
   double2* ZV;       // z, vol
-  double2* UXY;      // disp x, y   (rewritten by the position kernel each step)
+  double2* UXY;      // disp x, y: the resident displacement (written by the position kernel)
   double* UZ;        // disp z
   uint16_t* bloc;    // [entries] the bond entries as positions in their block's node table
   uint32_t* btab;    // [blocks * kLdsCap] per block: sorted pool indices of the nodes its rows reference

@@ -44,6 +44,25 @@ __global__ __launch_bounds__(256) void k_split(Dev d, uint64_t g0, uint64_t n) {
   d.UZ[g] = u.z;
 }
 
+// The displacement's resident form is the split copy (UXY, UZ); U holds it
+// only around host transfers (k_split after an upload, k_join before a
+// read-back).
+__device__ __forceinline__ double4 disp_of(const Dev& d, uint64_t g) {
+  const double2 a = d.UXY[g];
+  return make_double4(a.x, a.y, d.UZ[g], 0.0);
+}
+
+__device__ __forceinline__ void set_disp(const Dev& d, uint64_t g, double x, double y, double z) {
+  d.UXY[g] = make_double2(x, y);
+  d.UZ[g] = z;
+}
+
+__global__ __launch_bounds__(256) void k_join(Dev d, uint64_t g0, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  d.U[g0 + i] = disp_of(d, g0 + i);
+}
+
 // ---------------------------------------------------------------------------
 // Verlet position half (timeloop.h:537-540; overloads.h:51-70,149-159):
 //   disp += dt*vel + (dt*dt*0.5)*acc ; CurrPos = pos + disp
@@ -62,7 +81,7 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
     p = d.pid[g];
     jid = d.part[p].inst;
     const double dt = d.inst[jid].dt;
-    double4 u = d.U[g];
+    double4 u = disp_of(d, g);
     const double4 x = d.X[g];
     const double h = (dt * dt) * 0.5;
     double v0 = d.V[3 * g], v1 = d.V[3 * g + 1], v2 = d.V[3 * g + 2];
@@ -73,9 +92,7 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
     u.x = u.x + ((dt * v0) + (h * a0));
     u.y = u.y + ((dt * v1) + (h * a1));
     u.z = u.z + ((dt * v2) + (h * a2));
-    d.U[g] = u;
-    d.UXY[g] = make_double2(u.x, u.y);
-    d.UZ[g] = u.z;
+    set_disp(d, g, u.x, u.y, u.z);
     if (!(in_fast_range(u.x) & in_fast_range(u.y) & in_fast_range(u.z))) atomicOr(d.flags, kFlagSafeMath);
     const double4 ub = d.UB[g];
     const double e0 = u.x - ub.x, e1 = u.y - ub.y, e2 = u.z - ub.z;
@@ -170,7 +187,7 @@ __global__ __launch_bounds__(64) void k_centroid(Dev d) {
   double m0 = 0.0, m1 = 0.0, m2 = 0.0;
   for (uint64_t g = P.node_off; g < P.node_off + P.nnodes; g++) {
     const double4 x = d.X[g];
-    const double4 u = d.U[g];
+    const double4 u = disp_of(d, g);
     m0 += (x.x + u.x);
     m1 += (x.y + u.y);
     m2 += (x.z + u.z);
@@ -241,7 +258,7 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
   uint32_t key = kNoKey;
   if (live(I) && I.rebuild && (!I.nl_bdry_only || d.brank[g] != kNoKey)) {
     double4 x = d.X[g];
-    double4 u = d.U[g];
+    double4 u = disp_of(d, g);
     double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
     int64_t c[3];
     cell_of(cp, I.cell, c);
@@ -326,7 +343,7 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
   const DevInst& I = d.inst[P.inst];
   if (!live(I) || !I.rebuild) return;
   const double4 xi = d.X[g];
-  const double4 ui = d.U[g];
+  const double4 ui = disp_of(d, g);
   d.UB[g] = ui;
   if (I.nl_bdry_only && d.brank[g] == kNoKey) {   // not a boundary node: no contact
     d.ccnt[g] = 0;
@@ -358,7 +375,7 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
           if (Q.inst != P.inst) continue;
           if (pj == p && !selfc) continue;
           const double4 xj = d.X[j];
-          const double4 uj = d.U[j];
+          const double4 uj = disp_of(d, j);
           const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
                                  (uj.z - ui.z) + (xj.z - xi.z));
           if (!(dn <= I.rb)) continue;
@@ -467,7 +484,7 @@ __device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p
   double vi[3] = {0.0, 0.0, 0.0};
   if (n || wall || I.enable_torque) {
     xi = d.X[g];
-    ui = d.U[g];
+    ui = disp_of(d, g);
   }
   if (n || wall) {
     vi[0] = d.Vold[3 * g];
@@ -484,7 +501,7 @@ __device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p
     if (j & kBroken) break;   // the other particles' entries start here
     if (!selfc) continue;
     const double4 xj = d.X[j];
-    const double4 uj = d.U[j];
+    const double4 uj = disp_of(d, j);
     const double xx[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
     const double dd[3] = {(uj.x - ui.x) + xx[0], (uj.y - ui.y) + xx[1], (uj.z - ui.z) + xx[2]};
     const double dn = nrm3(dd[0], dd[1], dd[2]);
@@ -559,7 +576,7 @@ __device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p
     if (!(d.part[lo].movable || d.part[hi].movable)) continue;
     if (!intersects_box_fext(d.lrtb + 6 * lo, d.lrtb + 6 * hi, I.Rc)) continue;
     const double4 xj = d.X[j];
-    const double4 uj = d.U[j];
+    const double4 uj = disp_of(d, j);
     double dd[3], rv[3];
     if (below) {   // the partner is P_C's node i, we are P_N's node j
       dd[0] = (ui.x - uj.x) + (xi.x - xj.x);
@@ -1133,7 +1150,7 @@ __global__ __launch_bounds__(256) void k_clamp(Dev d) {
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
   if (!live(I) || !P.movable || (I.vc_on && p == I.vc_slot)) return;
-  d.U[g] = make_double4(0.0, 0.0, 0.0, 0.0);
+  set_disp(d, g, 0.0, 0.0, 0.0);
 }
 
 // Prescribed velocity (timeloop.h:685-764), one thread per job: the mean
@@ -1154,7 +1171,7 @@ __global__ void k_pv_mean(Dev d) {
   if (I.vc_fix_ang) {
     for (uint64_t g = P.node_off; g < P.node_off + P.nnodes; g++) {
       const double4 x = d.X[g];
-      const double4 u = d.U[g];
+      const double4 u = disp_of(d, g);
       c[0] += (x.x + u.x);
       c[1] += (x.y + u.y);
       c[2] += (x.z + u.z);
@@ -1180,7 +1197,7 @@ __global__ __launch_bounds__(256) void k_pv_apply(Dev d) {
   double vr[3] = {v[0] - mean[0], v[1] - mean[1], v[2] - mean[2]};
   if (I.vc_fix_ang) {
     const double4 x = d.X[g];
-    const double4 u = d.U[g];
+    const double4 u = disp_of(d, g);
     const double rv[3] = {(x.x + u.x) - I.vc_cent[0], (x.y + u.y) - I.vc_cent[1], (x.z + u.z) - I.vc_cent[2]};
     const double r = nrm3(rv[0], rv[1], rv[2]);
     double ru[3] = {0, 0, 0};
@@ -1197,7 +1214,7 @@ __global__ __launch_bounds__(256) void k_pv_apply(Dev d) {
     d.AFD[3 * g + k] = P.rho * (((2 / I.dt) * (pv - v[k])) - d.AFD[3 * g + k]) - d.F[3 * g + k];
   }
   if (d.nflag[g] & 1u) {
-    d.U[g] = make_double4(0.0, 0.0, 0.0, 0.0);
+    set_disp(d, g, 0.0, 0.0, 0.0);
     for (int k = 0; k < 3; k++) d.V[3 * g + k] = d.A[3 * g + k] = 0.0;
   }
 }
@@ -1249,7 +1266,7 @@ __global__ void k_reaction(Dev d) {
         const uint32_t h = d.blist[P.boff + r];
         if (!d.wmask[h]) continue;
         const double4 x = d.X[h];
-        const double4 u = d.U[h];
+        const double4 u = disp_of(d, h);
         const double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
         const double v[3] = {d.Vold[3 * h], d.Vold[3 * h + 1], d.Vold[3 * h + 2]};
         double fdummy[3] = {0, 0, 0};
@@ -1272,7 +1289,7 @@ __global__ void k_reaction(Dev d) {
       if (!d.wmask[g]) continue;
       const DevPart& P = d.part[d.pid[g]];
       double4 x = d.X[g];
-      double4 u = d.U[g];
+      double4 u = disp_of(d, g);
       double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
       double v[3] = {d.Vold[3 * g], d.Vold[3 * g + 1], d.Vold[3 * g + 2]};
       double fdummy[3] = {0, 0, 0};
