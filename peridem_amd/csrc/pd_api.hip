@@ -598,6 +598,7 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipMemsetAsync(d.deg, 0, N * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.soff, 0, S * sizeof(uint64_t), st));
   HIPCHK(c, hipMemsetAsync(d.swid, 0, S * sizeof(uint32_t), st));
+  HIPCHK(c, hipMemsetAsync(d.cstart, 0, ((size_t)T + 1) * sizeof(uint32_t), st));
   if (c->cap_entries) HIPCHK(c, hipMemsetAsync(d.bond, 0xFF, c->cap_entries * sizeof(uint32_t), st));
   if (c->cap_entries) HIPCHK(c, hipMemsetAsync(d.bloc, 0xFF, c->cap_entries * sizeof(uint16_t), st));
   HIPCHK(c, hipMemsetAsync(d.btab, 0, S / kBlockSlices * kLdsCap * sizeof(uint32_t), st));
@@ -1194,11 +1195,9 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
   if (any_torque(c)) hipLaunchKernelGGL(k_centroid, dim3(grid_of(d.P, 64)), dim3(64), 0, st, d);
   if (any_contact_possible(c)) {
     // every kernel of the list rebuild exits at once unless a job rebuilds
-    hipLaunchKernelGGL(k_clear_cells, dim3(grid_of((uint64_t)d.T + 1, 256)), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_cellkey, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
     const uint32_t nb = grid_of(d.T, kScanBlock);
     hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(kScanBlock), 0, st, d.ctl, d.cstart, d.T, d.scan_tmp);
-    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kScanBlock), 0, st, d.ctl, d.scan_tmp, nb, d.cstart + d.T);
     hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(kScanBlock), 0, st, d.ctl, d.cstart, d.T, d.scan_tmp, d.ccur);
     hipLaunchKernelGGL(k_scatter, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_build, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
@@ -1218,7 +1217,8 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
     hipLaunchKernelGGL(k_pv_mean, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
     hipLaunchKernelGGL(k_pv_apply, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   }
-  hipLaunchKernelGGL(k_finalize, dim3(grid_of(std::max<uint64_t>(d.P, d.ninst), 256)), dim3(256), 0, st, d);
+  // at least 64 blocks: after a rebuild they also clear the cell table
+  hipLaunchKernelGGL(k_finalize, dim3(grid_of(std::max<uint64_t>({d.P, d.ninst, 64 * 256}), 256)), dim3(256), 0, st, d);
   if ((rc = mark(5))) return rc;
   HIPCHK(c, hipGetLastError());
   return PD_OK;

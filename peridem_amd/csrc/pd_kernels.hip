@@ -5,7 +5,7 @@
 //   k_decide_pairs  per job: rebuild the contact candidate lists this step?
 //                and the particle-pair box test on the lagged boxes
 //                (timeloop.h:636-653)
-//   cell list    k_clear_cells -> k_cellkey -> scan -> k_scatter -> k_build
+//   cell list    k_cellkey -> scan -> k_scatter -> k_build
 //                (only in steps where some job rebuilds its candidate lists)
 //   bond kernel  PD bond force + fused irreversible break, per-node row sums
 //                (k_bond_lds: block tables staged in LDS; k_bond_t: unstaged),
@@ -14,7 +14,7 @@
 //                and the Verlet velocity half (velocity_half)
 //   k_reaction   wall reaction sum (jobs that report it, on its steps)
 //   k_clamp      clamped nodes' displacement (pools with clamped nodes)
-//   k_finalize   lagged boxes, wall motion
+//   k_finalize   lagged boxes, wall motion, cell table cleared after a rebuild
 // All floating point is FP64 with the reference's operation order; the build
 // uses -ffp-contract=off so no multiply-add is fused.
 #include "pd_device.h"
@@ -72,8 +72,10 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   __shared__ unsigned long long smin[3][256], smax[3][256];
   __shared__ uint32_t sp[256];
-  // the block's wall-contact flag of this step starts clear
+  // the block's wall-contact flag of this step starts clear, and so does the
+  // step's rebuild flag (set by k_decide_pairs)
   if (threadIdx.x == 0 && g < d.N) d.bflag[g >> 8] = 0;
+  if (g == 0) d.ctl->any_rebuild = 0;
   double cp[3] = {0, 0, 0};
   uint32_t p = 0xFFFFFFFFu, jid = 0xFFFFFFFFu;
   unsigned long long dkey = 0ull;
@@ -242,12 +244,8 @@ __device__ __forceinline__ void cell_of(const double* cp, double cell, int64_t* 
 }
 
 // Cell list of the nodes of the jobs that rebuild their candidate lists.
-__global__ __launch_bounds__(256) void k_clear_cells(Dev d) {
-  if (!d.ctl->any_rebuild) return;
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i <= d.T) d.cstart[i] = 0;
-}
-
+// The cell table (cstart) is zero when a rebuild starts: cleared at
+// reservation, then by k_finalize after every rebuild.
 __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
   if (!d.ctl->any_rebuild) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -269,7 +267,7 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
 }
 
 // exclusive scan of cstart[0..T) (counts) in place; cstart[T] = total.
-// Three phases: per-block scan, scan of block sums, add-back.
+// Two phases: per-block scan (k_scan1), add-back (k_scan3).
 constexpr int kScanBlock = 1024;
 __global__ __launch_bounds__(kScanBlock) void k_scan1(const DevCtl* ctl, uint32_t* a, uint32_t n, uint32_t* sums) {
   if (!ctl->any_rebuild) return;
@@ -287,37 +285,27 @@ __global__ __launch_bounds__(kScanBlock) void k_scan1(const DevCtl* ctl, uint32_
   if (i < n) a[i] = s[threadIdx.x] - v;
   if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = s[threadIdx.x];
 }
-__global__ __launch_bounds__(kScanBlock) void k_scan2(const DevCtl* ctl, uint32_t* sums, uint32_t nb, uint32_t* total) {
-  if (!ctl->any_rebuild) return;
-  __shared__ uint32_t s[kScanBlock];
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < nb; base += kScanBlock) {
-    uint32_t i = base + threadIdx.x;
-    uint32_t v = i < nb ? sums[i] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < kScanBlock; off <<= 1) {
-      uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
-      __syncthreads();
-      s[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < nb) sums[i] = s[threadIdx.x] - v + carry;
-    uint32_t blocktot = s[kScanBlock - 1];
-    __syncthreads();
-    carry += blocktot;
-  }
-  if (threadIdx.x == 0) *total = carry;
-}
+// Add-back: each block sums the totals of the blocks before it itself (exact
+// integer sums, L2-resident), so no separate pass scans the block totals;
+// the last block writes the grand total to a[n].
 __global__ __launch_bounds__(kScanBlock) void k_scan3(const DevCtl* ctl, uint32_t* a, uint32_t n, const uint32_t* sums,
                                                       uint32_t* cur) {
   if (!ctl->any_rebuild) return;
+  __shared__ uint32_t red[kScanBlock / 64];
+  uint32_t part = 0;
+  for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanBlock) part += sums[j];
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  uint32_t prefix = 0;
+  for (int w = 0; w < kScanBlock / 64; w++) prefix += red[w];
   uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
   if (i < n) {
-    uint32_t v = a[i] + sums[blockIdx.x];
+    uint32_t v = a[i] + prefix;
     a[i] = v;
     cur[i] = v;
   }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) a[n] = prefix + sums[blockIdx.x];
 }
 
 __global__ __launch_bounds__(256) void k_scatter(Dev d) {
@@ -1224,7 +1212,10 @@ __global__ __launch_bounds__(256) void k_pv_apply(Dev d) {
 __global__ __launch_bounds__(256) void k_finalize(Dev d) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < d.P) {
-    if (live(d.inst[d.part[q].inst]) && d.part[q].movable)
+    // the box of this step if the particle's job stepped: every node of a
+    // movable particle of a stepping job entered lrtb_next (the job's
+    // remaining count is being decremented by another thread of this launch)
+    if (d.part[q].movable && d.lrtb_next[6 * q] != ~0ull)
       for (int k = 0; k < 6; k++) d.lrtb[6 * q + k] = okey_inv(d.lrtb_next[6 * q + k]);
     for (int k = 0; k < 3; k++) {
       d.lrtb_next[6 * q + k] = ~0ull;
@@ -1240,9 +1231,13 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d) {
       I.remaining -= 1;
     }
   }
-  if (q == 0) {
-    if (d.ctl->steps_left) d.ctl->steps_left -= 1;
-    d.ctl->any_rebuild = 0;
+  if (q == 0 && d.ctl->steps_left) d.ctl->steps_left -= 1;
+  // after a rebuild, the cell table is cleared for the next one (nothing
+  // reads it after k_build); the flag itself is reset by the next step's
+  // position kernel
+  if (d.ctl->any_rebuild) {
+    const uint64_t n = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = q; i <= d.T; i += n) d.cstart[i] = 0;
   }
 }
 
