@@ -1,6 +1,8 @@
 """Summaries of rocprofv3 CSV output for the profiles/ folder.
 
   python -m peridem_amd.profsum trace <dir>            kernel-trace --stats table
+  python -m peridem_amd.profsum steps <dir>            the same from the per-dispatch
+        trace, each kernel's first SKIP dispatches left out when it has more
   python -m peridem_amd.profsum counters <dir> [kernel] per-dispatch PMC averages
   python -m peridem_amd.profsum traffic <fetch_dir> <write_dir> <copy_mib> <nodes> <bonds>
         HBM bytes per launch of every kernel, FETCH_SIZE/WRITE_SIZE scaled by
@@ -8,6 +10,12 @@
   python -m peridem_amd.profsum fp64 <dir> [out.json]
         FP64 wave-instructions per launch of every kernel (SQ_INSTS_VALU_ADD/
         MUL/FMA/TRANS_F64), for the FP64 issue utilisation
+
+Per-launch means (steps, counters, traffic, fp64) leave out each kernel's
+first SKIP = 4 dispatches when it has more: the first step of a pool times
+every bond-kernel shape with one warm-up and three timed launches on scratch
+copies, without the contact terms and velocity half that the step's own
+launches add, so the selected shape's step launches are the ones after those.
 """
 import collections
 import csv
@@ -38,11 +46,38 @@ def trace(d):
               f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
 
 
+SKIP = 4
+
+
+def _steps_only(vals):
+    """Values in dispatch order, less the autotuner's launches."""
+    vals = [v for _, v in sorted(vals)]
+    return vals[SKIP:] if len(vals) > SKIP else vals
+
+
 def _agg(d):
-    out = collections.defaultdict(lambda: collections.defaultdict(list))
+    raw = collections.defaultdict(lambda: collections.defaultdict(list))
     for r in _rows(d, "counter_collection.csv"):
-        out[_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        raw[_short(r["Kernel_Name"])][r["Counter_Name"]].append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+    out = collections.defaultdict(lambda: collections.defaultdict(list))
+    for k, cs in raw.items():
+        for c, v in cs.items():
+            out[k][c] = _steps_only(v)
     return out
+
+
+def steps(d):
+    per = collections.defaultdict(list)
+    for r in _rows(d, "kernel_trace.csv"):
+        per[_short(r["Kernel_Name"])].append(
+            (int(r["Dispatch_Id"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
+    rows = sorted(((k, _steps_only(v)) for k, v in per.items()), key=lambda kv: -sum(kv[1]))
+    total = sum(sum(v) for _, v in rows)
+    print("| kernel | launches | avg us | min us | max us | % of GPU time |")
+    print("|---|---|---|---|---|---|")
+    for k, v in rows:
+        print(f"| `{k}` | {len(v)} | {sum(v) / len(v):.1f} | {min(v):.1f} | {max(v):.1f} | "
+              f"{100 * sum(v) / total:.1f} |")
 
 
 def counters(d, kernel=None):
@@ -108,6 +143,8 @@ if __name__ == "__main__":
     cmd = sys.argv[1]
     if cmd == "trace":
         trace(sys.argv[2])
+    elif cmd == "steps":
+        steps(sys.argv[2])
     elif cmd == "counters":
         counters(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else None)
     elif cmd == "traffic":
