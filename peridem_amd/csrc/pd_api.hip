@@ -614,6 +614,7 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipMemsetAsync(d.reaction, 0, 18 * I * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.wmask, 0, N, st));
   HIPCHK(c, hipMemsetAsync(d.ccnt, 0, N, st));
+  HIPCHK(c, hipMemsetAsync(d.Vold, 0, 3 * N * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.UB, 0, N * sizeof(double4), st));
   HIPCHK(c, hipMemsetAsync(d.bflag, 0, N / 256 + 1, st));
   std::vector<DevPart> vparts(P);
@@ -1135,7 +1136,6 @@ static Dev step_view(const pd_ctx* c) {
   Dev d = c->d;
   d.S = c->hwm_slices;
   d.N = d.S * kSlice;
-  d.finish = 1;
   uint32_t T = 1024;
   while (T < 2 * d.N && T < c->d.T) T <<= 1;
   d.T = T;
@@ -1225,9 +1225,11 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
 }
 
 // Time every bond-kernel shape on the resident pool and keep the fastest.
-// The timed launches write their force sums and breaks into scratch copies,
-// so the state is untouched.  Pools too large for 32-bit record offsets have
-// one shape only.
+// The timed launches are whole step launches (bond sums, breaks, contact
+// terms, velocity half) writing into scratch copies of everything the kernel
+// writes (forces, bond entries, velocities, accelerations, applied force,
+// wall masks), so the state is untouched.  Pools too large for 32-bit record
+// offsets have one shape only.
 static int autotune(pd_ctx* c, uint32_t reps) {
   if (wide_pool(c)) {
     c->bond_shape = kDefaultBondShape;
@@ -1236,14 +1238,15 @@ static int autotune(pd_ctx* c, uint32_t reps) {
   Dev t = step_view(c);
   const uint64_t Np = t.S * kSlice;
   int rc = PD_OK;
-  double* F = nullptr;
+  double *F = nullptr, *A = nullptr, *V = nullptr, *AFD = nullptr;
   uint32_t* B = nullptr;
   uint16_t* B16 = nullptr;
+  uint8_t *WM = nullptr, *BF = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const uint64_t ne = std::max<uint64_t>(c->cap_entries, 1);
-  if ((rc = dalloc(c, &F, 3 * Np)) || (rc = dalloc(c, &B, ne)) || (rc = dalloc(c, &B16, ne))) return rc;
   auto release = [&]() {
-    for (void* q : {(void*)F, (void*)B, (void*)B16}) {
+    for (void* q : {(void*)F, (void*)B, (void*)B16, (void*)A, (void*)V, (void*)AFD, (void*)WM, (void*)BF}) {
+      if (!q) continue;
       auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
       if (it != c->allocs.end()) c->allocs.erase(it);
       (void)hipFree(q);
@@ -1252,7 +1255,19 @@ static int autotune(pd_ctx* c, uint32_t reps) {
     if (e1) (void)hipEventDestroy(e1);
   };
   auto run = [&]() -> int {
+    if ((rc = dalloc(c, &F, 3 * Np)) || (rc = dalloc(c, &B, ne)) || (rc = dalloc(c, &B16, ne)) ||
+        (rc = dalloc(c, &A, 3 * Np)) || (rc = dalloc(c, &V, 3 * Np)) ||
+        (c->d.AFD && (rc = dalloc(c, &AFD, 3 * Np))) || (rc = dalloc(c, &WM, Np)) ||
+        (rc = dalloc(c, &BF, Np / 256 + 1)))
+      return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(A, c->d.A, 3 * Np * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(V, c->d.V, 3 * Np * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    // the applied force exists once a prescribed velocity was set; only its
+    // particle's nodes write it
+    if (AFD) HIPCHK(c, hipMemcpyAsync(AFD, c->d.AFD, 3 * Np * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(WM, 0, Np, c->stream));
+    HIPCHK(c, hipMemsetAsync(BF, 0, Np / 256 + 1, c->stream));
     if (c->cap_entries) {
       HIPCHK(c, hipMemcpyAsync(B, c->d.bond, c->cap_entries * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(c, hipMemcpyAsync(B16, c->d.bloc, c->cap_entries * sizeof(uint16_t), hipMemcpyDeviceToDevice, c->stream));
@@ -1262,7 +1277,11 @@ static int autotune(pd_ctx* c, uint32_t reps) {
     t.F = F;
     t.bond = B;
     t.bloc = B16;
-    t.finish = 0;   // the contact terms and velocity half are the same code in every shape and write state
+    t.A = A;
+    t.V = V;
+    t.AFD = AFD;
+    t.wmask = WM;
+    t.bflag = BF;
     const dim3 grid((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds);
     int best = kDefaultBondShape;
     for (int v = 0; v < kBondShapes; v++) {

@@ -142,8 +142,7 @@ struct Dev {
   uint32_t* scan_tmp;
   uint64_t N, S, P, npairs, ninst;   // N = S*64 node slots; P, ninst incl. the void slot
   uint32_t T;        // hash table size (power of two)
-  uint32_t finish;   // the bond kernel finishes each node's step: contact terms and
-                     // the velocity half (0: the force sums only)
+  uint32_t pad;
 };
 
 // ---------------------------------------------------------------------------

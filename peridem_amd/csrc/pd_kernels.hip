@@ -446,14 +446,11 @@ struct ContactIn {
 };
 
 __device__ __forceinline__ ContactIn contact_in(const Dev& d, uint64_t g, uint32_t p, bool movable) {
-  ContactIn c{0.0, 0.0, 0.0, 0u};
-  if (d.finish) {
-    c.pa = d.pactive[p];
-    if (movable) {
-      c.e0 = d.E[3 * g];
-      c.e1 = d.E[3 * g + 1];
-      c.e2 = d.E[3 * g + 2];
-    }
+  ContactIn c{0.0, 0.0, 0.0, d.pactive[p]};
+  if (movable) {
+    c.e0 = d.E[3 * g];
+    c.e1 = d.E[3 * g + 1];
+    c.e2 = d.E[3 * g + 2];
   }
   return c;
 }
@@ -1030,7 +1027,7 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     rec<false>(d, (uint32_t)g, xi, ui);
     bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, H.frac, H.cnot, H.snot, f0, f1, f2);
   }
-  if (d.finish) finish_node(d, g, p, cin, f0, f1, f2);
+  finish_node(d, g, p, cin, f0, f1, f2);
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
@@ -1097,7 +1094,7 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
     rec_u<true>(d, (uint32_t)g, ui);
     bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, H.frac, H.cnot, H.snot, f0, f1, f2);
   }
-  if (d.finish) finish_node(d, g, p, cin, f0, f1, f2);
+  finish_node(d, g, p, cin, f0, f1, f2);
   d.F[3 * g] = f0;
   d.F[3 * g + 1] = f1;
   d.F[3 * g + 2] = f2;
