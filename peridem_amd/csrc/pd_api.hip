@@ -520,8 +520,10 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   c->cap_parts = pool->max_particles;
   c->cap_jobs = pool->max_jobs;
   const uint64_t P = c->cap_parts + 1, I = c->cap_jobs + 1;  // + the void slots
+  // cell hash table: a power of two >= the node slots (occupied cells of edge
+  // rb hold ~1-2 nodes each, so it is at most about half full)
   uint32_t T = 1024;
-  while (T < 2 * N && T < (1u << 31)) T <<= 1;
+  while (T < N && T < (1u << 31)) T <<= 1;
   Dev& d = c->d;
   d.N = N;
   d.S = S;
@@ -1137,7 +1139,7 @@ static Dev step_view(const pd_ctx* c) {
   d.S = c->hwm_slices;
   d.N = d.S * kSlice;
   uint32_t T = 1024;
-  while (T < 2 * d.N && T < c->d.T) T <<= 1;
+  while (T < d.N && T < c->d.T) T <<= 1;
   d.T = T;
   return d;
 }

@@ -358,10 +358,11 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
         for (uint32_t it = b0; it < b1; it++) {
           const uint32_t j = d.csorted[it];
           if (j == g) continue;
-          const uint32_t pj = d.pid[j];
-          const DevPart& Q = d.part[pj];
-          if (Q.inst != P.inst) continue;
-          if (pj == p && !selfc) continue;
+          // jobs and particles hold contiguous node ranges: membership is a
+          // range test (hash buckets are shared by every job)
+          if (j < I.node0 || j >= I.node1) continue;
+          const bool same = j >= P.node_off && j < P.node_off + P.nnodes;
+          if (same && !selfc) continue;
           const double4 xj = d.X[j];
           const double4 uj = disp_of(d, j);
           const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
@@ -369,10 +370,12 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
           if (!(dn <= I.rb)) continue;
           // the reference visits partners particle by particle (pair order =
           // node order of the particles), each in node order or, with
-          // nl_bdry_only, in boundary-list order
-          const uint32_t rank = I.nl_bdry_only ? d.brank[j] : (uint32_t)(j - Q.node_off);
-          const uint64_t key =
-              ((uint64_t)(pj != p) << 63) | ((uint64_t)Q.node_off << 32) | (uint64_t)rank;
+          // nl_bdry_only, in boundary-list order.  (particle's node_off,
+          // rank in node order) orders like the node index itself, the
+          // particles' ranges being disjoint and ascending.
+          uint64_t key = ((uint64_t)!same << 63) | (uint64_t)j;
+          if (I.nl_bdry_only)
+            key = ((uint64_t)!same << 63) | ((uint64_t)d.part[d.pid[j]].node_off << 32) | (uint64_t)d.brank[j];
           if (ncand == kMaxCand) {
             atomicOr(d.err, kErrCandOverflow);
             continue;
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
             ckey[w] = ckey[w - 1];
             w--;
           }
-          cand[w] = (pj == p ? 0u : kBroken) | j;
+          cand[w] = (same ? 0u : kBroken) | j;
           ckey[w] = key;
         }
       }
