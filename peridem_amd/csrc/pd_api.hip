@@ -377,6 +377,12 @@ static bool any_vc(pd_ctx* c) {
   return false;
 }
 
+static bool any_bdry_only(pd_ctx* c) {
+  for (auto& J : c->jobs)
+    if (J.used && J.run.nl_bdry_only) return true;
+  return false;
+}
+
 static bool any_clamped(pd_ctx* c) {
   for (auto& J : c->jobs)
     if (J.used)
@@ -1203,6 +1209,7 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
     hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(kScanBlock), 0, st, d.ctl, d.cstart, d.T, d.scan_tmp, d.ccur);
     hipLaunchKernelGGL(k_scatter, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_build, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
+    if (any_bdry_only(c)) hipLaunchKernelGGL(k_build_bdry, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
   }
   if ((rc = mark(2))) return rc;
   // the bond kernel sums each node's row, then adds the node's contact,

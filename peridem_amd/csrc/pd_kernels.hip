@@ -322,27 +322,112 @@ __global__ __launch_bounds__(256) void k_scatter(Dev d) {
 // particle ones only while self-contact is on, sorted by the reference's
 // accumulation order (same particle first by node index, then the others by
 // global node index = particle order, then node order).
+//
+// Two forms.  Without nl_bdry_only a list entry, (other particle) << 31 |
+// node index, is its own order key (particle order is node order), and the
+// list is kept in registers: kMaxCand words, ascending, empty slots all
+// ones; a candidate is passed through every slot with a compare-exchange
+// (static indices, no private memory), dropped if equal to one -- the
+// duplicate a hash bucket met twice gives -- and the pass stops once the
+// value carried is empty in every lane.  In crushed regions lists fill up,
+// and an insertion sort in private memory would be a long chain of dependent
+// memory round trips.  With nl_bdry_only the order is the boundary lists'
+// (k_build_bdry).
+template <bool BDRY>
+__device__ __forceinline__ bool build_head(const Dev& d, uint64_t g, uint32_t& p, double4& xi, double4& ui,
+                                           bool& selfc, int64_t* c) {
+  p = d.pid[g];
+  const DevPart& P = d.part[p];
+  const DevInst& I = d.inst[P.inst];
+  if (!live(I) || !I.rebuild || (I.nl_bdry_only != 0) != BDRY) return false;
+  xi = d.X[g];
+  ui = disp_of(d, g);
+  d.UB[g] = ui;
+  if (BDRY && d.brank[g] == kNoKey) {   // not a boundary node: no contact
+    d.ccnt[g] = 0;
+    return false;
+  }
+  selfc = self_active(P, I);
+  const double cpi[3] = {xi.x + ui.x, xi.y + ui.y, xi.z + ui.z};
+  cell_of(cpi, I.cell, c);
+  return true;
+}
+
 __global__ __launch_bounds__(256) void k_build(Dev d) {
   if (!d.ctl->any_rebuild) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
-  const uint32_t p = d.pid[g];
+  uint32_t p;
+  double4 xi, ui;
+  bool selfc;
+  int64_t c[3];
+  if (!build_head<false>(d, g, p, xi, ui, selfc, c)) return;
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
-  if (!live(I) || !I.rebuild) return;
-  const double4 xi = d.X[g];
-  const double4 ui = disp_of(d, g);
-  d.UB[g] = ui;
-  if (I.nl_bdry_only && d.brank[g] == kNoKey) {   // not a boundary node: no contact
-    d.ccnt[g] = 0;
-    return;
+  constexpr uint32_t kEmpty = ~0u;
+  uint32_t w[kMaxCand];
+#pragma unroll
+  for (int i = 0; i < kMaxCand; i++) w[i] = kEmpty;
+  bool over = false;
+  for (int s = 0; s < 27; s++) {
+    const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = s / 9 - 1;
+    const uint32_t key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
+    const uint32_t b0 = d.cstart[key], b1 = d.cstart[key + 1];
+    for (uint32_t it = b0; it < b1; it++) {
+      const uint32_t j = d.csorted[it];
+      if (j == g) continue;
+      // jobs and particles hold contiguous node ranges: membership is a
+      // range test (hash buckets are shared by every job)
+      if (j < I.node0 || j >= I.node1) continue;
+      const bool same = j >= P.node_off && j < P.node_off + P.nnodes;
+      if (same && !selfc) continue;
+      const double4 xj = d.X[j];
+      const double4 uj = disp_of(d, j);
+      const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
+                             (uj.z - ui.z) + (xj.z - xi.z));
+      if (!(dn <= I.rb)) continue;
+      uint32_t x = (same ? 0u : kBroken) | j;
+#pragma unroll
+      for (int i0 = 0; i0 < kMaxCand; i0 += 8) {
+        if (__builtin_amdgcn_ballot_w64(x != kEmpty) == 0) continue;   // nothing left to place
+#pragma unroll
+        for (int i = i0; i < i0 + 8; i++) {
+          const uint32_t y = w[i];
+          x = x == y ? kEmpty : x;
+          w[i] = x < y ? x : y;
+          x = x < y ? y : x;
+        }
+      }
+      over |= x != kEmpty;   // a candidate fell off the end
+    }
   }
-  const bool selfc = self_active(P, I);
-  const double cpi[3] = {xi.x + ui.x, xi.y + ui.y, xi.z + ui.z};
+  if (over) atomicOr(d.err, kErrCandOverflow);
+  uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
+  uint32_t ncand = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxCand; i++)
+    if (w[i] != kEmpty) {
+      cl[(uint64_t)i * kSlice] = w[i];
+      ncand++;
+    }
+  d.ccnt[g] = (uint8_t)ncand;
+}
+
+// Jobs with nl_bdry_only: candidates ordered by (other particle, particle
+// order, rank in the boundary list), kept by insertion into private memory.
+__global__ __launch_bounds__(256) void k_build_bdry(Dev d) {
+  if (!d.ctl->any_rebuild) return;
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= d.N) return;
+  uint32_t p;
+  double4 xi, ui;
+  bool selfc;
   int64_t c[3];
-  cell_of(cpi, I.cell, c);
+  if (!build_head<true>(d, g, p, xi, ui, selfc, c)) return;
+  const DevPart& P = d.part[p];
+  const DevInst& I = d.inst[P.inst];
   uint32_t cand[kMaxCand];
-  uint64_t ckey[kMaxCand];   // order key: other particle, particle order, rank in the particle
+  uint64_t ckey[kMaxCand];   // order key: other particle, particle order, rank in the boundary list
   uint32_t ncand = 0;
   uint32_t seen[27];
   int nseen = 0;
@@ -358,8 +443,6 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
         for (uint32_t it = b0; it < b1; it++) {
           const uint32_t j = d.csorted[it];
           if (j == g) continue;
-          // jobs and particles hold contiguous node ranges: membership is a
-          // range test (hash buckets are shared by every job)
           if (j < I.node0 || j >= I.node1) continue;
           const bool same = j >= P.node_off && j < P.node_off + P.nnodes;
           if (same && !selfc) continue;
@@ -368,26 +451,20 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
           const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
                                  (uj.z - ui.z) + (xj.z - xi.z));
           if (!(dn <= I.rb)) continue;
-          // the reference visits partners particle by particle (pair order =
-          // node order of the particles), each in node order or, with
-          // nl_bdry_only, in boundary-list order.  (particle's node_off,
-          // rank in node order) orders like the node index itself, the
-          // particles' ranges being disjoint and ascending.
-          uint64_t key = ((uint64_t)!same << 63) | (uint64_t)j;
-          if (I.nl_bdry_only)
-            key = ((uint64_t)!same << 63) | ((uint64_t)d.part[d.pid[j]].node_off << 32) | (uint64_t)d.brank[j];
+          const uint64_t key2 =
+              ((uint64_t)!same << 63) | ((uint64_t)d.part[d.pid[j]].node_off << 32) | (uint64_t)d.brank[j];
           if (ncand == kMaxCand) {
             atomicOr(d.err, kErrCandOverflow);
             continue;
           }
           uint32_t w = ncand++;
-          while (w > 0 && ckey[w - 1] > key) {
+          while (w > 0 && ckey[w - 1] > key2) {
             cand[w] = cand[w - 1];
             ckey[w] = ckey[w - 1];
             w--;
           }
           cand[w] = (same ? 0u : kBroken) | j;
-          ckey[w] = key;
+          ckey[w] = key2;
         }
       }
   uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
