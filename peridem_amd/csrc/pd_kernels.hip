@@ -369,12 +369,22 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
 #pragma unroll
   for (int i = 0; i < kMaxCand; i++) w[i] = kEmpty;
   bool over = false;
+  // each cell's bucket bounds are loaded one cell ahead, each member's index
+  // one member ahead, so their round trips overlap the work on the previous
+  uint32_t key = cell_hash(c[0] - 1, c[1] - 1, c[2] - 1, P.inst, d.T);
+  uint32_t nb0 = d.cstart[key], nb1 = d.cstart[key + 1];
   for (int s = 0; s < 27; s++) {
-    const int dx = s % 3 - 1, dy = (s / 3) % 3 - 1, dz = s / 9 - 1;
-    const uint32_t key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
-    const uint32_t b0 = d.cstart[key], b1 = d.cstart[key + 1];
+    const uint32_t b0 = nb0, b1 = nb1;
+    if (s < 26) {
+      const int t = s + 1, dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+      key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
+      nb0 = d.cstart[key];
+      nb1 = d.cstart[key + 1];
+    }
+    uint32_t nj = b0 < b1 ? d.csorted[b0] : 0u;
     for (uint32_t it = b0; it < b1; it++) {
-      const uint32_t j = d.csorted[it];
+      const uint32_t j = nj;
+      if (it + 1 < b1) nj = d.csorted[it + 1];
       if (j == g) continue;
       // jobs and particles hold contiguous node ranges: membership is a
       // range test (hash buckets are shared by every job)
