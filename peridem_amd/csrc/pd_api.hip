@@ -579,7 +579,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(bflag, N / 256 + 1);
   ALLOC(clist, N * (uint64_t)kMaxCand);
   ALLOC(ccnt, N);
-  ALLOC(UB, N);
+  ALLOC(UBXY, N);
+  ALLOC(UBZ, N);
   ALLOC(ckey, N);
   ALLOC(cstart, (size_t)T + 1);
   ALLOC(ccur, T);
@@ -623,7 +624,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipMemsetAsync(d.wmask, 0, N, st));
   HIPCHK(c, hipMemsetAsync(d.ccnt, 0, N, st));
   HIPCHK(c, hipMemsetAsync(d.Vold, 0, 3 * N * sizeof(double), st));
-  HIPCHK(c, hipMemsetAsync(d.UB, 0, N * sizeof(double4), st));
+  HIPCHK(c, hipMemsetAsync(d.UBXY, 0, N * sizeof(double2), st));
+  HIPCHK(c, hipMemsetAsync(d.UBZ, 0, N * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.bflag, 0, N / 256 + 1, st));
   std::vector<DevPart> vparts(P);
   for (auto& q : vparts) void_part(q, vi);

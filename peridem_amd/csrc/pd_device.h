@@ -133,7 +133,8 @@ struct Dev {
   // clist[(g >> 6) * 64 * kMaxCand + 64 k + (g & 63)], ccnt[g] entries
   uint32_t* clist;
   uint8_t* ccnt;
-  double4* UB;       // [N] displacement when the node's list was built
+  double2* UBXY;     // [N] displacement when the node's list was built: x, y
+  double* UBZ;       //     and z
   // contact cell list
   uint32_t* ckey;    // [N]
   uint32_t* cstart;  // [T+1]

@@ -23,6 +23,23 @@ namespace pd {
 
 __device__ __forceinline__ bool live(const DevInst& I) { return I.remaining != 0; }
 
+// A record every lane of a wavefront reads at the same index, read through
+// the scalar cache (the constant address space: the fields read this way are
+// not written during the launch).
+template <class T>
+using kptr = const T __attribute__((address_space(4)))*;
+
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T* base, uint32_t i) {
+  static_assert(sizeof(T) % 4 == 0, "records of whole words");
+  T v;
+  uint32_t* w = reinterpret_cast<uint32_t*>(&v);
+  const kptr<uint32_t> s = (kptr<uint32_t>)((kptr<T>)base + i);
+#pragma unroll
+  for (unsigned k = 0; k < sizeof(T) / 4; k++) w[k] = s[k];
+  return v;
+}
+
 // Does job I accumulate its wall reaction in this step?  On its last step and
 // on the last step of a pd_step call, so the value read back afterwards is the
 // reaction of the job's latest step, as Wall.reaction_* is (timeloop.h:397).
@@ -79,10 +96,31 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
   double cp[3] = {0, 0, 0};
   uint32_t p = 0xFFFFFFFFu, jid = 0xFFFFFFFFu;
   unsigned long long dkey = 0ull;
-  if (g < d.N && live(d.inst[d.part[d.pid[g]].inst])) {
+  bool movable = false, stepping = false;
+  double dt = 0.0;
+  if (g < d.N) {
+    // the particle's and job's fields, through the scalar cache in
+    // single-particle wavefronts
     p = d.pid[g];
-    jid = d.part[p].inst;
-    const double dt = d.inst[jid].dt;
+    const uint32_t pu = __builtin_amdgcn_readfirstlane(p);
+    if (__builtin_amdgcn_ballot_w64(p != pu) == 0) {
+      const DevPart P = ld_uniform(d.part, pu);
+      const DevInst I = ld_uniform(d.inst, P.inst);
+      stepping = live(I);
+      movable = P.movable != 0;
+      jid = P.inst;
+      dt = I.dt;
+    } else {
+      const DevPart& P = d.part[p];
+      const DevInst& I = d.inst[P.inst];
+      stepping = live(I);
+      movable = P.movable != 0;
+      jid = P.inst;
+      dt = I.dt;
+    }
+    if (!stepping) p = jid = 0xFFFFFFFFu;
+  }
+  if (stepping) {
     double4 u = disp_of(d, g);
     const double4 x = d.X[g];
     const double h = (dt * dt) * 0.5;
@@ -96,13 +134,13 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
     u.z = u.z + ((dt * v2) + (h * a2));
     set_disp(d, g, u.x, u.y, u.z);
     if (!(in_fast_range(u.x) & in_fast_range(u.y) & in_fast_range(u.z))) atomicOr(d.flags, kFlagSafeMath);
-    const double4 ub = d.UB[g];
-    const double e0 = u.x - ub.x, e1 = u.y - ub.y, e2 = u.z - ub.z;
+    const double2 ub = d.UBXY[g];
+    const double e0 = u.x - ub.x, e1 = u.y - ub.y, e2 = u.z - d.UBZ[g];
     dkey = (unsigned long long)__double_as_longlong((e0 * e0 + e1 * e1) + e2 * e2);
     cp[0] = x.x + u.x;
     cp[1] = x.y + u.y;
     cp[2] = x.z + u.z;
-    if (!d.part[p].movable) p = 0xFFFFFFFFu;
+    if (!movable) p = 0xFFFFFFFFu;
   }
   // largest displacement since the candidate lists were built, per job: a
   // wavefront is one 64-node slice, and a slice belongs to one job
@@ -342,7 +380,8 @@ __device__ __forceinline__ bool build_head(const Dev& d, uint64_t g, uint32_t& p
   if (!live(I) || !I.rebuild || (I.nl_bdry_only != 0) != BDRY) return false;
   xi = d.X[g];
   ui = disp_of(d, g);
-  d.UB[g] = ui;
+  d.UBXY[g] = make_double2(ui.x, ui.y);
+  d.UBZ[g] = ui.z;
   if (BDRY && d.brank[g] == kNoKey) {   // not a boundary node: no contact
     d.ccnt[g] = 0;
     return false;
@@ -709,20 +748,6 @@ __device__ __forceinline__ void velocity_half(const Dev& d, uint64_t g, uint32_t
 // straddle two) the particle's and job's records are wave-uniform and are
 // read through the scalar cache -- they are not written during the launch --
 // instead of as a chain of per-lane loads.
-template <class T>
-using kptr = const T __attribute__((address_space(4)))*;
-
-template <class T>
-__device__ __forceinline__ T ld_uniform(const T* base, uint32_t i) {
-  static_assert(sizeof(T) % 4 == 0, "records of whole words");
-  T v;
-  uint32_t* w = reinterpret_cast<uint32_t*>(&v);
-  const kptr<uint32_t> s = (kptr<uint32_t>)((kptr<T>)base + i);
-#pragma unroll
-  for (unsigned k = 0; k < sizeof(T) / 4; k++) w[k] = s[k];
-  return v;
-}
-
 __device__ __forceinline__ void finish_node(const Dev& d, uint64_t g, uint32_t p, const ContactIn& cin,
                                             double& f0, double& f1, double& f2) {
   const uint32_t pu = __builtin_amdgcn_readfirstlane(p);
