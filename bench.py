@@ -34,14 +34,17 @@ batched: 256 independent two-sphere impacts (node counts per sphere
 log-uniform on [500, 20k], speed U(5, 50) m/s, E U(20, 100) GPa with s0 and
 contact stiffness per instance) admitted into one job pool and stepped
 together, against the same jobs run one after another through the plain
-interface.  sims/hour counts whole jobs (upload, steps, read-back); every
-job's result digest is compared with its solo run.
+interface.  sims/hour counts whole jobs (context or pool set-up, upload,
+steps, and the read-back of every result array: per particle disp, CurrPos,
+vel, acc, force and intact bonds, and the wall planes and reaction); the
+digests that compare every job's results with its solo run are formed after
+the clock stops, on both sides.
 serving: a seeded Poisson stream of aggregate-breakage jobs (2-64 spheres and
 cubes of 2k-8k nodes each, step budgets drawn from 5k-50k and scaled by
 --serve-step-scale) through continuous batching with admission by device
 footprint, at several arrival rates; p50/p99 latency and sims/hour next to a
 FIFO queue of sequential plain-GPU runs with the same arrivals (service times
-measured).
+measured); a served job is done when its results are read back.
 """
 import argparse
 import json

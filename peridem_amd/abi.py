@@ -121,6 +121,20 @@ def _bind(lib, prefix):
     return fns
 
 
+def _read_connectivity(call, initial_pairs, check):
+    """Intact bonds of a particle as (i, q) pairs, i < q, in row order: one call
+    with the particle's initial bond count as capacity (bonds only break), the
+    count-then-fill protocol only if a caller's system holds fewer pairs."""
+    m = _u64()
+    out = np.empty((max(initial_pairs, 1), 2), dtype=np.uint32)
+    if call(_ptr(out, C.c_uint32), initial_pairs, C.byref(m)) == 0:
+        return out[:m.value]
+    check(call(None, 0, C.byref(m)), "get_connectivity")
+    out = np.empty((m.value, 2), dtype=np.uint32)
+    check(call(_ptr(out, C.c_uint32), m.value, C.byref(m)), "get_connectivity")
+    return out
+
+
 _LIBS = {}
 
 
@@ -241,12 +255,8 @@ class Engine:
         self._check(self.f["set_state"](self.ctx, p, *[_ptr(a) for a in arrs]), "set_state")
 
     def connectivity(self, p):
-        m = _u64()
-        self._check(self.f["get_connectivity"](self.ctx, p, None, 0, C.byref(m)), "get_connectivity")
-        out = np.empty((m.value, 2), dtype=np.uint32)
-        self._check(self.f["get_connectivity"](self.ctx, p, _ptr(out, C.c_uint32), m.value,
-                                               C.byref(m)), "get_connectivity")
-        return out
+        return _read_connectivity(lambda buf, cap, m: self.f["get_connectivity"](self.ctx, p, buf, cap, m),
+                                  len(self.sys.particles[p].conn), self._check)
 
     def wall(self, instance=0):
         lrtb, reac = np.empty(6), np.empty(18)
@@ -442,13 +452,9 @@ class Pool:
         return out
 
     def connectivity(self, job, p):
-        m = _u64()
-        self._check(self.lib.pd_job_get_connectivity(self.ctx, job, p, None, 0, C.byref(m)),
-                    "job_get_connectivity")
-        out = np.empty((m.value, 2), dtype=np.uint32)
-        self._check(self.lib.pd_job_get_connectivity(self.ctx, job, p, _ptr(out, C.c_uint32),
-                                                     m.value, C.byref(m)), "job_get_connectivity")
-        return out
+        return _read_connectivity(
+            lambda buf, cap, m: self.lib.pd_job_get_connectivity(self.ctx, job, p, buf, cap, m),
+            len(self.systems[job].particles[p].conn), self._check)
 
     def wall(self, job):
         lrtb, reac = np.empty(6), np.empty(18)

@@ -1788,8 +1788,10 @@ int pd_job_get_connectivity(pd_ctx* c, uint32_t job, uint32_t pi, uint32_t* pair
   std::vector<uint32_t> ent(hi - lo);
   if (!ent.empty())
     HIPCHK(c, hipMemcpy(ent.data(), c->d.bond + lo, ent.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  // NbdArr2conn (timeloop.h:20-44): pairs (i, q) with i < q, in row order
-  std::vector<uint32_t> out;
+  // NbdArr2conn (timeloop.h:20-44): pairs (i, q) with i < q, in row order,
+  // written straight into the caller's buffer while they fit (a buffer of
+  // the particle's initial bond count always does: bonds only break)
+  uint64_t n = 0;
   for (uint32_t i = 0; i < p.n; i++) {
     const uint64_t g = p.off + i, t = (g & ~(uint64_t)255) + nl[i], s = t / kSlice, l = t % kSlice;
     for (uint32_t k = 0; k < deg[i]; k++) {
@@ -1797,17 +1799,18 @@ int pd_job_get_connectivity(pd_ctx* c, uint32_t job, uint32_t pi, uint32_t* pair
       if (e & kBroken) continue;
       const uint32_t q = e - (uint32_t)p.off;
       if (i < q) {
-        out.push_back(i);
-        out.push_back(q);
+        if (pairs && n < cap) {
+          pairs[2 * n] = i;
+          pairs[2 * n + 1] = q;
+        }
+        n++;
       }
     }
   }
-  *m = out.size() / 2;
-  if (!pairs) return PD_OK;
-  if (cap < *m)
+  *m = n;
+  if (pairs && cap < n)
     return fail(c, PD_E_ARG, "connectivity buffer holds %llu pairs, %llu needed", (unsigned long long)cap,
-                (unsigned long long)*m);
-  if (!out.empty()) memcpy(pairs, out.data(), out.size() * sizeof(uint32_t));
+                (unsigned long long)n);
   return PD_OK;
 }
 

@@ -34,19 +34,29 @@ from . import abi
 PREPARE_WORKERS = 16
 
 
-def digest_state(read_state, read_conn, read_wall, nparticles):
-    """SHA-1 over every particle's disp, CurrPos, vel, acc, force, its intact
-    bonds and the wall planes and reaction: equal digests = bitwise-equal results."""
-    h = hashlib.sha1()
+def read_results(read_state, read_conn, read_wall, nparticles):
+    """Every result array of a job as read back from the device: per particle
+    disp, CurrPos, vel, acc, force and its intact bonds, then the wall planes
+    and reaction."""
+    out = []
     for p in range(nparticles):
         st = read_state(p)
-        for k in ("disp", "currpos", "vel", "acc", "force"):
-            h.update(np.ascontiguousarray(st[k]).tobytes())
-        h.update(np.ascontiguousarray(read_conn(p)).tobytes())
-    lrtb, reac = read_wall()
-    h.update(np.ascontiguousarray(lrtb).tobytes())
-    h.update(np.ascontiguousarray(reac).tobytes())
+        out.extend(st[k] for k in ("disp", "currpos", "vel", "acc", "force"))
+        out.append(read_conn(p))
+    out.extend(read_wall())
+    return out
+
+
+def digest_arrays(arrays):
+    """SHA-1 over result arrays: equal digests = bitwise-equal results."""
+    h = hashlib.sha1()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()
+
+
+def digest_state(read_state, read_conn, read_wall, nparticles):
+    return digest_arrays(read_results(read_state, read_conn, read_wall, nparticles))
 
 
 def pool_capacity(systems, slack=1.0):
@@ -76,8 +86,10 @@ class Done:
 def run_batch(systems, nsteps, digests=False):
     """Admit all systems as jobs of one pool and step them to completion.
 
-    Returns (wall seconds incl. admission and read-back, device ms of the
-    stepping, digests, admission seconds)."""
+    Returns (wall seconds incl. admission and the read-back of every result
+    array, device ms of the stepping, digests, admission seconds).  The
+    digests are computed after the clock stops: they verify, they are not part
+    of a job."""
     cap = pool_capacity(systems)
     t0 = time.perf_counter()
     pool = abi.Pool(cap["max_nodes"], cap["max_bond_entries"], cap["max_particles"], cap["max_jobs"])
@@ -102,13 +114,14 @@ def run_batch(systems, nsteps, digests=False):
             pool.step(n)
             dev_ms += pool.last_step_ms()
             left -= n
-        out = []
+        res = []
         if digests:
             for j, s in zip(jobs, systems):
-                out.append(digest_state(lambda p, j=j: pool.state(j, p),
+                res.append(read_results(lambda p, j=j: pool.state(j, p),
                                         lambda p, j=j: pool.connectivity(j, p),
                                         lambda j=j: pool.wall(j), len(s.particles)))
         t2 = time.perf_counter()
+        out = [digest_arrays(r) for r in res]
         return t2 - t0, dev_ms, out, t1 - t0
     finally:
         pool.close()
@@ -117,8 +130,9 @@ def run_batch(systems, nsteps, digests=False):
 def run_sequential(systems, nsteps, digests=False):
     """Each system alone through the plain interface, one after another.
 
-    Returns per-job wall seconds (context creation, upload, stepping and
-    read-back) and digests."""
+    Returns per-job wall seconds (context creation, upload, stepping and the
+    read-back of every result array) and digests (computed after each job's
+    clock stops)."""
     times, out = [], []
     budgets = nsteps if isinstance(nsteps, (list, tuple)) else [nsteps] * len(systems)
     for s, n in zip(systems, budgets):
@@ -126,11 +140,11 @@ def run_sequential(systems, nsteps, digests=False):
         e = abi.gpu_engine(s)
         try:
             e.step(n)
-            d = digest_state(e.state, e.connectivity, e.wall, len(s.particles)) if digests else None
+            res = read_results(e.state, e.connectivity, e.wall, len(s.particles)) if digests else None
         finally:
             e.close()
         times.append(time.perf_counter() - t0)
-        out.append(d)
+        out.append(digest_arrays(res) if digests else None)
     return times, out
 
 
@@ -168,6 +182,7 @@ class Server:
     queue: deque = field(init=False, default_factory=deque)
     live: dict = field(init=False, default_factory=dict)
     done: list = field(init=False, default_factory=list)
+    results: list = field(init=False, default_factory=list)   # read-back arrays per done job, until serving ends
 
     def __post_init__(self):
         self.pool = abi.Pool(self.max_nodes, self.max_bond_entries, self.max_particles, self.max_jobs)
@@ -227,13 +242,14 @@ class Server:
     def harvest(self, now_fn):
         for job in [j for j in self.live if self.pool.remaining(j) == 0]:
             rid, arrival, admitted, system, nsteps = self.live.pop(job)
-            d = None
+            res = None
             if self.digests:
-                d = digest_state(lambda p: self.pool.state(job, p),
-                                 lambda p: self.pool.connectivity(job, p),
-                                 lambda: self.pool.wall(job), len(system.particles))
+                res = read_results(lambda p: self.pool.state(job, p),
+                                   lambda p: self.pool.connectivity(job, p),
+                                   lambda: self.pool.wall(job), len(system.particles))
             self.pool.remove_job(job)
-            self.done.append(Done(rid, arrival, admitted, now_fn(), nsteps, system.nodes, d))
+            self.done.append(Done(rid, arrival, admitted, now_fn(), nsteps, system.nodes))
+            self.results.append(res)
 
     def serve(self, requests, clock=time.perf_counter):
         """requests: [(arrival seconds from start, system, nsteps)] in arrival order.
@@ -253,6 +269,11 @@ class Server:
                 continue
             self.advance()
             self.harvest(now)
+        # the digests verify the results; they are formed once serving is over
+        for d, res in zip(self.done, self.results):
+            if res is not None:
+                d.digest = digest_arrays(res)
+        self.results = []
         return self.done
 
 

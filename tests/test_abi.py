@@ -62,3 +62,33 @@ def test_library_loads_without_gpu():
     lib, fns = abi.load_gpu_lib()
     for s in abi.SYMBOLS:
         assert s in fns
+
+
+def test_connectivity_read_back_is_one_call_unless_the_bound_is_short():
+    """The wrappers pass the particle's initial bond count as capacity and read
+    the pairs in one call; a library that reports more pairs than that (the
+    capacity check failing) is asked for the count first."""
+    import ctypes as C
+    import numpy as np
+    from peridem_amd import abi
+    pairs = np.array([[0, 1], [0, 2], [1, 3]], dtype=np.uint32)
+    calls = []
+
+    def fake(buf, cap, m):
+        calls.append(cap)
+        m._obj.value = len(pairs)
+        if buf is None:
+            return 0
+        if cap < len(pairs):
+            return -1
+        C.memmove(buf, pairs.ctypes.data, pairs.nbytes)
+        return 0
+
+    def check(rc, what):
+        assert rc == 0, what
+
+    got = abi._read_connectivity(fake, 5, check)
+    assert np.array_equal(got, pairs) and calls == [5]
+    calls.clear()
+    got = abi._read_connectivity(fake, 2, check)
+    assert np.array_equal(got, pairs) and calls == [2, 0, 3]

@@ -55,7 +55,7 @@ def make_server(capacity, chunk):
     srv = serve.Server.__new__(serve.Server)
     srv.max_nodes = srv.max_bond_entries = srv.max_particles = srv.max_jobs = 0
     srv.chunk, srv.digests, srv.prepare_ahead = chunk, False, 0
-    srv.queue, srv.live, srv.done = serve.deque(), {}, []
+    srv.queue, srv.live, srv.done, srv.results = serve.deque(), {}, [], []
     srv.pool = FakePool(capacity)
     srv.steps_run, srv.device_ms, srv.peak_live, srv.workers = 0, 0.0, 0, None
     return srv
