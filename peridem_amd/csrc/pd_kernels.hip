@@ -557,10 +557,11 @@ __device__ bool bonded_alive(const Dev& d, uint64_t g, uint32_t j, bool sorted) 
 // planes, external force, torque, then pair contact in pair order.
 //
 // The bond kernel calls it once the node's row is summed.  Everything it
-// reads is final before the bond kernel starts (positions, velocities,
-// candidate lists, lagged boxes, active pairs, centroids) except the bond
-// state, and of that it reads only node g's own row, which the same thread
-// has just updated -- the post-break row std::find sees (contact.h:783).
+// reads is final before the bond kernel starts (positions, the start-of-step
+// velocities in Vold, candidate lists, lagged boxes, active pairs and wall
+// prefilter bits, centroids) except the bond state, and of that it reads only
+// node g's own row, which the same thread has just updated -- the post-break
+// row std::find sees (contact.h:783).
 //
 // The candidate list of g holds every partner within rb, same particle
 // first (q ascending), then the other particles by global node index (=
