@@ -221,7 +221,6 @@ static void fill_inst(const pd_run_desc& r, DevInst& D) {
   const double h = 0.5 * skin * (1.0 - 1e-6);
   D.du_lim2 = h * h;
   D.du2 = 0;
-  D.rebuild = 0;
   D.dirty = 1;
   D.dt = r.dt;
   D.allow_wall = r.allow_wall;
@@ -358,7 +357,7 @@ static bool rows_sorted(const JobRows& R, uint64_t g0, uint64_t g1) {
   return true;
 }
 
-static bool any_contact_possible(pd_ctx* c) {
+static bool any_contact_possible(const pd_ctx* c) {
   if (c->d.npairs) return true;
   for (auto& J : c->jobs)
     if (J.used && J.run.self_contact && J.run.enable_fracture) return true;
@@ -1146,6 +1145,7 @@ static Dev step_view(const pd_ctx* c) {
   Dev d = c->d;
   d.S = c->hwm_slices;
   d.N = d.S * kSlice;
+  d.lists = any_contact_possible(c) ? 1u : 0u;
   uint32_t T = 1024;
   while (T < d.N && T < c->d.T) T <<= 1;
   d.T = T;
@@ -1201,11 +1201,11 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
   if ((rc = mark(0))) return rc;
   hipLaunchKernelGGL(k_position, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if ((rc = mark(1))) return rc;
-  hipLaunchKernelGGL(k_decide_pairs, dim3(grid_of(std::max({d.ninst, d.npairs, d.P}), 256)), dim3(256), 0, st, d);
+  // pair box test, wall prefilter and (for rebuilding jobs) cell keys
+  hipLaunchKernelGGL(k_cellkey, dim3(grid_of(std::max({d.N, d.ninst, d.npairs, d.P}), 256)), dim3(256), 0, st, d);
   if (any_torque(c)) hipLaunchKernelGGL(k_centroid, dim3(grid_of(d.P, 64)), dim3(64), 0, st, d);
-  if (any_contact_possible(c)) {
+  if (d.lists) {
     // every kernel of the list rebuild exits at once unless a job rebuilds
-    hipLaunchKernelGGL(k_cellkey, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
     const uint32_t nb = grid_of(d.T, kScanBlock);
     hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(kScanBlock), 0, st, d.ctl, d.cstart, d.T, d.scan_tmp);
     hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(kScanBlock), 0, st, d.ctl, d.cstart, d.T, d.scan_tmp, d.ccur);

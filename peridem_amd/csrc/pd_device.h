@@ -51,7 +51,6 @@ struct DevInst {
   double rb;                // candidate-list radius: Rc (1 + 1e-9) + skin
   double du_lim2;           // rebuild when some |u - u_build|^2 exceeds this
   unsigned long long du2;   // max over steps of |u - u_build|^2 since the build (bits of a double >= 0)
-  uint32_t rebuild;         // candidate lists are rebuilt in this step
   uint32_t dirty;           // rebuild at the next step (set by the host)
   double dt;
   double wall[6];           // x_min, y_min, z_min, x_max, y_max, z_max
@@ -143,7 +142,7 @@ struct Dev {
   uint32_t* scan_tmp;
   uint64_t N, S, P, npairs, ninst;   // N = S*64 node slots; P, ninst incl. the void slot
   uint32_t T;        // hash table size (power of two)
-  uint32_t pad;
+  uint32_t lists;    // some job of the pool can have contact: candidate lists are kept
 };
 
 // ---------------------------------------------------------------------------

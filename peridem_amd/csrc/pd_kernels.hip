@@ -2,10 +2,10 @@
 //
 // One step (timeloop.h:395-797) is the launch sequence
 //   k_position   Verlet position half + per-particle box of CurrPos
-//   k_decide_pairs  per job: rebuild the contact candidate lists this step?
-//                and the particle-pair box test on the lagged boxes
-//                (timeloop.h:636-653)
-//   cell list    k_cellkey -> scan -> k_scatter -> k_build
+//   k_cellkey    the particle-pair box test on the lagged boxes
+//                (timeloop.h:636-653), the wall prefilter, and the cell keys
+//                of the nodes of jobs that rebuild their candidate lists
+//   cell list    scan -> k_scatter -> k_build
 //                (only in steps where some job rebuilds its candidate lists)
 //   bond kernel  PD bond force + fused irreversible break, per-node row sums
 //                (k_bond_lds: block tables staged in LDS; k_bond_t: unstaged),
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
   __shared__ unsigned long long smin[3][256], smax[3][256];
   __shared__ uint32_t sp[256];
   // the block's wall-contact flag of this step starts clear, and so does the
-  // step's rebuild flag (set by k_decide_pairs)
+  // step's rebuild flag (set by k_cellkey)
   if (threadIdx.x == 0 && g < d.N) d.bflag[g >> 8] = 0;
   if (g == 0) d.ctl->any_rebuild = 0;
   double cp[3] = {0, 0, 0};
@@ -197,22 +197,13 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
 // built.  While no node of the job has moved more than skin/2 since (less a
 // relative 1e-6 margin for rounding), every pair within R_c is in the lists,
 // so the exact reference tests applied to the list members find exactly the
-// reference's contact pairs.  One thread per job decides, from the largest
-// displacement since the build, whether its lists are rebuilt this step.
-__device__ __forceinline__ void decide_job(Dev& d, uint64_t q) {
-  DevInst& I = d.inst[q];
-  uint32_t r = 0;
-  if (live(I)) {
-    const double du2 = __longlong_as_double((long long)I.du2);
-    r = (I.dirty != 0) || (du2 > I.du_lim2);
-    I.dirty = 0;
-    if (r) {
-      atomicOr(&d.ctl->any_rebuild, 1u);
-      atomicAdd(&d.counters[2], 1ull);
-      I.du2 = 0ull;   // the lists built this step start from the current displacement
-    }
-  }
-  I.rebuild = r;
+// reference's contact pairs.  Whether a job rebuilds its lists this step
+// follows from the largest displacement since the build and the host's
+// dirty flag; every kernel of the step that needs it evaluates it alike (its
+// inputs do not change between k_position and k_finalize, which applies the
+// resets).
+__device__ __forceinline__ bool rebuilds(const DevInst& I) {
+  return live(I) && ((I.dirty != 0) || (__longlong_as_double((long long)I.du2) > I.du_lim2));
 }
 
 // ---------------------------------------------------------------------------
@@ -264,15 +255,6 @@ __device__ __forceinline__ void wall_test(Dev& d, uint64_t q) {
   if (!within_interior_fext(d.lrtb + 6 * q, I.wall, I.Rc)) atomicOr(&d.pactive[q], 4u);
 }
 
-// The three are independent; one launch does all (thread q: job q, pair q,
-// particle q).
-__global__ __launch_bounds__(256) void k_decide_pairs(Dev d) {
-  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < d.ninst) decide_job(d, q);
-  if (q < d.npairs) pair_test(d, q);
-  if (q < d.P) wall_test(d, q);
-}
-
 __device__ __forceinline__ bool self_active(const DevPart& P, const DevInst& I) {
   return P.movable && I.enable_fracture && P.breakable && I.self_contact && P.stoppable;
 }
@@ -281,18 +263,24 @@ __device__ __forceinline__ void cell_of(const double* cp, double cell, int64_t* 
   for (int k = 0; k < 3; k++) c[k] = (int64_t)floor(cp[k] / cell);
 }
 
-// Cell list of the nodes of the jobs that rebuild their candidate lists.
-// The cell table (cstart) is zero when a rebuild starts: cleared at
-// reservation, then by k_finalize after every rebuild.
+// One launch per step for the independent per-step tests (thread q: pair q,
+// particle q, job q) and the cell keys of the nodes of the jobs that rebuild
+// their candidate lists.  The cell table (cstart) is zero when a rebuild
+// starts: cleared at reservation, then by k_finalize after every rebuild.
 __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
-  if (!d.ctl->any_rebuild) return;
-  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < d.npairs) pair_test(d, q);
+  if (q < d.P) wall_test(d, q);
+  if (!d.lists) return;   // no job of the pool can have contact: no lists
+  if (q < d.ninst && rebuilds(d.inst[q])) atomicOr(&d.ctl->any_rebuild, 1u);
+  const uint64_t g = q;
   if (g >= d.N) return;
-  uint32_t p = d.pid[g];
+  const uint32_t p = d.pid[g];
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
+  if (!rebuilds(I)) return;
   uint32_t key = kNoKey;
-  if (live(I) && I.rebuild && (!I.nl_bdry_only || d.brank[g] != kNoKey)) {
+  if (!I.nl_bdry_only || d.brank[g] != kNoKey) {
     double4 x = d.X[g];
     double4 u = disp_of(d, g);
     double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
@@ -350,6 +338,8 @@ __global__ __launch_bounds__(256) void k_scatter(Dev d) {
   if (!d.ctl->any_rebuild) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
+  // only the nodes of rebuilding jobs hold a key of this step
+  if (!rebuilds(d.inst[d.part[d.pid[g]].inst])) return;
   uint32_t key = d.ckey[g];
   if (key == kNoKey) return;
   uint32_t slot = atomicAdd(&d.ccur[key], 1u);
@@ -377,7 +367,7 @@ __device__ __forceinline__ bool build_head(const Dev& d, uint64_t g, uint32_t& p
   p = d.pid[g];
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
-  if (!live(I) || !I.rebuild || (I.nl_bdry_only != 0) != BDRY) return false;
+  if (!rebuilds(I) || (I.nl_bdry_only != 0) != BDRY) return false;
   xi = d.X[g];
   ui = disp_of(d, g);
   d.UBXY[g] = make_double2(ui.x, ui.y);
@@ -1339,6 +1329,11 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d) {
   if (q < d.ninst) {
     DevInst& I = d.inst[q];
     if (live(I)) {
+      if (rebuilds(I)) {
+        atomicAdd(&d.counters[2], 1ull);
+        I.du2 = 0ull;   // the lists built this step start from this step's displacement
+      }
+      I.dirty = 0;
       for (int k = 0; k < 6; k++) I.wall[k] += I.speed[k] * I.dt;
       I.t += 1;
       I.remaining -= 1;
