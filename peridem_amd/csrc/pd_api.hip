@@ -220,7 +220,6 @@ static void fill_inst(const pd_run_desc& r, DevInst& D) {
   D.cell = D.rb;
   const double h = 0.5 * skin * (1.0 - 1e-6);
   D.du_lim2 = h * h;
-  D.du2 = 0;
   D.dirty = 1;
   D.dt = r.dt;
   D.allow_wall = r.allow_wall;
@@ -570,6 +569,7 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(lrtb_next, 6 * P);
   ALLOC(pactive, P);
   ALLOC(counters, 8);
+  ALLOC(du2s, I * kDuSlots * kDuStride);
   ALLOC(err, 1);
   ALLOC(flags, 1);
   ALLOC(ctl, 1);
@@ -616,6 +616,7 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipMemsetAsync(d.lrtb, 0, 6 * P * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.pactive, 0, P * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.counters, 0, 8 * sizeof(unsigned long long), st));
+  HIPCHK(c, hipMemsetAsync(d.du2s, 0, I * kDuSlots * kDuStride * sizeof(unsigned long long), st));
   HIPCHK(c, hipMemsetAsync(d.err, 0, sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.flags, 0, sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.ctl, 0, sizeof(DevCtl), st));
@@ -1024,6 +1025,8 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
   }
   UP(d.inst + jid, &I, 1);
   UP(d.reaction + 18 * jid, zero18, 18);
+  HIPCHK(c, hipMemset(d.du2s + (uint64_t)jid * kDuSlots * kDuStride, 0,
+                      kDuSlots * kDuStride * sizeof(unsigned long long)));
 #undef UP
   if (B->fl) {
     uint32_t cur = 0;

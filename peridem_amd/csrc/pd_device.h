@@ -41,6 +41,13 @@ struct DevPart {
   uint64_t boff;
 };
 
+// The displacement maximum of a job since its candidate lists were built is
+// kept in one slot per XCD (the position kernel's blocks update the slot of
+// the XCD they run on), each slot in its own 128-byte line, so no single
+// word takes the atomics of the whole device.
+constexpr uint32_t kDuSlots = 8;
+constexpr uint32_t kDuStride = 16;   // u64 per slot
+
 // Per-job (independent system) parameters: Contact + RectWall
 // (contact.h:104-183, 185-376), the run switches of run_timeloop, and the
 // job's own clock.  A job with remaining == 0 is frozen (finished, or a free
@@ -50,7 +57,6 @@ struct DevInst {
   double cell;              // cell edge of the candidate search (= rb)
   double rb;                // candidate-list radius: Rc (1 + 1e-9) + skin
   double du_lim2;           // rebuild when some |u - u_build|^2 exceeds this
-  unsigned long long du2;   // max over steps of |u - u_build|^2 since the build (bits of a double >= 0)
   uint32_t dirty;           // rebuild at the next step (set by the host)
   double dt;
   double wall[6];           // x_min, y_min, z_min, x_max, y_max, z_max
@@ -120,6 +126,9 @@ struct Dev {
   unsigned long long* lrtb_next;  // [P*6] ordered-key min/max accumulators
   uint32_t* pactive; // [P] particle has an active contact pair this step
   uint32_t* pairs;   // [npairs*2] particle pairs (i<j) of the same instance
+  // per job, kDuSlots running maxima of |u - u_build|^2 since the list build
+  // (bits of doubles >= 0), one per XCD, each in its own 128-byte line
+  unsigned long long* du2s;
   unsigned long long* counters;   // [0] pair computations
   uint32_t* err;
   uint32_t* flags;   // [1] kFlagSafeMath (sticky)

@@ -74,6 +74,11 @@ __device__ __forceinline__ void set_disp(const Dev& d, uint64_t g, double x, dou
   d.UZ[g] = z;
 }
 
+// slot s of job's displacement maximum since its list build
+__device__ __forceinline__ unsigned long long* du2_slot(const Dev& d, uint64_t job, uint32_t s) {
+  return d.du2s + (job * kDuSlots + s) * kDuStride;
+}
+
 __global__ __launch_bounds__(256) void k_join(Dev d, uint64_t g0, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -142,18 +147,22 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
     cp[2] = x.z + u.z;
     if (!movable) p = 0xFFFFFFFFu;
   }
-  // largest displacement since the candidate lists were built, per job: a
-  // wavefront is one 64-node slice, and a slice belongs to one job
+  // largest displacement since the candidate lists were built, per job: the
+  // maximum over the wavefront, then over the block (a 256-node block
+  // belongs to one job), then one atomic per block into the slot of the XCD
+  // it runs on, only when it exceeds what the slot holds
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned long long o = __shfl_xor(dkey, off, 64);
     const uint32_t oj = __shfl_xor(jid, off, 64);
     dkey = o > dkey ? o : dkey;
     jid = oj < jid ? oj : jid;
   }
-  // du2 is a running maximum since the last build: only a wave that exceeds
-  // the value it reads updates it, so a large job costs few atomics per step
-  if ((threadIdx.x & 63) == 0 && jid != 0xFFFFFFFFu && dkey > d.inst[jid].du2)
-    atomicMax(&d.inst[jid].du2, dkey);
+  __shared__ unsigned long long sdu[4];
+  __shared__ uint32_t sjob[4];
+  if ((threadIdx.x & 63) == 0) {
+    sdu[threadIdx.x >> 6] = dkey;
+    sjob[threadIdx.x >> 6] = jid;
+  }
   // block-level min/max per particle: nodes of one block belong to few
   // particles; reduce runs of equal particle id through LDS.
   int t = threadIdx.x;
@@ -163,6 +172,19 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
     smax[k][t] = (p != 0xFFFFFFFFu) ? okey(cp[k]) : 0ull;
   }
   __syncthreads();
+  if (t == 0) {
+    unsigned long long m = 0ull;
+    uint32_t job = 0xFFFFFFFFu;
+    for (int w = 0; w < 4; w++)
+      if (sjob[w] != 0xFFFFFFFFu) {
+        job = sjob[w];
+        m = sdu[w] > m ? sdu[w] : m;
+      }
+    if (job != 0xFFFFFFFFu) {
+      unsigned long long* slot = du2_slot(d, job, blockIdx.x & (kDuSlots - 1));
+      if (m > *slot) atomicMax(slot, m);
+    }
+  }
   for (int off = 1; off < 256; off <<= 1) {
     unsigned long long mn[3], mx[3];
     bool take = (t + off < 256) && sp[t + off] == p;
@@ -202,8 +224,34 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
 // dirty flag; every kernel of the step that needs it evaluates it alike (its
 // inputs do not change between k_position and k_finalize, which applies the
 // resets).
-__device__ __forceinline__ bool rebuilds(const DevInst& I) {
-  return live(I) && ((I.dirty != 0) || (__longlong_as_double((long long)I.du2) > I.du_lim2));
+__device__ __forceinline__ bool rebuilds(const Dev& d, uint32_t job) {
+  const DevInst& I = d.inst[job];
+  if (!live(I)) return false;
+  unsigned long long m = 0ull;
+#pragma unroll
+  for (uint32_t s = 0; s < kDuSlots; s++) {
+    const unsigned long long v = *du2_slot(d, job, s);
+    m = v > m ? v : m;
+  }
+  return (I.dirty != 0) || (__longlong_as_double((long long)m) > I.du_lim2);
+}
+
+// The same for a node's job, evaluated once per wavefront through the scalar
+// cache when the wavefront's nodes share one job (all but the padding slots
+// at a job's end); the job records and slots are not written by the kernels
+// that ask.
+__device__ __forceinline__ bool rebuilds_node(const Dev& d, uint32_t job) {
+  const uint32_t ju = __builtin_amdgcn_readfirstlane(job);
+  if (__builtin_amdgcn_ballot_w64(job != ju) != 0) return rebuilds(d, job);
+  const DevInst I = ld_uniform(d.inst, ju);
+  if (!live(I)) return false;
+  unsigned long long m = 0ull;
+#pragma unroll
+  for (uint32_t s = 0; s < kDuSlots; s++) {
+    const unsigned long long v = ld_uniform(d.du2s, (ju * kDuSlots + s) * kDuStride);
+    m = v > m ? v : m;
+  }
+  return (I.dirty != 0) || (__longlong_as_double((long long)m) > I.du_lim2);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,13 +320,13 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
   if (q < d.npairs) pair_test(d, q);
   if (q < d.P) wall_test(d, q);
   if (!d.lists) return;   // no job of the pool can have contact: no lists
-  if (q < d.ninst && rebuilds(d.inst[q])) atomicOr(&d.ctl->any_rebuild, 1u);
+  if (q < d.ninst && rebuilds(d, (uint32_t)q)) atomicOr(&d.ctl->any_rebuild, 1u);
   const uint64_t g = q;
   if (g >= d.N) return;
   const uint32_t p = d.pid[g];
   const DevPart& P = d.part[p];
+  if (!rebuilds_node(d, P.inst)) return;
   const DevInst& I = d.inst[P.inst];
-  if (!rebuilds(I)) return;
   uint32_t key = kNoKey;
   if (!I.nl_bdry_only || d.brank[g] != kNoKey) {
     double4 x = d.X[g];
@@ -339,7 +387,7 @@ __global__ __launch_bounds__(256) void k_scatter(Dev d) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
   // only the nodes of rebuilding jobs hold a key of this step
-  if (!rebuilds(d.inst[d.part[d.pid[g]].inst])) return;
+  if (!rebuilds_node(d, d.part[d.pid[g]].inst)) return;
   uint32_t key = d.ckey[g];
   if (key == kNoKey) return;
   uint32_t slot = atomicAdd(&d.ccur[key], 1u);
@@ -367,7 +415,7 @@ __device__ __forceinline__ bool build_head(const Dev& d, uint64_t g, uint32_t& p
   p = d.pid[g];
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
-  if (!rebuilds(I) || (I.nl_bdry_only != 0) != BDRY) return false;
+  if (!rebuilds_node(d, P.inst) || (I.nl_bdry_only != 0) != BDRY) return false;
   xi = d.X[g];
   ui = disp_of(d, g);
   d.UBXY[g] = make_double2(ui.x, ui.y);
@@ -1329,9 +1377,10 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d) {
   if (q < d.ninst) {
     DevInst& I = d.inst[q];
     if (live(I)) {
-      if (rebuilds(I)) {
+      if (rebuilds(d, (uint32_t)q)) {
         atomicAdd(&d.counters[2], 1ull);
-        I.du2 = 0ull;   // the lists built this step start from this step's displacement
+        // the lists built this step start from this step's displacement
+        for (uint32_t s = 0; s < kDuSlots; s++) *du2_slot(d, q, s) = 0ull;
       }
       I.dirty = 0;
       for (int k = 0; k < 6; k++) I.wall[k] += I.speed[k] * I.dt;
