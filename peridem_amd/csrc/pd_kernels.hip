@@ -92,8 +92,6 @@ __global__ __launch_bounds__(256) void k_join(Dev d, uint64_t g0, uint64_t n) {
 // end of the step (timeloop.h:779).
 __global__ __launch_bounds__(256) void k_position(Dev d) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  __shared__ unsigned long long smin[3][256], smax[3][256];
-  __shared__ uint32_t sp[256];
   // the block's wall-contact flag of this step starts clear, and so does the
   // step's rebuild flag (set by k_cellkey)
   if (threadIdx.x == 0 && g < d.N) d.bflag[g >> 8] = 0;
@@ -163,52 +161,90 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
     sdu[threadIdx.x >> 6] = dkey;
     sjob[threadIdx.x >> 6] = jid;
   }
-  // block-level min/max per particle: nodes of one block belong to few
-  // particles; reduce runs of equal particle id through LDS.
-  int t = threadIdx.x;
-  sp[t] = p;
+  // min/max of CurrPos per particle, as order-preserving keys.  Particles
+  // hold contiguous node ranges, so a particle's nodes in a wavefront form
+  // one run: a segmented reduction over the wavefront leaves each run's
+  // extremes in the run's first lane.  The wavefront's first and last runs
+  // may continue in the neighbouring wavefronts: they are merged across the
+  // block's four wavefronts in LDS and published once per run of the block;
+  // runs inside a wavefront publish directly.
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long mn[3], mx[3];
   for (int k = 0; k < 3; k++) {
-    smin[k][t] = (p != 0xFFFFFFFFu) ? okey(cp[k]) : ~0ull;
-    smax[k][t] = (p != 0xFFFFFFFFu) ? okey(cp[k]) : 0ull;
+    mn[k] = (p != 0xFFFFFFFFu) ? okey(cp[k]) : ~0ull;
+    mx[k] = (p != 0xFFFFFFFFu) ? okey(cp[k]) : 0ull;
   }
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t po = __shfl_down(p, off, 64);
+    const bool take = (lane + off < 64) && po == p;
+    for (int k = 0; k < 3; k++) {
+      const unsigned long long a = __shfl_down(mn[k], off, 64), b = __shfl_down(mx[k], off, 64);
+      if (take) {
+        mn[k] = a < mn[k] ? a : mn[k];
+        mx[k] = b > mx[k] ? b : mx[k];
+      }
+    }
+  }
+  const uint32_t pup = __shfl_up(p, 1, 64);
+  const uint32_t p0 = __shfl(p, 0, 64), plast = __shfl(p, 63, 64);
+  const bool head = lane == 0 || pup != p;
+  // per wavefront: entry 0 = its first run, entry 1 = its last run (void if
+  // the wavefront is one run)
+  __shared__ unsigned long long rmin[4][2][3], rmax[4][2][3];
+  __shared__ uint32_t rp[4][2];
+  if (head && p != 0xFFFFFFFFu && p != p0 && p != plast) {   // a run inside the wavefront
+    for (int k = 0; k < 3; k++) {
+      atomicMin(&d.lrtb_next[6 * p + k], mn[k]);
+      atomicMax(&d.lrtb_next[6 * p + 3 + k], mx[k]);
+    }
+  }
+  const int e = lane == 0 ? 0 : 1;
+  if (lane == 0 || (head && p == plast && p != p0)) {
+    rp[w][e] = p;
+    for (int k = 0; k < 3; k++) {
+      rmin[w][e][k] = mn[k];
+      rmax[w][e][k] = mx[k];
+    }
+  }
+  if (lane == 0 && p0 == plast) rp[w][1] = 0xFFFFFFFFu;
   __syncthreads();
-  if (t == 0) {
+  if (threadIdx.x == 0) {
+    // the job's displacement maximum: one atomic per block into its XCD's slot
     unsigned long long m = 0ull;
     uint32_t job = 0xFFFFFFFFu;
-    for (int w = 0; w < 4; w++)
-      if (sjob[w] != 0xFFFFFFFFu) {
-        job = sjob[w];
-        m = sdu[w] > m ? sdu[w] : m;
+    for (int v = 0; v < 4; v++)
+      if (sjob[v] != 0xFFFFFFFFu) {
+        job = sjob[v];
+        m = sdu[v] > m ? sdu[v] : m;
       }
     if (job != 0xFFFFFFFFu) {
       unsigned long long* slot = du2_slot(d, job, blockIdx.x & (kDuSlots - 1));
       if (m > *slot) atomicMax(slot, m);
     }
-  }
-  for (int off = 1; off < 256; off <<= 1) {
-    unsigned long long mn[3], mx[3];
-    bool take = (t + off < 256) && sp[t + off] == p;
-    for (int k = 0; k < 3; k++) {
-      mn[k] = smin[k][t];
-      mx[k] = smax[k][t];
-      if (take) {
-        unsigned long long a = smin[k][t + off], b = smax[k][t + off];
-        mn[k] = a < mn[k] ? a : mn[k];
-        mx[k] = b > mx[k] ? b : mx[k];
+    // the wavefronts' edge runs in node order, equal neighbours merged
+    uint32_t cur = 0xFFFFFFFFu;
+    unsigned long long cmn[3] = {~0ull, ~0ull, ~0ull}, cmx[3] = {0ull, 0ull, 0ull};
+    for (int i = 0; i <= 8; i++) {
+      const uint32_t q = i < 8 ? rp[i >> 1][i & 1] : 0xFFFFFFFEu;   // end marker
+      if (i < 8 && (i & 1) && q == 0xFFFFFFFFu) continue;             // no separate last run
+      if (q != cur) {
+        if (cur != 0xFFFFFFFFu)
+          for (int k = 0; k < 3; k++) {
+            atomicMin(&d.lrtb_next[6 * cur + k], cmn[k]);
+            atomicMax(&d.lrtb_next[6 * cur + 3 + k], cmx[k]);
+          }
+        cur = q == 0xFFFFFFFEu ? 0xFFFFFFFFu : q;
+        for (int k = 0; k < 3; k++) {
+          cmn[k] = ~0ull;
+          cmx[k] = 0ull;
+        }
       }
-    }
-    __syncthreads();
-    for (int k = 0; k < 3; k++) {
-      smin[k][t] = mn[k];
-      smax[k][t] = mx[k];
-    }
-    __syncthreads();
-  }
-  // the first thread of each run of equal ids publishes
-  if (p != 0xFFFFFFFFu && (t == 0 || sp[t - 1] != p)) {
-    for (int k = 0; k < 3; k++) {
-      atomicMin(&d.lrtb_next[6 * p + k], smin[k][t]);
-      atomicMax(&d.lrtb_next[6 * p + 3 + k], smax[k][t]);
+      if (i < 8 && cur != 0xFFFFFFFFu)
+        for (int k = 0; k < 3; k++) {
+          const unsigned long long a = rmin[i >> 1][i & 1][k], b = rmax[i >> 1][i & 1][k];
+          cmn[k] = a < cmn[k] ? a : cmn[k];
+          cmx[k] = b > cmx[k] ? b : cmx[k];
+        }
     }
   }
 }
