@@ -276,9 +276,15 @@ static int check_particles(pd_ctx* c, const pd_particle_desc* P, uint32_t n) {
 
 // f(i) for i < n on up to 16 host threads (one when the job is small);
 // items are handed out one at a time.
+// Job layouts in progress on this process: a caller that lays out several
+// jobs at once on its own threads gets correspondingly fewer threads per
+// layout, instead of every layout starting its own full set.
+static std::atomic<int> g_layouts{0};
+
 template <typename F>
 static void parallel_for(uint64_t n, uint64_t work, F f) {
   unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  nt = std::max(1u, nt / (unsigned)std::max(1, g_layouts.load(std::memory_order_relaxed)));
   if (work < (1u << 20)) nt = 1;
   nt = (unsigned)std::min<uint64_t>(nt, n);
   if (nt <= 1) {
@@ -705,6 +711,10 @@ int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t n
     t_err = tmp.err;
     return rc;
   }
+  g_layouts.fetch_add(1, std::memory_order_relaxed);
+  struct Done {
+    ~Done() { g_layouts.fetch_sub(1, std::memory_order_relaxed); }
+  } done;
   pd_job_blob* B = new pd_job_blob();
   B->run = *run;
   B->np = np;
