@@ -240,7 +240,9 @@ int pd_set_graphs(pd_ctx* ctx, int on);
  * 10 (staged), or 4 while that is not usable.  pd_autotune re-runs that with reps timed launches per shape, returning
  * the choice and (when ms != NULL) the mean time of each shape [count], -1
  * for a shape not usable on this pool; pd_set_bond_shape forces one (-1: tune
- * at the next step); pd_get_bond_shape reports the shape launches use. */
+ * at the next step); pd_get_bond_shape reports the shape launches use.
+ * Shape index pd_bond_shape_count() (never tuned) is the unstaged kernel with
+ * 64-bit record offsets that large pools use; it can be forced on any pool. */
 int pd_bond_shape_count(void);
 int pd_bond_shape_info(int shape, int* unroll, int* pipelined, int* staged);
 int pd_set_bond_shape(pd_ctx* ctx, int shape);

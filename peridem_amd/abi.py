@@ -519,12 +519,24 @@ def bond_shapes():
     built with (pipeline: 0 none, 1 software-pipelined, 2 pipelined and unrolled
     twice; staged: block node tables and static records in LDS)."""
     lib, _ = load_gpu_lib()
-    out = []
-    for k in range(lib.pd_bond_shape_count()):
-        u, p, st = C.c_int(), C.c_int(), C.c_int()
-        lib.pd_bond_shape_info(k, C.byref(u), C.byref(p), C.byref(st))
-        out.append((u.value, p.value, bool(st.value)))
-    return out
+    return [bond_shape_info(k) for k in range(lib.pd_bond_shape_count())]
+
+
+def bond_shape_info(k):
+    """(unroll, pipeline, staged) of bond shape k."""
+    lib, _ = load_gpu_lib()
+    u, p, st = C.c_int(), C.c_int(), C.c_int()
+    if lib.pd_bond_shape_info(k, C.byref(u), C.byref(p), C.byref(st)) != PD_OK:
+        raise ValueError(f"no bond shape {k}")
+    return (u.value, p.value, bool(st.value))
+
+
+def wide_bond_shape():
+    """Shape index of the unstaged bond kernel with 64-bit record offsets
+    (what pools of 2^27 node slots or more use); never tuned, but it can be
+    forced with set_bond_shape on any pool."""
+    lib, _ = load_gpu_lib()
+    return lib.pd_bond_shape_count()
 
 
 def selftest_math(n, seed=1):

@@ -156,7 +156,7 @@ struct pd_ctx {
   pd_profile acc{};
   bool use_graph = true, graph_dirty = true;
   uint64_t hwm_slices = 0;      // end of the highest node range in use
-  int bond_shape = -1;          // index into bond_shapes(); -1: tune at the next step
+  int bond_shape = -1;          // index into bond_shapes() or kWideShape; -1: tune at the next step
   uint32_t lds_bad = 0;         // resident jobs whose block tables exceed kLdsCap
   uint64_t tuned_nodes = 0;     // resident nodes when the shape was tuned
   double tune_ms[kBondShapes] = {};
@@ -1174,9 +1174,16 @@ static bool shape_usable(const pd_ctx* c, int v) {
 }
 
 // The shape a launch uses: the selected one while usable, else the default.
+// kWideShape (one past the tuned shapes) selects bond_wide on any pool.
 static int effective_shape(const pd_ctx* c) {
+  if (c->bond_shape == kWideShape) return kWideShape;
   const int v = c->bond_shape < 0 ? kDefaultBondShape : c->bond_shape;
   return shape_usable(c, v) ? v : kDefaultBondShape;
+}
+
+static BondShape launch_shape(const pd_ctx* c) {
+  const int v = effective_shape(c);
+  return wide_pool(c) || v == kWideShape ? bond_wide() : bond_shapes()[v];
 }
 
 // Phase times of one profiled step (waits for its last mark).
@@ -1231,7 +1238,7 @@ static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
   // wall, external and torque terms and takes the Verlet velocity half;
   // grid padded to a multiple of the XCD count (the bond kernel remaps
   // blocks)
-  hipLaunchKernelGGL(wide_pool(c) ? bond_wide().fn : bond_shapes()[effective_shape(c)].fn,
+  hipLaunchKernelGGL(launch_shape(c).fn,
                      dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
   if ((rc = mark(3))) return rc;
   if (any_reaction(c)) hipLaunchKernelGGL(k_reaction, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
@@ -1440,16 +1447,17 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
 int pd_bond_shape_count(void) { return kBondShapes; }
 
 int pd_bond_shape_info(int shape, int* unroll, int* pipelined, int* staged) {
-  if (shape < 0 || shape >= kBondShapes) return PD_E_ARG;
-  if (unroll) *unroll = bond_shapes()[shape].unroll;
-  if (pipelined) *pipelined = bond_shapes()[shape].pipe;
-  if (staged) *staged = bond_shapes()[shape].lds;
+  if (shape < 0 || shape > kWideShape) return PD_E_ARG;
+  const BondShape b = shape == kWideShape ? bond_wide() : bond_shapes()[shape];
+  if (unroll) *unroll = b.unroll;
+  if (pipelined) *pipelined = b.pipe;
+  if (staged) *staged = b.lds;
   return PD_OK;
 }
 
 int pd_set_bond_shape(pd_ctx* c, int shape) {
   if (!c) return PD_E_ARG;
-  if (shape < -1 || shape >= kBondShapes) return fail(c, PD_E_ARG, "no bond shape %d", shape);
+  if (shape < -1 || shape > kWideShape) return fail(c, PD_E_ARG, "no bond shape %d", shape);
   c->bond_shape = shape;
   c->graph_dirty = true;
   return PD_OK;

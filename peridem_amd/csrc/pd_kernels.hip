@@ -1313,6 +1313,9 @@ __host__ inline const BondShape* bond_shapes() {
 // The default shape with 64-bit record addressing, for pools too large for
 // 32-bit offsets (never tuned).
 __host__ inline BondShape bond_wide() { return {k_bond_t<4, 2, false>, 4, 2, 0}; }
+// Shape index of bond_wide, selectable on any pool so that the 64-bit form
+// can be checked on small ones.
+constexpr int kWideShape = kBondShapes;
 
 // ---------------------------------------------------------------------------
 // Displacement of the clamped nodes (timeloop.h:767-774), after the bond

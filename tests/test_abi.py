@@ -64,6 +64,20 @@ def test_library_loads_without_gpu():
         assert s in fns
 
 
+def test_bond_shape_table_ends_with_the_wide_shape():
+    """The tuned shapes come first; the index one past them is the unstaged
+    kernel with 64-bit record offsets; the index after that is no shape.
+    Table queries touch no device."""
+    shapes = abi.bond_shapes()
+    k = abi.wide_bond_shape()
+    assert k == len(shapes) and any(sh[2] for sh in shapes)
+    assert abi.bond_shape_info(k) == (4, 2, False)
+    with pytest.raises(ValueError):
+        abi.bond_shape_info(k + 1)
+    with pytest.raises(ValueError):
+        abi.bond_shape_info(-1)
+
+
 def test_connectivity_read_back_is_one_call_unless_the_bound_is_short():
     """The wrappers pass the particle's initial bond count as capacity and read
     the pairs in one call; a library that reports more pairs than that (the

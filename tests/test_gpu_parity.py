@@ -137,6 +137,27 @@ def test_every_bond_shape_matches_oracle(oracle_lib):
             g.close()
 
 
+def test_wide_bond_shape_matches_oracle(oracle_lib):
+    """The 64-bit-offset bond kernel of large pools, forced on a small one,
+    gives the oracle's bits through a run that breaks bonds."""
+    s = small_impact(seed=21)
+    k = abi.wide_bond_shape()
+    assert abi.bond_shape_info(k) == (4, 2, False)
+    print(f"[parity] wide bond shape {k}", flush=True)
+    g = abi.gpu_engine(s)
+    o = oracle_lib.engine(s)
+    try:
+        g.set_bond_shape(k)
+        g.step(72)
+        o.step(72)
+        assert g.bond_shape() == k
+        compare_engines(g, o, s, "wide shape")
+        assert g.counters()["alive_bonds"] < g.counters()["total_bonds"], "no bond broke"
+    finally:
+        g.close()
+        o.close()
+
+
 def test_switching_between_staged_and_global_shapes_keeps_breaks(oracle_lib):
     """Both forms of a bond entry (pool index and position in the block's
     node table) carry the broken flag: alternating staged and unstaged shapes
