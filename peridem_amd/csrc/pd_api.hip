@@ -681,7 +681,7 @@ struct pd_job_blob {
   std::vector<HostPart> parts;    // off job-relative
   std::vector<uint64_t> soff;     // relative to the job's first bond entry
   std::vector<uint32_t> bond;     // job-relative node indices, kPad padding
-  std::vector<uint16_t> bloc;     // the entries as positions in their block's table, kPad16 padding
+  std::vector<uint16_t> bloc;     // the entries as 16 * positions in their block's table, kPad16 padding
   std::vector<uint32_t> btab;     // per block, kLdsCap slots: sorted job-relative node indices
   std::vector<uint2> binfo;       // per block: table length, position of the block's first node
   std::vector<uint8_t> lane_node, node_lane;   // see JobRows
@@ -901,7 +901,7 @@ int pd_job_prepare(const pd_run_desc* run, const pd_particle_desc* P, uint32_t n
       const uint64_t t = (g & ~(uint64_t)255) + R.node_lane[g];
       uint16_t* col = B->bloc.data() + B->soff[t / kSlice] + (t % kSlice);
       for (uint64_t k = R.rstart[g]; k < R.rstart[g + 1]; k++)
-        col[(k - R.rstart[g]) * kSlice] = (uint16_t)pos(R.rows[k]);
+        col[(k - R.rstart[g]) * kSlice] = (uint16_t)(16 * pos(R.rows[k]));
     }
     std::copy(t.begin(), t.end(), B->btab.begin() + b * kLdsCap);
     B->binfo[b] = make_uint2((uint32_t)t.size(), pos((uint32_t)n0));

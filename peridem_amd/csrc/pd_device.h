@@ -15,11 +15,14 @@ constexpr int kSlice = 64;                  // nodes per bond slice (= wavefront
 // Node blocks of the staged bond kernel: 4 slices = one 256-thread workgroup.
 // Jobs occupy whole blocks.  Each block keeps the sorted table of the nodes
 // its rows reference (its own nodes included), at most kLdsCap of them, and
-// each bond entry also exists as a u16 position in that table.
+// each bond entry also exists as a u16: 16 * its position in that table (the
+// byte offset of the node's 16-byte LDS records), or kDead16.
 constexpr int kBlockSlices = 4;
 constexpr uint32_t kLdsCap = 2240;          // 2240 * (16 + 16 + 4) B = 80 KB of LDS per workgroup
-constexpr uint32_t kDead16 = 0x8000u;       // high bit of a u16 entry: bond erased
+constexpr uint32_t kLive16 = 16 * kLdsCap;  // u16 entries below this are live table offsets
+constexpr uint32_t kDead16 = 0xFFFFu;       // u16 entry of an erased bond
 constexpr uint32_t kPad16 = 0xFFFFu;        // u16 slice padding entry
+static_assert(kLive16 <= kDead16, "table offsets must fit a u16 below the dead value");
 constexpr int kMaxCand = 48;                // contact candidates kept per node (list capacity)
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 

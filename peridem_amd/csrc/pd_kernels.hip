@@ -987,33 +987,43 @@ struct GlobalRows {
   // both forms of the entry carry the broken flag
   __device__ __forceinline__ void broke(uint32_t k, uint32_t e) const {
     *R.at(k) = e | kBroken;
-    uint16_t* q = R16.at(k);
-    *q = (uint16_t)(*q | kDead16);
+    *R16.at(k) = (uint16_t)kDead16;
   }
 };
 
-// LdsRows: the u16 entries (position in the 256-node block's table of the
-// nodes its rows reference, bit 15 = broken); the block's table and the
-// static part of those nodes' records (x, y, z, vol) staged in LDS once per
-// launch, the displacement gathered from the split copies through the
-// table.  A dead entry uses the node's own position in the table.  The
-// table lookup (resolve) runs one pipeline stage before the gather, so the
-// displacement loads issue without waiting on LDS.
-__shared__ double2 s_xy[kLdsCap], s_zv[kLdsCap];
-__shared__ uint32_t s_id[kLdsCap];   // 8 * pool node index: the byte offset of its UZ record
+// LdsRows: the u16 entries (16 * position in the 256-node block's table of
+// the nodes its rows reference, i.e. the byte offset of the node's 16-byte
+// LDS records; kDead16 = broken); the block's table and the static part of
+// those nodes' records (x, y, z, vol) staged in LDS once per launch, the
+// displacement gathered from the split copies through the table.  A dead
+// entry uses the node's own records.  The table lookup (resolve) runs one
+// pipeline stage before the gather, so the displacement loads issue without
+// waiting on LDS.
+// One LDS object, the index table first, so that every table's offset fits
+// the 16-bit immediate of an LDS load and no base register is needed.
+struct LdsTables {
+  uint32_t id[kLdsCap];   // 8 * pool node index: the byte offset of its UZ record
+  double2 xy[kLdsCap];
+  double2 zv[kLdsCap];
+};
+static_assert(sizeof(LdsTables) - sizeof(double2) * kLdsCap < 65536, "LDS table offsets exceed 16 bits");
+__shared__ LdsTables s_tab;
 struct LdsRows {
   const Dev& d;
   SliceRows<uint16_t> R16;
   SliceRows<uint32_t> R;
-  uint32_t self;   // the node's position in the table
+  uint32_t self16;   // 16 * the node's position in the table
   static constexpr uint32_t kPadE = kPad16;
   __device__ __forceinline__ const SliceRows<uint16_t>& rows() const { return R16; }
-  __device__ __forceinline__ bool alive(uint32_t e) const { return (e & kDead16) == 0; }
-  __device__ __forceinline__ uint32_t pos(uint32_t e) const { return e < kDead16 ? e : self; }
+  __device__ __forceinline__ bool alive(uint32_t e) const { return e < kLive16; }
+  __device__ __forceinline__ uint32_t off(uint32_t e) const { return e < kLive16 ? e : self16; }
+  __device__ __forceinline__ static uint32_t id_of(uint32_t off16) {
+    return *(const uint32_t*)((const char*)s_tab.id + (off16 >> 2));
+  }
   template <int U>
   __device__ __forceinline__ void resolve(const uint32_t (&e)[U], uint32_t (&id8)[U]) const {
 #pragma unroll
-    for (int u = 0; u < U; u++) id8[u] = s_id[pos(e[u])];
+    for (int u = 0; u < U; u++) id8[u] = id_of(off(e[u]));
   }
   template <int U>
   __device__ __forceinline__ void gather(const uint32_t (&e)[U], const uint32_t (&id8)[U], double4 (&xj)[U],
@@ -1023,14 +1033,15 @@ struct LdsRows {
       const double2 uxy = *(const double2*)((const char*)d.UXY + (id8[u] << 1));
       const double uz = *(const double*)((const char*)d.UZ + id8[u]);
       uj[u] = make_double4(uxy.x, uxy.y, uz, 0.0);
-      const uint32_t t = pos(e[u]);
-      const double2 xy = s_xy[t], zv = s_zv[t];
+      const uint32_t t = off(e[u]);
+      const double2 xy = *(const double2*)((const char*)s_tab.xy + t);
+      const double2 zv = *(const double2*)((const char*)s_tab.zv + t);
       xj[u] = make_double4(xy.x, xy.y, zv.x, zv.y);
     }
   }
   __device__ __forceinline__ void broke(uint32_t k, uint32_t e) const {
-    *R.at(k) = (s_id[e] >> 3) | kBroken;
-    *R16.at(k) = (uint16_t)(e | kDead16);
+    *R.at(k) = (id_of(e) >> 3) | kBroken;
+    *R16.at(k) = (uint16_t)kDead16;
   }
 };
 
@@ -1227,9 +1238,10 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
 // stages its node table and those nodes' static records in LDS (80 KB: two
 // workgroups per CU), then every gather of x, y, z, vol is an LDS read and
 // only the displacement is gathered from L2.  Used when every job of the
-// pool has block tables of at most kLdsCap nodes.
+// pool has block tables of at most kLdsCap nodes.  The tables hold a CU to
+// two workgroups (two waves per SIMD), so registers are allotted for two.
 template <int U, int PIPE>
-__global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
+__global__ __launch_bounds__(256, 2) void k_bond_lds(Dev d) {
   const uint64_t blk = xcd_block();
   const uint64_t t = blk * blockDim.x + threadIdx.x;   // slot
   {
@@ -1256,9 +1268,9 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
       for (uint32_t k = 0; k < kStage; k++) {
         const uint32_t i = i0 + k * blockDim.x;
         if (i < cnt) {
-          s_id[i] = id[k] << 3;
-          s_xy[i] = xy[k];
-          s_zv[i] = zv[k];
+          s_tab.id[i] = id[k] << 3;
+          s_tab.xy[i] = xy[k];
+          s_tab.zv[i] = zv[k];
         }
       }
     }
@@ -1277,8 +1289,9 @@ __global__ __launch_bounds__(256) void k_bond_lds(Dev d) {
     slice_of(d, t, base, W);
     const uint32_t lane = (uint32_t)(t & 63);
     const uint32_t self = d.binfo[blk].y + (uint32_t)(g - blk * blockDim.x);
-    const LdsRows S{d, {(const char*)(d.bloc + base), lane * 2u}, {(const char*)(d.bond + base), lane * 4u}, self};
-    const double2 xy = s_xy[self], zv = s_zv[self];
+    const LdsRows S{d, {(const char*)(d.bloc + base), lane * 2u}, {(const char*)(d.bond + base), lane * 4u},
+                    16 * self};
+    const double2 xy = s_tab.xy[self], zv = s_tab.zv[self];
     const double4 xi = make_double4(xy.x, xy.y, zv.x, zv.y);
     double4 ui;
     rec_u<true>(d, (uint32_t)g, ui);
