@@ -262,6 +262,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (a plain copy "
                          "kernel of known size that calibrates FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--bond-shape", type=int, default=-1,
+                    help="force this bond-kernel shape instead of the autotuner's choice "
+                         "(profiling passes pin one so that every pass times the same kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -284,6 +287,8 @@ def main():
         f"(generated in {time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
     e = abi.gpu_engine(s)
+    if args.bond_shape >= 0:
+        e.set_bond_shape(args.bond_shape)
     log(f"[bench] committed to the GPU in {time.perf_counter() - t0:.1f} s")
     e.step(args.warmup)
     # same number of steps replayed from the step graphs, no marks: logged

@@ -90,8 +90,21 @@ def counters(d, kernel=None):
             print(f"  {c:32s} {sum(v) / len(v):18.1f}")
 
 
+def _steady(dirs, counter):
+    """Per kernel, the dispatches of the pass (of several, comma-separated
+    directories) in which it ran most: a pass that pins the bond-kernel shape
+    gives that shape's step launches, not the autotuner's back-to-back ones,
+    which find their inputs still cached."""
+    out = {}
+    for d in dirs.split(","):
+        for k, cs in _agg(d).items():
+            if counter in cs and len(cs[counter]) > len(out.get(k, {}).get(counter, [])):
+                out[k] = cs
+    return out
+
+
 def traffic(fd, wd, copy_mib, nodes, bonds):
-    f, w = _agg(fd), _agg(wd)
+    f, w = _steady(fd, "FETCH_SIZE"), _steady(wd, "WRITE_SIZE")
     copy_bytes = copy_mib * 1024 * 1024
     # the plain copy: the largest copyBuffer dispatch
     cf = max(f["__amd_rocclr_copyBuffer"]["FETCH_SIZE"])
@@ -103,7 +116,7 @@ def traffic(fd, wd, copy_mib, nodes, bonds):
     print("|---|---|---|---|")
     res = {}
     for k in f:
-        if not k.startswith("pd::"):
+        if not k.startswith("pd::") or k not in w:
             continue
         fb = sum(f[k]["FETCH_SIZE"]) / len(f[k]["FETCH_SIZE"]) * 1024 * fs
         wb = sum(w[k]["WRITE_SIZE"]) / len(w[k]["WRITE_SIZE"]) * 1024 * ws
@@ -123,7 +136,7 @@ FP64_COUNTERS = ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VAL
 
 
 def fp64(d):
-    a = _agg(d)
+    a = _steady(d, FP64_COUNTERS[0])
     print("| kernel | ADD | MUL | FMA | TRANS | FP64 wave-instructions per launch | all VALU |")
     print("|---|---|---|---|---|---|---|")
     res = {}
