@@ -475,3 +475,131 @@ def test_velocity_constraint_off_changes_nothing(oracle_lib):
         assert np.array_equal(a.state(0)[k], b.state(0)[k]), k
     a.close()
     b.close()
+
+
+# ---------------------------------------------------------------- wall friction and damping
+def _wall_cap(Rc, d):
+    """Spherical cap of a node's contact sphere cut by a wall at distance d:
+    volume pi/3 (R-d)^2 (2R-d) and centroid distance from the centre
+    pi/4 (R^2-d^2)^2 / volume (contact.h:14-39, textbook solid of revolution)."""
+    area = math.pi / 3 * (Rc - d) ** 2 * (2 * Rc - d)
+    cen = math.pi / 4 * (Rc * Rc - d * d) ** 2 / area
+    return cen, area
+
+
+@pytest.mark.parametrize("v,capped", [((3.0, 0.0, -4.0), False), ((3e-3, 0.0, -4e-3), True)])
+def test_wall_friction_tangents_cap_and_reaction(oracle_lib, v, capped):
+    """A node 3e-5 below the y_max wall sliding along it (contact.h:440-486):
+    the friction opposes the sliding direction in the plane of the two
+    tangents (z, x) = ((dir+1)%3, (dir+2)%3) with magnitude mu |Fn|, or, when
+    the sliding speed is below mu |Fn| dt / rho, the reference's small-velocity
+    value |v_t| / rho / dt; the y_max reaction carries -vol times each term."""
+    Rc, Kn, mu, rho, dt, vol = 9e-5, 4.6e27, 0.3, 2440.0, 1e-8, 2e-12
+    hi = 1e-3
+    p = one_particle([[5e-4, hi - 3e-5, 5e-4]], vol=[vol])
+    p.rho = rho
+    p.vel = np.array([v], float)
+    s = system([p], allow_wall=1, compute_wall_reaction=1, allow_friction=1,
+               friction_coefficient=mu, normal_stiffness=Kn, wall_lrtb=(0, 0, 0, hi, hi, hi))
+    e, st = force_after_one_step(oracle_lib, s)
+    cen, area = _wall_cap(Rc, 3e-5)
+    Fn = Kn * (Rc - cen) * area
+    vt = np.array([v[0], 0.0, v[2]])                  # tangential velocity (the wall is at rest)
+    speed = np.linalg.norm(vt)
+    assert (speed < mu * Fn * dt / rho) == capped
+    mag = speed / rho / dt if capped else mu * Fn
+    want = -mag * vt / speed + np.array([0.0, -Fn, 0.0])   # below the wall: pushed down in y
+    f = st[0]["force"][0]
+    assert np.allclose(f, want, rtol=1e-12, atol=0)
+    assert f[0] < 0 < f[2]                             # opposite to (+x, -z) sliding
+    _, reac = e.wall()
+    assert np.allclose(reac[4], -vol * want, rtol=1e-12, atol=0)      # y_max plane
+    assert np.all(reac[[0, 1, 2, 3, 5]] == 0)
+    e.close()
+
+
+def test_wall_damping_against_a_moving_wall(oracle_lib):
+    """A node approaching the z_min wall while that wall moves up
+    (contact.h:488-498): damping -(2 zeta sqrt(Kn rho) sqrt(A)) (v_z - w)
+    opposes the velocity relative to the wall, the contact distance is the one
+    after the position half-step, and the z_min reaction is -vol (Fn + f_d)."""
+    Rc, Kn, zeta, rho, dt, vol = 9e-5, 4.6e27, 0.1, 2440.0, 1e-8, 3e-12
+    vz, w = -3.0, 2.0
+    p = one_particle([[5e-4, 5e-4, 3e-5]], vol=[vol])
+    p.rho = rho
+    p.vel = np.array([[0.0, 0.0, vz]])
+    s = system([p], allow_wall=1, compute_wall_reaction=1, allow_damping=1, allow_friction=1,
+               damping_ratio=zeta, normal_stiffness=Kn, wall_lrtb=(0, 0, 0, 1e-3, 1e-3, 1e-3),
+               wall_speed=(0, 0, w, 0, 0, 0))
+    e, st = force_after_one_step(oracle_lib, s)
+    d = 3e-5 + dt * vz                                 # after the Verlet position half
+    cen, area = _wall_cap(Rc, d)
+    Fn = Kn * (Rc - cen) * area
+    fd = -(2 * zeta * math.sqrt(Kn * rho) * math.sqrt(area)) * (vz - w)
+    assert fd > 0 and fd > 1e-3 * Fn                   # the term is not negligible
+    f = st[0]["force"][0]
+    assert f[2] == pytest.approx(Fn + fd, rel=1e-12)
+    assert f[0] == 0 and f[1] == 0                     # no tangential motion: no friction
+    _, reac = e.wall()
+    assert reac[2, 2] == pytest.approx(-vol * (Fn + fd), rel=1e-12)
+    assert np.all(reac[2, :2] == 0) and np.all(reac[[0, 1, 3, 4, 5]] == 0)
+    # the wall moved by w dt after the step (timeloop.h:784-797)
+    lrtb, _ = e.wall()
+    assert lrtb[2] == pytest.approx(w * dt, rel=1e-15)
+    e.close()
+
+
+# ---------------------------------------------------------------- external force ramp
+@pytest.mark.parametrize("first", [1, 3])
+def test_gradient_extforce_ramp(oracle_lib, first):
+    """gradient_extforce (timeloop.h:581-592): the external force density of
+    step t is min(t / extforce_maxstep, 1) times extforce, t counting from the
+    run's first step (1 for a fresh run).  With maxstep 4 and extforce -8 the
+    applied values are -2, -4, -6, -8, -8, ... exactly."""
+    p = one_particle([[0, 0, 0]])
+    p.ext = np.array([[0.0, 0.0, -8.0]])
+    s = system([p], gradient_extforce=1, extforce_maxstep=4, first_step=first)
+    e = oracle_lib.engine(s)
+    got = []
+    for _ in range(6):
+        e.step(1)
+        f = e.state(0)["force"][0]
+        assert f[0] == 0 and f[1] == 0
+        got.append(f[2])
+    want = [-2.0 * min(t, 4) for t in range(first, first + 6)]
+    assert got == want
+    # with the ramp off the full force acts from the first step
+    e2 = oracle_lib.engine(system([p], gradient_extforce=0, extforce_maxstep=4))
+    e2.step(1)
+    assert e2.state(0)["force"][0, 2] == -8.0
+    e.close()
+    e2.close()
+
+
+# ---------------------------------------------------------------- self contact, |xi| >= Rc
+def test_self_contact_beyond_contact_radius_uses_the_contact_spring(oracle_lib):
+    """Unbonded nodes of one particle whose reference distance |xi| is at
+    least R_c (contact.h:789-791) interact as a pair contact does: reference
+    length R_c and spring K_n R_c -- not |xi| and c0, the |xi| < R_c branch.
+    Each node takes its own action term (self-contact has no reaction)."""
+    Rc, Kn = 9e-5, 4.6e31
+    xi = 1e-4
+    p = one_particle([[0, 0, 0], [xi, 0, 0]], vol=[2e-12, 5e-12])
+    p.disp = np.array([[0.0, 0, 0], [-4e-5, 0, 0]])    # now 6e-5 apart
+    p.snot = 1.0
+    s = system([p], self_contact=1, enable_fracture=1, normal_stiffness=Kn)
+    _, st = force_after_one_step(oracle_lib, s)
+    f = st[0]["force"]
+    dn = 6e-5
+    K = Kn * Rc
+    assert K != pytest.approx(p.cnot, rel=0.5)          # the branches differ
+    f0 = K * ((dn - Rc) / Rc) * 5e-12                   # on node 0 along +x (repulsive: < 0)
+    f1 = K * ((dn - Rc) / Rc) * 2e-12                   # on node 1 along -x
+    assert f[0, 0] == pytest.approx(f0, rel=1e-13) and f0 < 0
+    assert f[1, 0] == pytest.approx(-f1, rel=1e-13)
+    assert np.all(f[:, 1:] == 0)
+    # beyond R_c now: no force at all
+    p.disp = np.array([[0.0, 0, 0], [-0.5e-5, 0, 0]])
+    _, st = force_after_one_step(oracle_lib, system([p], self_contact=1, enable_fracture=1,
+                                                    normal_stiffness=Kn))
+    assert np.all(st[0]["force"] == 0)
