@@ -1314,7 +1314,9 @@ static int autotune(pd_ctx* c, uint32_t reps) {
     t.wmask = WM;
     t.bflag = BF;
     const dim3 grid((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds);
+    // fastest timed shape; the default shape if none is usable
     int best = kDefaultBondShape;
+    double best_ms = HUGE_VAL;
     for (int v = 0; v < kBondShapes; v++) {
       if (!shape_usable(c, v)) {
         c->tune_ms[v] = -1.0;   // not timed
@@ -1329,7 +1331,10 @@ static int autotune(pd_ctx* c, uint32_t reps) {
       float ms = 0;
       HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
       c->tune_ms[v] = ms / reps;
-      if (c->tune_ms[v] < c->tune_ms[best]) best = v;
+      if (c->tune_ms[v] < best_ms) {
+        best_ms = c->tune_ms[v];
+        best = v;
+      }
     }
     c->bond_shape = best;
     return PD_OK;
@@ -1909,10 +1914,10 @@ int pd_get_profile(pd_ctx* c, pd_profile* out) {
 int pd_selftest_math(uint64_t n, uint64_t seed, uint64_t* mismatch_div, uint64_t* mismatch_sqrt) {
   if (!mismatch_div || !mismatch_sqrt) return PD_E_ARG;
   unsigned long long* dm = nullptr;
-  if (hipMalloc((void**)&dm, 2 * sizeof(unsigned long long)) != hipSuccess) return PD_E_HIP;
+  if (hipMalloc((void**)&dm, 3 * sizeof(unsigned long long)) != hipSuccess) return PD_E_HIP;
   int rc = PD_OK;
-  unsigned long long hm[2] = {0, 0};
-  if (hipMemset(dm, 0, 2 * sizeof(unsigned long long)) != hipSuccess) rc = PD_E_HIP;
+  unsigned long long hm[3] = {0, 0, 0};
+  if (hipMemset(dm, 0, 3 * sizeof(unsigned long long)) != hipSuccess) rc = PD_E_HIP;
   if (rc == PD_OK) {
     hipLaunchKernelGGL(k_selftest_math, dim3(2048), dim3(256), 0, 0, n, seed, dm);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||

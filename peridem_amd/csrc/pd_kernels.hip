@@ -1541,11 +1541,51 @@ __device__ __forceinline__ double make_double(uint64_t mant, int e, int neg) {
   uint64_t b = ((uint64_t)(e + 1023) << 52) | (mant & 0xFFFFFFFFFFFFFull) | ((uint64_t)neg << 63);
   return __longlong_as_double((long long)b);
 }
+// Hard cases of correct rounding.  For an odd integer B of b bits let
+// A = +-(2^b)^-1 mod B, lifted into [2^52, 2^53) by multiples of B.  Then
+// A 2^b = n B +- 1, so the quotient A 2^(b-53) / B = n 2^-53 +- 2^-53 / B:
+// for a quotient in [1, 2) and odd n that is a rounding midpoint missed by
+// about 2^-106 relative (Euclid's inverse, no rounding anywhere).
+__device__ uint64_t inv_mod_odd(uint64_t a, uint64_t m) {   // a^-1 mod m, gcd(a, m) = 1
+  int64_t t = 0, nt = 1;
+  uint64_t r = m, nr = a;
+  while (nr != 0) {
+    const uint64_t q = r / nr;
+    const int64_t tt = t - (int64_t)q * nt;
+    t = nt;
+    nt = tt;
+    const uint64_t rr = r - q * nr;
+    r = nr;
+    nr = rr;
+  }
+  return t < 0 ? (uint64_t)(t + (int64_t)m) : (uint64_t)t;
+}
+__device__ uint64_t midpoint_numerator(uint64_t B, int b, bool below) {
+  uint64_t A = inv_mod_odd((1ull << b) % B, B);
+  if (below) A = B - A;
+  if (A < (1ull << 52)) {
+    const uint64_t lifted = A + ((1ull << 52) - A + B - 1) / B * B;
+    if (lifted < (1ull << 53)) A = lifted;   // else keep A: still A 2^b = n B +- 1
+  }
+  return A;
+}
+
+// Each sample i checks, bit for bit against the IEEE operators:
+//  random operands spanning the fast domain -- div_core(a, rcp_core(b)),
+//  sqrt_core(x), sqrt_rcp_core(x).s and division through its reciprocal;
+//  hard operands -- quotients within ~2^-106 of a rounding midpoint, by
+//  divisors with random, all-ones-like and near-power-of-two mantissas and
+//  by computed roots (through sqrt_rcp_core), and square roots within
+//  d^2 2^-107 of a midpoint: sqrt(X 2^54) for X = 2^52 + d (+1), d odd,
+//  lies just below (above) the odd 54-bit integer 2^53 + d.
+// mism[0] += division mismatches, mism[1] += sqrt mismatches,
+// mism[2] += samples evaluated (n in total).
 __global__ __launch_bounds__(256) void k_selftest_math(uint64_t n, uint64_t seed,
                                                        unsigned long long* mism) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t bad_div = 0, bad_sqrt = 0;
+  uint32_t bad_div = 0, bad_sqrt = 0, done = 0;
   for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    done++;
     uint64_t r0 = mix64(seed ^ (i * 4 + 0)), r1 = mix64(seed ^ (i * 4 + 1));
     uint64_t r2 = mix64(seed ^ (i * 4 + 2)), r3 = mix64(seed ^ (i * 4 + 3));
     // denominator in [2^-300, 2^301): |e| or |xi|
@@ -1580,9 +1620,49 @@ __global__ __launch_bounds__(256) void k_selftest_math(uint64_t n, uint64_t seed
     const bool eq2 = (a2 == 0.0) ? (q2 == 0.0 && q2i == 0.0)
                                  : (__double_as_longlong(q2) == __double_as_longlong(q2i));
     bad_div += eq2 ? 0u : 1u;
+
+    // --- hard cases
+    const uint64_t r5 = mix64(seed ^ ~(i * 4 + 2)), r6 = mix64(seed ^ ~(i * 4 + 3));
+    const uint64_t m53 = (1ull << 53) - 1;
+    uint64_t B;
+    switch (r5 & 3) {
+      case 0: B = m53 - 2 * ((r5 >> 2) & 0xFFFF); break;           // all ones, nearly
+      case 1: B = (1ull << 52) + 1 + 2 * ((r5 >> 2) & 0xFFFF); break; // just above 2^52
+      default: B = (((r5 >> 2) & m53) | (1ull << 52) | 1ull) & m53; break;
+    }
+    const uint64_t A = midpoint_numerator(B, 53, (r5 >> 62) & 1);
+    const int hb = (int)(r6 % 601) - 300;
+    const int ha = hb - (int)((r6 >> 16) % 653);
+    const double hbv = ldexp((double)B, hb - 52);
+    // exponent ha whatever A's length (A = 1 for B = 2^53 - 1, where
+    // 2^53 = 1 mod B): numerators stay inside the fast domain
+    double hav = ldexp((double)A, ha - (63 - __builtin_clzll(A)));
+    if ((r6 >> 63) & 1) hav = -hav;
+    bad_div += (__double_as_longlong(div_core(hav, rcp_core(hbv))) ==
+                __double_as_longlong(hav / hbv)) ? 0u : 1u;
+    // by a computed root: B = the root's mantissa without trailing zeros
+    const uint64_t sm = ((uint64_t)__double_as_longlong(sx) & ((1ull << 52) - 1)) | (1ull << 52);
+    const int tz = __builtin_ctzll(sm);
+    const uint64_t RB = sm >> tz;
+    if (RB > 1) {
+      const uint64_t RA = midpoint_numerator(RB, 53 - tz, (r6 >> 62) & 1);
+      double ra = ldexp((double)RA, es + 1 - (int)((r6 >> 32) % 653) - (63 - __builtin_clzll(RA)));
+      if ((r5 >> 61) & 1) ra = -ra;
+      bad_div += (__double_as_longlong(div_core(ra, sr.r)) == __double_as_longlong(ra / sx)) ? 0u : 1u;
+    }
+    // square roots next to a midpoint
+    const uint64_t dd = ((r5 >> 20) & 0xFFFFF) | 1ull;
+    const uint64_t X = (1ull << 52) + dd + ((r5 >> 40) & 1);
+    const int k2 = (int)((r6 >> 40) % 600) - 353;     // x = X 2^(54 + 2 k2), exponent in [-600, 600]
+    const double hx = ldexp((double)X, 54 + 2 * k2);
+    const double hs = sqrt(hx);
+    bad_sqrt += (__double_as_longlong(sqrt_core(hx)) == __double_as_longlong(hs)) ? 0u : 1u;
+    const SqrtRcp hr = sqrt_rcp_core(hx);
+    bad_sqrt += (__double_as_longlong(hr.s) == __double_as_longlong(hs)) ? 0u : 1u;
   }
   if (bad_div) atomicAdd(&mism[0], (unsigned long long)bad_div);
   if (bad_sqrt) atomicAdd(&mism[1], (unsigned long long)bad_sqrt);
+  if (done) atomicAdd(&mism[2], (unsigned long long)done);
 }
 
 }  // namespace pd

@@ -209,6 +209,7 @@ def test_staged_shapes_fall_back_when_block_tables_overflow(oracle_lib):
         best, ms = g.autotune(1)
         assert not shapes[best][2]
         assert all(ms[k] == -1 for k in staged) and all(ms[k] > 0 for k in range(len(shapes)) if k not in staged)
+        assert ms[best] == min(ms[k] for k in range(len(shapes)) if ms[k] > 0)   # the fastest timed shape
         g.step(4)
         o.step(4)
         compare_engines(g, o, s, "after autotune")
@@ -228,6 +229,7 @@ def test_autotune_leaves_state_untouched(oracle_lib):
         best, ms = g.autotune(2)
         assert 0 <= best < len(abi.bond_shapes()) and g.bond_shape() == best
         assert np.all(ms > 0)
+        assert ms[best] == ms.min()                  # the tuner keeps the fastest shape
         for p in range(len(s.particles)):
             after = g.state(p)
             for f in ("disp", "vel", "acc", "force"):
