@@ -289,6 +289,18 @@ void pd_destroy(pd_ctx* ctx);
 int pd_math_mode(pd_ctx* ctx, int* safe);
 int pd_selftest_math(uint64_t n, uint64_t seed, uint64_t* mismatch_div, uint64_t* mismatch_sqrt);
 
+/* pd_selftest_math without any device allocation, for callers that cannot
+ * allocate: harnesses that forbid hipMalloc, serving processes that check the
+ * arithmetic before admitting jobs.  No counterpart in the reference.  Same
+ * kernel and counts as pd_selftest_math; *evaluated (may be NULL) receives
+ * the number of samples checked (n).  The counters live in a 256-byte
+ * file-scope device array of the library, one copy per GPU, held until the
+ * process ends; every call on a GPU clears and reuses that GPU's copy, so
+ * calls that overlap on one GPU race.  n above 2^40 is rejected (PD_E_ARG);
+ * errors are read with pd_last_error(NULL) on the calling thread. */
+int pd_selftest_math_noalloc(uint64_t n, uint64_t seed, uint64_t* mismatch_div,
+                             uint64_t* mismatch_sqrt, uint64_t* evaluated);
+
 /* Library version and build target, for logs. */
 const char* pd_version(void);
 

@@ -155,6 +155,9 @@ def load_gpu_lib(path=GPU_LIB):
         lib.pd_math_mode.restype = C.c_int
         lib.pd_selftest_math.argtypes = [_u64, _u64, C.POINTER(_u64), C.POINTER(_u64)]
         lib.pd_selftest_math.restype = C.c_int
+        lib.pd_selftest_math_noalloc.argtypes = [_u64, _u64, C.POINTER(_u64), C.POINTER(_u64),
+                                                 C.POINTER(_u64)]
+        lib.pd_selftest_math_noalloc.restype = C.c_int
         P = C.c_void_p
         for name, res, args in (
                 ("pd_reserve", C.c_int, [P, C.POINTER(PoolDesc)]),
@@ -547,6 +550,18 @@ def selftest_math(n, seed=1):
     if rc != PD_OK:
         raise PDError(f"pd_selftest_math: {ERRORS.get(rc, rc)}")
     return md.value, ms.value
+
+
+def selftest_math_noalloc(n, seed=1):
+    """(division mismatches, sqrt mismatches, samples evaluated) from the
+    allocation-free entry point (counters in the library's static scratch)."""
+    lib, _ = load_gpu_lib()
+    md, ms, ev = _u64(), _u64(), _u64()
+    rc = lib.pd_selftest_math_noalloc(n, seed, C.byref(md), C.byref(ms), C.byref(ev))
+    if rc != PD_OK:
+        lib.pd_last_error.restype = C.c_char_p
+        raise PDError(f"pd_selftest_math_noalloc: {ERRORS.get(rc, rc)}")
+    return md.value, ms.value, ev.value
 
 
 def gpu_engine(system):

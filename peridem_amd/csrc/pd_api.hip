@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1928,6 +1929,72 @@ int pd_selftest_math(uint64_t n, uint64_t seed, uint64_t* mismatch_div, uint64_t
   *mismatch_div = hm[0];
   *mismatch_sqrt = hm[1];
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// The math self-test without device allocation.  Its counters live in the
+// file-scope array `scratch` (one copy per GPU, each held until the process
+// ends).  Host code never passes the array's name to a kernel: it takes the
+// device address of the copy on the current GPU with hipGetSymbolAddress at
+// the first call on that GPU and uses only that address afterwards.
+constexpr int kScratchWords = 32;                     // 256 bytes; 3 are used
+constexpr uint64_t kSelftestMaxSamples = 1ull << 40;  // per-thread counters are 32-bit
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+static void* g_scratch_ptr[kScratchDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_on_current_device(unsigned long long** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices) {
+    t_err = "pd_selftest_math_noalloc: no usable current device";
+    return PD_E_HIP;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      t_err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+      return PD_E_HIP;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = (unsigned long long*)g_scratch_ptr[dev];
+  return PD_OK;
+}
+
+int pd_selftest_math_noalloc(uint64_t n, uint64_t seed, uint64_t* mismatch_div,
+                             uint64_t* mismatch_sqrt, uint64_t* evaluated) {
+  if (!mismatch_div || !mismatch_sqrt) {
+    t_err = "pd_selftest_math_noalloc: null output pointer";
+    return PD_E_ARG;
+  }
+  if (n > kSelftestMaxSamples) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "pd_selftest_math_noalloc: n = %llu exceeds the maximum of %llu samples",
+             (unsigned long long)n, (unsigned long long)kSelftestMaxSamples);
+    t_err = buf;
+    return PD_E_ARG;
+  }
+  unsigned long long* dm = nullptr;
+  int rc = scratch_on_current_device(&dm);
+  if (rc) return rc;
+  // the array keeps the previous call's counts: clear them (fill kernel, no memset)
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, 0, (uint32_t*)dm, (uint64_t)6, 0u);
+  hipLaunchKernelGGL(k_selftest_math, dim3(2048), dim3(256), 0, 0, n, seed, dm);
+  unsigned long long hm[3] = {0, 0, 0};
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(hm, dm, sizeof hm, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    t_err = std::string("pd_selftest_math_noalloc: ") + hipGetErrorString(e);
+    return PD_E_HIP;
+  }
+  *mismatch_div = hm[0];
+  *mismatch_sqrt = hm[1];
+  if (evaluated) *evaluated = hm[2];
+  return PD_OK;
 }
 
 int pd_math_mode(pd_ctx* c, int* safe) {
