@@ -98,6 +98,8 @@ typedef struct pd_counters {
   uint64_t nodes;
   uint64_t particles;
   uint64_t cell_rebuilds;      /* contact candidate rebuilds */
+  uint64_t list_capacity;      /* contact candidates a node's list holds (grows on demand) */
+  uint64_t list_regrows;       /* times the lists grew inside a pd_step */
 } pd_counters;
 
 typedef struct pd_ctx pd_ctx;

@@ -66,7 +66,7 @@ class PoolDesc(C.Structure):
 class Counters(C.Structure):
     _fields_ = [(n, _u64) for n in ("steps", "bond_updates", "pair_computations",
                                     "alive_bonds", "total_bonds", "nodes", "particles",
-                                    "cell_rebuilds")]
+                                    "cell_rebuilds", "list_capacity", "list_regrows")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -466,6 +466,11 @@ class Pool:
 
     def set_flag(self, job, flag, value):
         self._check(self.lib.pd_job_set_flag(self.ctx, job, flag, int(value)), "job_set_flag")
+
+    def counters(self):
+        c = Counters()
+        self._check(self.f["get_counters"](self.ctx, C.byref(c)), "get_counters")
+        return c.as_dict()
 
     def set_graphs(self, on=True):
         self._check(self.lib.pd_set_graphs(self.ctx, int(on)), "set_graphs")

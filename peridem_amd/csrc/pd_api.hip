@@ -44,9 +44,11 @@ constexpr int kProfRing = 16;   // event sets of profiled steps in flight
 constexpr int kProfLag = 8;     // a profiled step's marks are read this many steps later
 // device bytes per node slot (X, U, V, Vold, A, F, E, pid, nflag, deg, wmask, ckey,
 // csorted, ~2 hash buckets) and per bond entry
-// node arrays + the bond kernel's record copies + block tables (kLdsCap per
-// 256 nodes); per entry the u32 and u16 forms
-constexpr uint64_t kBytesPerNode = 32 + 32 + 5 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16 + 56 + 36 + 2;
+// node arrays + the bond kernel's record copies and both displacement
+// buffers + block tables (kLdsCap per 256 nodes); per entry the u32 and u16
+// forms.  The candidate lists (4 B per node and list entry) are sized for the
+// pool as a whole.
+constexpr uint64_t kBytesPerNode = 32 + 32 + 5 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16 + 80 + 36 + 2;
 constexpr uint64_t kBytesPerEntry = 6;
 
 struct HostPart {
@@ -73,6 +75,7 @@ struct Job {
   bool vc_on = false;        // a prescribed-velocity particle is set and switched on
   int64_t vc_part = -1;      // its particle index in the job
   bool lds_ok = true;        // every block table fits kLdsCap
+  std::string failed;        // why the job was stopped (empty: it was not)
 };
 
 // first-fit allocator over [0, cap)
@@ -163,6 +166,12 @@ struct pd_ctx {
   double tune_ms[kBondShapes] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  int graph_par = 0;            // displacement buffer the captured graph starts from
+  // the two displacement buffers; ucur holds the current displacement
+  double2* uxy[2] = {nullptr, nullptr};
+  double* uz[2] = {nullptr, nullptr};
+  int ucur = 0;
+  uint64_t list_regrows = 0;    // candidate-list capacity increases
 };
 
 static int fail(pd_ctx* c, int code, const char* fmt, ...) {
@@ -551,6 +560,13 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(ZV, N);
   ALLOC(UXY, N);
   ALLOC(UZ, N);
+  ALLOC(UXYo, N);
+  ALLOC(UZo, N);
+  c->uxy[0] = d.UXY;
+  c->uz[0] = d.UZ;
+  c->uxy[1] = d.UXYo;
+  c->uz[1] = d.UZo;
+  c->ucur = 0;
   ALLOC(bloc, std::max<uint64_t>(c->cap_entries, 1));
   ALLOC(btab, S / kBlockSlices * kLdsCap);
   ALLOC(binfo, S / kBlockSlices);
@@ -577,13 +593,14 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(pactive, P);
   ALLOC(counters, 8);
   ALLOC(du2s, I * kDuSlots * kDuStride);
-  ALLOC(err, 1);
+  ALLOC(jneed, I);
   ALLOC(flags, 1);
   ALLOC(ctl, 1);
   ALLOC(reaction, 18 * I);
   ALLOC(wmask, N);
   ALLOC(bflag, N / 256 + 1);
-  ALLOC(clist, N * (uint64_t)kMaxCand);
+  d.ccap = kMaxCand;
+  ALLOC(clist, N * (uint64_t)d.ccap);
   ALLOC(ccnt, N);
   ALLOC(UBXY, N);
   ALLOC(UBZ, N);
@@ -604,6 +621,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipMemsetAsync(d.ZV, 0, N * sizeof(double2), st));
   HIPCHK(c, hipMemsetAsync(d.UXY, 0, N * sizeof(double2), st));
   HIPCHK(c, hipMemsetAsync(d.UZ, 0, N * sizeof(double), st));
+  HIPCHK(c, hipMemsetAsync(d.UXYo, 0, N * sizeof(double2), st));
+  HIPCHK(c, hipMemsetAsync(d.UZo, 0, N * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.V, 0, 3 * N * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.A, 0, 3 * N * sizeof(double), st));
   HIPCHK(c, hipMemsetAsync(d.F, 0, 3 * N * sizeof(double), st));
@@ -624,7 +643,7 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipMemsetAsync(d.pactive, 0, P * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.counters, 0, 8 * sizeof(unsigned long long), st));
   HIPCHK(c, hipMemsetAsync(d.du2s, 0, I * kDuSlots * kDuStride * sizeof(unsigned long long), st));
-  HIPCHK(c, hipMemsetAsync(d.err, 0, sizeof(uint32_t), st));
+  HIPCHK(c, hipMemsetAsync(d.jneed, 0, I * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.flags, 0, sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.ctl, 0, sizeof(DevCtl), st));
   HIPCHK(c, hipMemsetAsync(d.reaction, 0, 18 * I * sizeof(double), st));
@@ -1209,8 +1228,13 @@ static int accumulate_marks(pd_ctx* c, hipEvent_t* ev) {
 
 // One step's launches; with `ev` set, a mark is recorded before each phase
 // and after the last (PD_PROFILE_BOND: only around the bond kernel).
-static int launch_step(pd_ctx* c, hipEvent_t* ev = nullptr) {
-  const Dev d = step_view(c);
+static int launch_step(pd_ctx* c, int par, hipEvent_t* ev = nullptr) {
+  Dev d = step_view(c);
+  // the step reads the displacement of buffer par and writes buffer par ^ 1
+  d.UXYo = c->uxy[par];
+  d.UZo = c->uz[par];
+  d.UXY = c->uxy[par ^ 1];
+  d.UZ = c->uz[par ^ 1];
   hipStream_t st = c->stream;
   const uint64_t Np = d.S * kSlice;
   auto mark = [&](int k) -> int {
@@ -1364,13 +1388,13 @@ static int drop_graph(pd_ctx* c) {
 
 // Capture kGraphSteps steps into a graph; captured again whenever the launch
 // sequence or its arguments change (jobs added or removed, switches).
-static int ensure_graph(pd_ctx* c) {
-  if (c->gexec && !c->graph_dirty) return PD_OK;
+static int ensure_graph(pd_ctx* c, int par) {
+  if (c->gexec && !c->graph_dirty && c->graph_par == par) return PD_OK;
   int rc = drop_graph(c);
   if (rc) return rc;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   for (int s = 0; s < kGraphSteps; s++) {
-    rc = launch_step(c);
+    rc = launch_step(c, par ^ (s & 1));
     if (rc) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(c->stream, &g);
@@ -1381,6 +1405,113 @@ static int ensure_graph(pd_ctx* c) {
   HIPCHK(c, hipStreamEndCapture(c->stream, &c->graph));
   HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
   c->graph_dirty = false;
+  c->graph_par = par;   // kGraphSteps is even: a replay ends on the buffer it starts from
+  return PD_OK;
+}
+
+static void set_current_disp(pd_ctx* c, int par) {
+  c->ucur = par;
+  c->d.UXY = c->uxy[par];
+  c->d.UZ = c->uz[par];
+  c->d.UXYo = c->uxy[par ^ 1];
+  c->d.UZo = c->uz[par ^ 1];
+}
+
+// Launch nsteps steps from the current displacement buffer (graph replays
+// and single launches), wait for them and read the control words.
+static int run_steps(pd_ctx* c, uint32_t nsteps, DevCtl* ctl, double* ms_out) {
+  int rc;
+  // steps_left drives reaction_step(): the reaction is formed on the last
+  // step of this call (and on each job's own last step)
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, &c->d.ctl->steps_left, (uint64_t)1, nsteps);
+  HIPCHK(c, hipGetLastError());
+  uint32_t done = 0;
+  int par = c->ucur;
+  if (c->prof) {
+    // marks cannot be timed from inside a graph: profiled steps launch one
+    // by one, and each step's marks are read kProfLag steps later, so the
+    // host waits on work that finished long ago and the queue stays full
+    for (; done < nsteps; done++, par ^= 1) {
+      if (done >= (uint32_t)kProfLag && (rc = accumulate_marks(c, c->pev[(done - kProfLag) % kProfRing])))
+        return rc;
+      if ((rc = launch_step(c, par, c->pev[done % kProfRing]))) return rc;
+    }
+    for (uint32_t i = nsteps > (uint32_t)kProfLag ? nsteps - kProfLag : 0; i < nsteps; i++)
+      if ((rc = accumulate_marks(c, c->pev[i % kProfRing]))) return rc;
+  } else {
+    if (c->use_graph && nsteps >= (uint32_t)kGraphSteps) {
+      // a graph captured from the other buffer: one single step first
+      if (c->gexec && !c->graph_dirty && c->graph_par != par) {
+        if ((rc = launch_step(c, par))) return rc;
+        par ^= 1;
+        done++;
+      }
+      if (nsteps - done >= (uint32_t)kGraphSteps) {
+        if ((rc = ensure_graph(c, par))) return rc;
+        for (; done + kGraphSteps <= nsteps; done += kGraphSteps) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+      }
+    }
+    for (; done < nsteps; done++, par ^= 1)
+      if ((rc = launch_step(c, par))) return rc;
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *ms_out = ms;
+  HIPCHK(c, hipMemcpy(ctl, c->d.ctl, sizeof *ctl, hipMemcpyDeviceToHost));
+  return PD_OK;
+}
+
+// The pool halted inside a step: some candidate list was longer than the
+// capacity.  The state is that of the end of the last complete step (the
+// halted step's displacement went to the buffer that is not current); the
+// step's box accumulators, pair flags, pair count and cell counts are
+// cleared.  Jobs whose lists need more than kMaxCap entries are stopped
+// there (PD_E_OVERFLOW names them, pd_job_remaining reports it); for the
+// rest the lists grow and every job rebuilds its lists at its next step.
+static int recover_overflow(pd_ctx* c, std::string* stopped) {
+  std::vector<uint32_t> need(c->d.ninst);
+  HIPCHK(c, hipMemcpy(need.data(), c->d.jneed, need.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  uint32_t grow = 0;
+  int rc;
+  for (uint32_t q = 0; q < c->jobs.size(); q++) {
+    Job& J = c->jobs[q];
+    if (!J.used || !need[q]) continue;
+    if (need[q] > kMaxCap) {
+      char buf[200];
+      snprintf(buf, sizeof buf, "job %u stopped: a node has %u contact candidates within R_c + skin, more than %u",
+               q, need[q], kMaxCap);
+      J.failed = buf;
+      J.remaining = 0;
+      const uint64_t zero = 0;
+      HIPCHK(c, hipMemcpy((char*)(c->d.inst + q) + offsetof(DevInst, remaining), &zero, sizeof zero,
+                          hipMemcpyHostToDevice));
+      if (!stopped->empty()) *stopped += "; ";
+      *stopped += buf;
+    } else {
+      grow = std::max(grow, need[q]);
+    }
+  }
+  uint32_t relist = 0;
+  if (grow > c->d.ccap) {
+    const uint32_t ncap = std::min<uint32_t>(kMaxCap, std::max<uint32_t>(grow, (grow + grow / 4 + 15) / 16 * 16));
+    uint32_t* nl = nullptr;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = dalloc(c, &nl, c->d.N * (uint64_t)ncap))) return rc;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->d.clist);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    HIPCHK(c, hipFree(c->d.clist));
+    c->d.clist = nl;
+    c->d.ccap = ncap;
+    c->list_regrows++;
+    relist = 1;
+    c->graph_dirty = true;
+  }
+  hipLaunchKernelGGL(k_rollback, dim3(256), dim3(256), 0, c->stream, c->d, c->d.T, relist);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return PD_OK;
 }
 
@@ -1392,7 +1523,13 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
   if (nsteps == 0) return PD_OK;
   bool any = false;
   for (auto& J : c->jobs) any |= J.used && J.remaining > 0;
-  if (!any) return PD_OK;
+  if (!any) {
+    // a plain run whose job was stopped keeps reporting why
+    if (c->committed)
+      for (auto& J : c->jobs)
+        if (J.used && !J.failed.empty()) return fail(c, PD_E_OVERFLOW, "%s", J.failed.c_str());
+    return PD_OK;
+  }
   int rc;
   if (c->bond_shape < 0) {
     // small pools: the launch is too short for shape to matter more than
@@ -1403,50 +1540,40 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
       c->bond_shape = kSmallPoolShape;   // kDefaultBondShape while it is not usable
     }
   }
-  // steps_left drives reaction_step(): the reaction is formed on the last
-  // step of this call (and on each job's own last step)
-  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, &c->d.ctl->steps_left, (uint64_t)1, nsteps);
-  HIPCHK(c, hipGetLastError());
-  uint32_t done = 0;
-  if (c->prof) {
-    // marks cannot be timed from inside a graph: profiled steps launch one
-    // by one, and each step's marks are read kProfLag steps later, so the
-    // host waits on work that finished long ago and the queue stays full
-    for (; done < nsteps; done++) {
-      if (done >= (uint32_t)kProfLag && (rc = accumulate_marks(c, c->pev[(done - kProfLag) % kProfRing])))
-        return rc;
-      if ((rc = launch_step(c, c->pev[done % kProfRing]))) return rc;
+  uint32_t left = nsteps;
+  double total_ms = 0;
+  std::string stopped;
+  while (left) {
+    DevCtl ctl{};
+    double ms = 0;
+    const int par0 = c->ucur;
+    if ((rc = run_steps(c, left, &ctl, &ms))) return rc;
+    total_ms += ms;
+    const uint32_t done = ctl.halt ? left - ctl.steps_left : left;
+    set_current_disp(c, par0 ^ (int)(done & 1));
+    for (auto& J : c->jobs) {
+      if (!J.used || !J.remaining) continue;
+      const uint64_t k = std::min<uint64_t>(J.remaining, done);
+      if (J.remaining != kUnlimited) J.remaining -= k;
+      J.steps_done += k;
+      J.steps_since_set += k;
+      J.reaction_valid = J.run.compute_wall_reaction && J.run.allow_wall;
+      c->cnt.bond_updates += J.movable_bonds * k;
     }
-    for (uint32_t i = nsteps > (uint32_t)kProfLag ? nsteps - kProfLag : 0; i < nsteps; i++)
-      if ((rc = accumulate_marks(c, c->pev[i % kProfRing]))) return rc;
-  } else {
-    if (c->use_graph && nsteps >= (uint32_t)kGraphSteps) {
-      if ((rc = ensure_graph(c))) return rc;
-      for (; done + kGraphSteps <= nsteps; done += kGraphSteps) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
-    }
-    for (; done < nsteps; done++)
-      if ((rc = launch_step(c))) return rc;
+    c->cnt.steps += done;
+    left -= done;
+    if (!ctl.halt) break;
+    const uint32_t cap0 = c->d.ccap;
+    const size_t nstop = stopped.size();
+    if ((rc = recover_overflow(c, &stopped))) return rc;
+    if (c->d.ccap == cap0 && stopped.size() == nstop)
+      return fail(c, PD_E_OVERFLOW, "contact candidate lists could not grow (capacity %u)", cap0);
+    bool live = false;
+    for (auto& J : c->jobs) live |= J.used && J.remaining > 0;
+    if (!live) break;
   }
-  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-  HIPCHK(c, hipEventSynchronize(c->ev1));
-  float ms = 0;
-  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  c->last_ms = ms;
-  for (auto& J : c->jobs) {
-    if (!J.used || !J.remaining) continue;
-    const uint64_t k = std::min<uint64_t>(J.remaining, nsteps);
-    if (J.remaining != kUnlimited) J.remaining -= k;
-    J.steps_done += k;
-    J.steps_since_set += k;
-    J.reaction_valid = J.run.compute_wall_reaction && J.run.allow_wall;
-    c->cnt.bond_updates += J.movable_bonds * k;
-  }
-  c->cnt.steps += nsteps;
-  uint32_t err = 0;
-  HIPCHK(c, hipMemcpy(&err, c->d.err, sizeof err, hipMemcpyDeviceToHost));
-  if (err & kErrCandOverflow)
-    return fail(c, PD_E_OVERFLOW, "more than %d contact candidates within R_c + skin of a node", kMaxCand);
+  c->last_ms = total_ms;
+  if (!stopped.empty()) return fail(c, PD_E_OVERFLOW, "%s", stopped.c_str());
   return PD_OK;
 }
 
@@ -1496,6 +1623,7 @@ int pd_job_remaining(pd_ctx* c, uint32_t job, uint64_t* remaining) {
   if (!c || !remaining) return PD_E_ARG;
   if (job >= c->jobs.size() || !c->jobs[job].used) return fail(c, PD_E_ARG, "no job %u", job);
   *remaining = c->jobs[job].remaining;
+  if (!c->jobs[job].failed.empty()) return fail(c, PD_E_OVERFLOW, "%s", c->jobs[job].failed.c_str());
   return PD_OK;
 }
 
@@ -1880,6 +2008,8 @@ int pd_get_counters(pd_ctx* c, pd_counters* out) {
     unsigned long long rb = 0;
     HIPCHK(c, hipMemcpy(&rb, c->d.counters + 2, sizeof rb, hipMemcpyDeviceToHost));
     out->cell_rebuilds = rb;
+    out->list_capacity = c->d.ccap;
+    out->list_regrows = c->list_regrows;
     unsigned long long* dcount = c->d.counters + 1;
     HIPCHK(c, hipMemsetAsync(dcount, 0, sizeof(unsigned long long), c->stream));
     if (c->cap_entries) {

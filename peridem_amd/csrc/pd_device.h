@@ -23,11 +23,12 @@ constexpr uint32_t kLive16 = 16 * kLdsCap;  // u16 entries below this are live t
 constexpr uint32_t kDead16 = 0xFFFFu;       // u16 entry of an erased bond
 constexpr uint32_t kPad16 = 0xFFFFu;        // u16 slice padding entry
 static_assert(kLive16 <= kDead16, "table offsets must fit a u16 below the dead value");
-constexpr int kMaxCand = 48;                // contact candidates kept per node (list capacity)
+// Contact candidate lists: a node's list is built in passes of kMaxCand
+// entries (held in registers); the pool's list capacity (Dev::ccap, at least
+// kMaxCand) grows on demand up to kMaxCap, the most a u8 count holds.
+constexpr int kMaxCand = 48;
+constexpr uint32_t kMaxCap = 255;
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
-
-// error bits raised by kernels, read back by the host after pd_step
-constexpr uint32_t kErrCandOverflow = 1u;
 // state bits of Dev::flags
 constexpr uint32_t kFlagSafeMath = 1u;  // an operand left the fast-path range: use IEEE ops
 
@@ -82,6 +83,13 @@ struct DevInst {
 struct DevCtl {
   uint32_t steps_left;      // steps left in the current pd_step call
   uint32_t any_rebuild;     // some job rebuilds its candidate lists this step
+  // A candidate list longer than the capacity halts the pool inside the step
+  // that builds it: every later kernel of the call exits at once, so the
+  // state is that of the end of the previous step (the displacement of the
+  // halted step went to the other buffer) and the host grows the lists and
+  // runs the step again.
+  uint32_t halt;
+  uint32_t need;            // longest list the halted build asked for
 };
 
 struct Dev {
@@ -96,6 +104,11 @@ struct Dev {
   double2* ZV;       // z, vol
   double2* UXY;      // disp x, y: the resident displacement (written by the position kernel)
   double* UZ;        // disp z
+  // The displacement is double-buffered: the position kernel reads the
+  // previous step's buffer (UXYo, UZo) and writes the step's own (UXY, UZ),
+  // which every later kernel of the step reads; the host alternates them.
+  double2* UXYo;
+  double* UZo;
   uint16_t* bloc;    // [entries] the bond entries as positions in their block's node table
   uint32_t* btab;    // [blocks * kLdsCap] per block: sorted pool indices of the nodes its rows reference
   uint2* binfo;      // [blocks] (table length, position of the block's first node in it)
@@ -132,8 +145,9 @@ struct Dev {
   // per job, kDuSlots running maxima of |u - u_build|^2 since the list build
   // (bits of doubles >= 0), one per XCD, each in its own 128-byte line
   unsigned long long* du2s;
-  unsigned long long* counters;   // [0] pair computations
-  uint32_t* err;
+  unsigned long long* counters;   // [0] pair computations, [2] list rebuilds,
+                                  // [3] pair computations of the running step
+  uint32_t* jneed;   // [ninst] longest candidate list a job's halted build asked for
   uint32_t* flags;   // [1] kFlagSafeMath (sticky)
   DevCtl* ctl;
   // wall reaction (per instance, 6 planes x 3) and helpers
@@ -141,7 +155,7 @@ struct Dev {
   uint8_t* wmask;    // [N] planes in contact this step (reaction steps only)
   uint8_t* bflag;    // [N/256+1] block has a wall contact
   // contact candidate lists (Verlet lists with a skin): entry k of node g at
-  // clist[(g >> 6) * 64 * kMaxCand + 64 k + (g & 63)], ccnt[g] entries
+  // clist[(g >> 6) * 64 * ccap + 64 k + (g & 63)], ccnt[g] entries
   uint32_t* clist;
   uint8_t* ccnt;
   double2* UBXY;     // [N] displacement when the node's list was built: x, y
@@ -155,6 +169,7 @@ struct Dev {
   uint64_t N, S, P, npairs, ninst;   // N = S*64 node slots; P, ninst incl. the void slot
   uint32_t T;        // hash table size (power of two)
   uint32_t lists;    // some job of the pool can have contact: candidate lists are kept
+  uint32_t ccap;     // candidate list capacity per node (kMaxCand .. kMaxCap)
 };
 
 // ---------------------------------------------------------------------------

@@ -40,6 +40,12 @@ __device__ __forceinline__ T ld_uniform(const T* base, uint32_t i) {
   return v;
 }
 
+// The pool was halted earlier in this pd_step call by a candidate list
+// longer than the capacity: the kernel does nothing (read once per
+// wavefront; the flag only goes up during the list build, and a build that
+// reads it clear just carries on).
+__device__ __forceinline__ bool halted(const Dev& d) { return ld_uniform(&d.ctl->halt, 0u) != 0; }
+
 // Does job I accumulate its wall reaction in this step?  On its last step and
 // on the last step of a pd_step call, so the value read back afterwards is the
 // reaction of the job's latest step, as Wall.reaction_* is (timeloop.h:397).
@@ -91,6 +97,7 @@ __global__ __launch_bounds__(256) void k_join(Dev d, uint64_t g0, uint64_t n) {
 // plus the min/max of CurrPos that becomes the particle's lagged box at the
 // end of the step (timeloop.h:779).
 __global__ __launch_bounds__(256) void k_position(Dev d) {
+  if (halted(d)) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // the block's wall-contact flag of this step starts clear, and so does the
   // step's rebuild flag (set by k_cellkey)
@@ -123,8 +130,14 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
     }
     if (!stepping) p = jid = 0xFFFFFFFFu;
   }
+  if (g < d.N && !stepping) {
+    // frozen and void slots keep their displacement in the step's buffer
+    d.UXY[g] = d.UXYo[g];
+    d.UZ[g] = d.UZo[g];
+  }
   if (stepping) {
-    double4 u = disp_of(d, g);
+    const double2 uo = d.UXYo[g];
+    double4 u = make_double4(uo.x, uo.y, d.UZo[g], 0.0);
     const double4 x = d.X[g];
     const double h = (dt * dt) * 0.5;
     double v0 = d.V[3 * g], v1 = d.V[3 * g + 1], v2 = d.V[3 * g + 2];
@@ -294,6 +307,7 @@ __device__ __forceinline__ bool rebuilds_node(const Dev& d, uint32_t job) {
 // mean_CurrPos (particle2.h:94-103) of every particle of a job with torque on:
 // one thread per particle sums pos + disp in node order, as the reference.
 __global__ __launch_bounds__(64) void k_centroid(Dev d) {
+  if (halted(d)) return;
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.P) return;
   const DevPart& P = d.part[q];
@@ -323,7 +337,7 @@ __device__ __forceinline__ void pair_test(Dev& d, uint64_t q) {
   const DevInst& I = d.inst[Pi.inst];
   if (!live(I) || !(Pi.movable || Pj.movable)) return;
   if (!intersects_box_fext(d.lrtb + 6 * i, d.lrtb + 6 * j, I.Rc)) return;
-  atomicAdd(&d.counters[0], 1ull);
+  atomicAdd(&d.counters[3], 1ull);   // moved into the total by k_finalize
   // bit 1: the particle's nodes are contact candidates this step;
   // bit 0: the particle receives contact terms (only stoppable ones do)
   atomicOr(&d.pactive[i], 2u | (Pi.stoppable ? 1u : 0u));
@@ -352,6 +366,7 @@ __device__ __forceinline__ void cell_of(const double* cp, double cell, int64_t* 
 // their candidate lists.  The cell table (cstart) is zero when a rebuild
 // starts: cleared at reservation, then by k_finalize after every rebuild.
 __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
+  if (halted(d)) return;
   const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < d.npairs) pair_test(d, q);
   if (q < d.P) wall_test(d, q);
@@ -380,7 +395,7 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
 // Two phases: per-block scan (k_scan1), add-back (k_scan3).
 constexpr int kScanBlock = 1024;
 __global__ __launch_bounds__(kScanBlock) void k_scan1(const DevCtl* ctl, uint32_t* a, uint32_t n, uint32_t* sums) {
-  if (!ctl->any_rebuild) return;
+  if (ctl->halt || !ctl->any_rebuild) return;
   __shared__ uint32_t s[kScanBlock];
   uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
   uint32_t v = i < n ? a[i] : 0;
@@ -400,7 +415,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan1(const DevCtl* ctl, uint32_
 // the last block writes the grand total to a[n].
 __global__ __launch_bounds__(kScanBlock) void k_scan3(const DevCtl* ctl, uint32_t* a, uint32_t n, const uint32_t* sums,
                                                       uint32_t* cur) {
-  if (!ctl->any_rebuild) return;
+  if (ctl->halt || !ctl->any_rebuild) return;
   __shared__ uint32_t red[kScanBlock / 64];
   uint32_t part = 0;
   for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanBlock) part += sums[j];
@@ -419,7 +434,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan3(const DevCtl* ctl, uint32_
 }
 
 __global__ __launch_bounds__(256) void k_scatter(Dev d) {
-  if (!d.ctl->any_rebuild) return;
+  if (halted(d) || !d.ctl->any_rebuild) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
   // only the nodes of rebuilding jobs hold a key of this step
@@ -466,8 +481,17 @@ __device__ __forceinline__ bool build_head(const Dev& d, uint64_t g, uint32_t& p
   return true;
 }
 
+// A node's list is longer than the pool's capacity: the pool halts (the
+// host grows the lists and runs the step again) and the job records the
+// length it needs.
+__device__ __forceinline__ void list_overflow(const Dev& d, uint32_t job, uint32_t need) {
+  atomicMax(&d.jneed[job], need);
+  atomicMax(&d.ctl->need, need);
+  atomicOr(&d.ctl->halt, 1u);
+}
+
 __global__ __launch_bounds__(256) void k_build(Dev d) {
-  if (!d.ctl->any_rebuild) return;
+  if (halted(d) || !d.ctl->any_rebuild) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
   uint32_t p;
@@ -478,68 +502,81 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
   constexpr uint32_t kEmpty = ~0u;
-  uint32_t w[kMaxCand];
+  uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * d.ccap) + (g & 63);
+  // passes of kMaxCand entries: pass k keeps the kMaxCand smallest entries
+  // above the largest one of pass k-1
+  uint32_t total = 0, lb = 0;
+  bool more = true, first = true;
+  while (more) {
+    uint32_t w[kMaxCand];
 #pragma unroll
-  for (int i = 0; i < kMaxCand; i++) w[i] = kEmpty;
-  bool over = false;
-  // each cell's bucket bounds are loaded one cell ahead, each member's index
-  // one member ahead, so their round trips overlap the work on the previous
-  uint32_t key = cell_hash(c[0] - 1, c[1] - 1, c[2] - 1, P.inst, d.T);
-  uint32_t nb0 = d.cstart[key], nb1 = d.cstart[key + 1];
-  for (int s = 0; s < 27; s++) {
-    const uint32_t b0 = nb0, b1 = nb1;
-    if (s < 26) {
-      const int t = s + 1, dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-      key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
-      nb0 = d.cstart[key];
-      nb1 = d.cstart[key + 1];
-    }
-    uint32_t nj = b0 < b1 ? d.csorted[b0] : 0u;
-    for (uint32_t it = b0; it < b1; it++) {
-      const uint32_t j = nj;
-      if (it + 1 < b1) nj = d.csorted[it + 1];
-      if (j == g) continue;
-      // jobs and particles hold contiguous node ranges: membership is a
-      // range test (hash buckets are shared by every job)
-      if (j < I.node0 || j >= I.node1) continue;
-      const bool same = j >= P.node_off && j < P.node_off + P.nnodes;
-      if (same && !selfc) continue;
-      const double4 xj = d.X[j];
-      const double4 uj = disp_of(d, j);
-      const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
-                             (uj.z - ui.z) + (xj.z - xi.z));
-      if (!(dn <= I.rb)) continue;
-      uint32_t x = (same ? 0u : kBroken) | j;
-#pragma unroll
-      for (int i0 = 0; i0 < kMaxCand; i0 += 8) {
-        if (__builtin_amdgcn_ballot_w64(x != kEmpty) == 0) continue;   // nothing left to place
-#pragma unroll
-        for (int i = i0; i < i0 + 8; i++) {
-          const uint32_t y = w[i];
-          x = x == y ? kEmpty : x;
-          w[i] = x < y ? x : y;
-          x = x < y ? y : x;
-        }
+    for (int i = 0; i < kMaxCand; i++) w[i] = kEmpty;
+    bool over = false;
+    // each cell's bucket bounds are loaded one cell ahead, each member's index
+    // one member ahead, so their round trips overlap the work on the previous
+    uint32_t key = cell_hash(c[0] - 1, c[1] - 1, c[2] - 1, P.inst, d.T);
+    uint32_t nb0 = d.cstart[key], nb1 = d.cstart[key + 1];
+    for (int s = 0; s < 27; s++) {
+      const uint32_t b0 = nb0, b1 = nb1;
+      if (s < 26) {
+        const int t = s + 1, dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+        key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
+        nb0 = d.cstart[key];
+        nb1 = d.cstart[key + 1];
       }
-      over |= x != kEmpty;   // a candidate fell off the end
-    }
-  }
-  if (over) atomicOr(d.err, kErrCandOverflow);
-  uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
-  uint32_t ncand = 0;
+      uint32_t nj = b0 < b1 ? d.csorted[b0] : 0u;
+      for (uint32_t it = b0; it < b1; it++) {
+        const uint32_t j = nj;
+        if (it + 1 < b1) nj = d.csorted[it + 1];
+        if (j == g) continue;
+        // jobs and particles hold contiguous node ranges: membership is a
+        // range test (hash buckets are shared by every job)
+        if (j < I.node0 || j >= I.node1) continue;
+        const bool same = j >= P.node_off && j < P.node_off + P.nnodes;
+        if (same && !selfc) continue;
+        uint32_t x = (same ? 0u : kBroken) | j;
+        if (!first && x <= lb) continue;   // placed by an earlier pass
+        const double4 xj = d.X[j];
+        const double4 uj = disp_of(d, j);
+        const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
+                               (uj.z - ui.z) + (xj.z - xi.z));
+        if (!(dn <= I.rb)) continue;
 #pragma unroll
-  for (int i = 0; i < kMaxCand; i++)
-    if (w[i] != kEmpty) {
-      cl[(uint64_t)i * kSlice] = w[i];
-      ncand++;
+        for (int i0 = 0; i0 < kMaxCand; i0 += 8) {
+          if (__builtin_amdgcn_ballot_w64(x != kEmpty) == 0) continue;   // nothing left to place
+#pragma unroll
+          for (int i = i0; i < i0 + 8; i++) {
+            const uint32_t y = w[i];
+            x = x == y ? kEmpty : x;
+            w[i] = x < y ? x : y;
+            x = x < y ? y : x;
+          }
+        }
+        over |= x != kEmpty;   // a candidate fell off the end: another pass
+      }
     }
-  d.ccnt[g] = (uint8_t)ncand;
+#pragma unroll
+    for (int i = 0; i < kMaxCand; i++)
+      if (w[i] != kEmpty) {
+        if (total < d.ccap) cl[(uint64_t)total * kSlice] = w[i];
+        total++;
+      }
+    more = over;
+    lb = w[kMaxCand - 1];
+    first = false;
+  }
+  if (total > d.ccap) {
+    list_overflow(d, P.inst, total);
+    total = d.ccap;
+  }
+  d.ccnt[g] = (uint8_t)total;
 }
 
 // Jobs with nl_bdry_only: candidates ordered by (other particle, particle
-// order, rank in the boundary list), kept by insertion into private memory.
+// order, rank in the boundary list), kept by insertion into private memory,
+// in passes of kMaxCand as above.
 __global__ __launch_bounds__(256) void k_build_bdry(Dev d) {
-  if (!d.ctl->any_rebuild) return;
+  if (halted(d) || !d.ctl->any_rebuild) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
   uint32_t p;
@@ -549,9 +586,7 @@ __global__ __launch_bounds__(256) void k_build_bdry(Dev d) {
   if (!build_head<true>(d, g, p, xi, ui, selfc, c)) return;
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
-  uint32_t cand[kMaxCand];
-  uint64_t ckey[kMaxCand];   // order key: other particle, particle order, rank in the boundary list
-  uint32_t ncand = 0;
+  uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * d.ccap) + (g & 63);
   uint32_t seen[27];
   int nseen = 0;
   for (int dz = -1; dz <= 1; dz++)
@@ -560,39 +595,58 @@ __global__ __launch_bounds__(256) void k_build_bdry(Dev d) {
         const uint32_t key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
         bool dup = false;
         for (int q = 0; q < nseen; q++) dup |= (seen[q] == key);
-        if (dup) continue;
-        seen[nseen++] = key;
-        const uint32_t b0 = d.cstart[key], b1 = d.cstart[key + 1];
-        for (uint32_t it = b0; it < b1; it++) {
-          const uint32_t j = d.csorted[it];
-          if (j == g) continue;
-          if (j < I.node0 || j >= I.node1) continue;
-          const bool same = j >= P.node_off && j < P.node_off + P.nnodes;
-          if (same && !selfc) continue;
-          const double4 xj = d.X[j];
-          const double4 uj = disp_of(d, j);
-          const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
-                                 (uj.z - ui.z) + (xj.z - xi.z));
-          if (!(dn <= I.rb)) continue;
-          const uint64_t key2 =
-              ((uint64_t)!same << 63) | ((uint64_t)d.part[d.pid[j]].node_off << 32) | (uint64_t)d.brank[j];
-          if (ncand == kMaxCand) {
-            atomicOr(d.err, kErrCandOverflow);
-            continue;
-          }
-          uint32_t w = ncand++;
-          while (w > 0 && ckey[w - 1] > key2) {
-            cand[w] = cand[w - 1];
-            ckey[w] = ckey[w - 1];
-            w--;
-          }
-          cand[w] = (same ? 0u : kBroken) | j;
-          ckey[w] = key2;
-        }
+        if (!dup) seen[nseen++] = key;
       }
-  uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
-  for (uint32_t k = 0; k < ncand; k++) cl[(uint64_t)k * kSlice] = cand[k];
-  d.ccnt[g] = (uint8_t)ncand;
+  uint32_t total = 0;
+  uint64_t lbkey = 0;
+  bool more = true, first = true;
+  while (more) {
+    uint32_t cand[kMaxCand];
+    uint64_t ckey[kMaxCand];   // order key: other particle, particle order, rank in the boundary list
+    uint32_t ncand = 0;
+    bool over = false;
+    for (int q = 0; q < nseen; q++) {
+      const uint32_t b0 = d.cstart[seen[q]], b1 = d.cstart[seen[q] + 1];
+      for (uint32_t it = b0; it < b1; it++) {
+        const uint32_t j = d.csorted[it];
+        if (j == g) continue;
+        if (j < I.node0 || j >= I.node1) continue;
+        const bool same = j >= P.node_off && j < P.node_off + P.nnodes;
+        if (same && !selfc) continue;
+        const uint64_t key2 =
+            ((uint64_t)!same << 63) | ((uint64_t)d.part[d.pid[j]].node_off << 32) | (uint64_t)d.brank[j];
+        if (!first && key2 <= lbkey) continue;   // placed by an earlier pass
+        const double4 xj = d.X[j];
+        const double4 uj = disp_of(d, j);
+        const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
+                               (uj.z - ui.z) + (xj.z - xi.z));
+        if (!(dn <= I.rb)) continue;
+        if (ncand == kMaxCand) {   // keep the kMaxCand smallest keys
+          over = true;
+          if (key2 > ckey[kMaxCand - 1]) continue;
+          ncand--;
+        }
+        uint32_t w = ncand++;
+        while (w > 0 && ckey[w - 1] > key2) {
+          cand[w] = cand[w - 1];
+          ckey[w] = ckey[w - 1];
+          w--;
+        }
+        cand[w] = (same ? 0u : kBroken) | j;
+        ckey[w] = key2;
+      }
+    }
+    for (uint32_t k = 0; k < ncand; k++, total++)
+      if (total < d.ccap) cl[(uint64_t)total * kSlice] = cand[k];
+    more = over;
+    if (ncand) lbkey = ckey[ncand - 1];
+    first = false;
+  }
+  if (total > d.ccap) {
+    list_overflow(d, P.inst, total);
+    total = d.ccap;
+  }
+  d.ccnt[g] = (uint8_t)total;
 }
 
 // Is j an intact bond of node g's row?  Rows hold the post-break state of
@@ -668,7 +722,7 @@ __device__ __forceinline__ void add_contact(const Dev& d, uint64_t g, uint32_t p
   const bool pairc = (in.pa & 1u) != 0;
   const bool wall = (in.pa & 4u) && (!I.nl_bdry_only || d.brank[g] != kNoKey);
   const uint32_t n = (selfc || pairc) ? (uint32_t)d.ccnt[g] : 0u;
-  const uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * kMaxCand) + (g & 63);
+  const uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * d.ccap) + (g & 63);
   double4 xi = make_double4(0.0, 0.0, 0.0, 0.0), ui = xi;
   double vi[3] = {0.0, 0.0, 0.0};
   if (n || wall || I.enable_torque) {
@@ -1204,6 +1258,7 @@ __device__ __forceinline__ void slice_of(const Dev& d, uint64_t t, uint64_t& bas
 // OFF32: see rec.
 template <int U, int PIPE, bool OFF32>
 __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
+  if (halted(d)) return;
   const uint64_t t = xcd_block() * blockDim.x + threadIdx.x;   // slot
   // padded node range: S slices of 64; whole wavefronts exit together
   if (t >= d.S * kSlice) return;
@@ -1242,6 +1297,7 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
 // two workgroups (two waves per SIMD), so registers are allotted for two.
 template <int U, int PIPE>
 __global__ __launch_bounds__(256, 2) void k_bond_lds(Dev d) {
+  if (halted(d)) return;
   const uint64_t blk = xcd_block();
   const uint64_t t = blk * blockDim.x + threadIdx.x;   // slot
   {
@@ -1335,6 +1391,7 @@ constexpr int kWideShape = kBondShapes;
 // kernel has zeroed their velocity and acceleration.  Launched only for pools
 // with clamped nodes.
 __global__ __launch_bounds__(256) void k_clamp(Dev d) {
+  if (halted(d)) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N || !(d.nflag[g] & 1u)) return;
   const uint32_t p = d.pid[g];
@@ -1348,6 +1405,7 @@ __global__ __launch_bounds__(256) void k_clamp(Dev d) {
 // velocity and, for the angular part, mean_CurrPos of the particle, summed
 // in node order like the reference.
 __global__ void k_pv_mean(Dev d) {
+  if (halted(d)) return;
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.ninst) return;
   DevInst& I = d.inst[q];
@@ -1375,6 +1433,7 @@ __global__ void k_pv_mean(Dev d) {
 // assigned velocity as the reference forms it, then the clamped nodes.  The
 // reference's radial / tangential split (v_r, v_t, t_unit) feeds nothing.
 __global__ __launch_bounds__(256) void k_pv_apply(Dev d) {
+  if (halted(d)) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
   const uint32_t p = d.pid[g];
@@ -1413,6 +1472,7 @@ __global__ __launch_bounds__(256) void k_pv_apply(Dev d) {
 // Lagged boxes of movable particles (timeloop.h:779), reset of the per-step
 // flags, wall motion (timeloop.h:784-797).
 __global__ __launch_bounds__(256) void k_finalize(Dev d) {
+  if (halted(d)) return;
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < d.P) {
     // the box of this step if the particle's job stepped: every node of a
@@ -1440,7 +1500,12 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d) {
       I.remaining -= 1;
     }
   }
-  if (q == 0 && d.ctl->steps_left) d.ctl->steps_left -= 1;
+  if (q == 0) {
+    if (d.ctl->steps_left) d.ctl->steps_left -= 1;
+    // the step's pair computations count once the step is complete
+    d.counters[0] += d.counters[3];
+    d.counters[3] = 0ull;
+  }
   // after a rebuild, the cell table is cleared for the next one (nothing
   // reads it after k_build); the flag itself is reset by the next step's
   // position kernel
@@ -1455,6 +1520,7 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d) {
 // per instance walks its node range; blocks without a wall contact are
 // skipped through bflag.
 __global__ void k_reaction(Dev d) {
+  if (halted(d)) return;
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.ninst) return;
   const DevInst I = d.inst[q];
@@ -1502,6 +1568,33 @@ __global__ void k_reaction(Dev d) {
     }
   }
   for (int k = 0; k < 18; k++) d.reaction[18 * q + k] = reac[k];
+}
+
+// Back to the end of the last complete step after a halt (the host makes
+// the other displacement buffer current): the halted step's box
+// accumulators, pair flags, pair count, cell counts and list lengths asked
+// for are cleared, and the halt itself.  With relist, every live job
+// rebuilds its candidate lists at its next step (they were reallocated).
+__global__ __launch_bounds__(256) void k_rollback(Dev d, uint32_t T, uint32_t relist) {
+  const uint64_t n = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t q = q0; q < d.P; q += n) {
+    for (int k = 0; k < 3; k++) {
+      d.lrtb_next[6 * q + k] = ~0ull;
+      d.lrtb_next[6 * q + 3 + k] = 0ull;
+    }
+    d.pactive[q] = 0;
+  }
+  for (uint64_t i = q0; i <= T; i += n) d.cstart[i] = 0;
+  for (uint64_t q = q0; q < d.ninst; q += n) {
+    d.jneed[q] = 0;
+    if (relist && live(d.inst[q])) d.inst[q].dirty = 1;
+  }
+  if (q0 == 0) {
+    d.counters[3] = 0ull;
+    d.ctl->halt = 0;
+    d.ctl->need = 0;
+  }
 }
 
 // fill (graph-safe replacement of a memset)

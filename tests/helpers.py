@@ -114,3 +114,49 @@ def small_jobs(n, seed, walls_every=3):
         if k % 5 == 3:
             s.dt = 7e-9
     return out
+
+
+def candidate_schedule(system, nsteps, oracle_lib):
+    """The steps at which a job rebuilds its contact candidate lists and the
+    longest list each rebuild needs, from oracle states (CPU): a list holds
+    the nodes j of the job within rb = R_c (1 + 1e-9) + 0.3 R_c of node i,
+    same-particle ones only with self-contact, measured on the displacement
+    after the step's position half; lists are rebuilt at step 1 and whenever
+    a node has moved more than half the skin since the last build."""
+    Rc = system.contact_rad
+    skin = 0.3 * Rc
+    rb = Rc * (1 + 1e-9) + skin + 1e-300
+    half = 0.5 * skin * (1 - 1e-6)
+    X = np.concatenate([p.pos for p in system.particles])
+    pid = np.concatenate([np.full(p.n, i) for i, p in enumerate(system.particles)])
+    selfc = bool(system.self_contact and system.enable_fracture)
+    e = oracle_lib.engine(system)
+    ub, out = None, []
+    try:
+        for k in range(1, nsteps + 1):
+            e.step(1)
+            U = np.concatenate([e.state(p)["disp"] for p in range(len(system.particles))])
+            if ub is not None and ((U - ub) ** 2).sum(1).max() <= half * half:
+                continue
+            ub = U.copy()
+            d = (U[None, :, :] - U[:, None, :]) + (X[None, :, :] - X[:, None, :])
+            dn = np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2])
+            ok = dn <= rb
+            np.fill_diagonal(ok, False)
+            if not selfc:
+                ok &= pid[None, :] != pid[:, None]
+            out.append((k, int(ok.sum(1).max())))
+    finally:
+        e.close()
+    return out
+
+
+def passing_spheres(Rc_h=1.8, v=1500.0, seed=21):
+    """Two spheres with a contact spring too soft to stop them (K_n 1e20):
+    they pass through each other, so the nodes of one come within the list
+    radius of the other's nodes in growing numbers."""
+    h = 1e-4
+    s = gen.two_sphere_impact(R=4 * h, h=h, v=v, seed=seed, gap_h=1.0, b_movable=1, self_contact=0)
+    s.contact_rad = Rc_h * h
+    s.normal_stiffness = 1e20
+    return s

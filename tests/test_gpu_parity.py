@@ -338,26 +338,3 @@ def test_pair_activity_lags_one_step_like_the_oracle(oracle_lib):
     s = gen.System(particles=[A, B], dt=1e-8, contact_rad=Rc, normal_stiffness=4.6e31,
                    friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=0)
     run_pair(s, (1, 2, 3), oracle_lib, "lagged pair boxes")
-
-
-@pytest.mark.parametrize("bdry_only", [0, 1])
-def test_more_candidates_than_a_list_holds_is_an_error(bdry_only):
-    """A contact radius of 2.2 lattice spacings with self-contact on puts ~70
-    nodes within the list radius of an interior node, more than the 48 a
-    candidate list holds: the step returns PD_E_OVERFLOW with a message (the
-    list is never silently truncated), for both list forms (entries as their
-    own order keys, and boundary-list order with nl_bdry_only)."""
-    rng = np.random.default_rng(31)
-    h = 1e-4
-    p = gen.make_particle(gen.lattice_sphere(5.0), h, np.zeros(3), rng)
-    s = gen.System(particles=[p], dt=1e-8, contact_rad=2.2 * h, normal_stiffness=1e12,
-                   self_contact=1, enable_fracture=1, nl_bdry_only=bdry_only)
-    if bdry_only:
-        p.bdry = np.arange(p.n, dtype=np.uint32)
-    print(f"[parity] candidate overflow, nl_bdry_only={bdry_only}: nodes={s.nodes}", flush=True)
-    e = abi.gpu_engine(s)
-    try:
-        with pytest.raises(abi.PDError, match="contact candidates"):
-            e.step(1)
-    finally:
-        e.close()
