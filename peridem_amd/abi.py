@@ -87,7 +87,14 @@ class Profile(C.Structure):
 
 
 class PDError(RuntimeError):
-    pass
+    """A failed library call; `code` is the PD_E* status (None when not from a call)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+PD_E_OVERFLOW = -5
 
 
 def _bind(lib, prefix):
@@ -212,7 +219,7 @@ class Engine:
         if rc != PD_OK:
             msg = self.f["last_error"](self.ctx) if self.ctx else b""
             raise PDError(f"{self.prefix}{what}: {ERRORS.get(rc, rc)}: "
-                          f"{(msg or b'').decode(errors='replace')}")
+                          f"{(msg or b'').decode(errors='replace')}", rc)
 
     def step(self, n=1):
         self._check(self.f["step"](self.ctx, n), "step")
@@ -398,7 +405,7 @@ class Pool:
         if rc != PD_OK:
             msg = self.f["last_error"](self.ctx) if self.ctx else b""
             raise PDError(f"pd_{what}: {ERRORS.get(rc, rc)}: "
-                          f"{(msg or b'').decode(errors='replace')}")
+                          f"{(msg or b'').decode(errors='replace')}", rc)
 
     def add_job(self, system, nsteps=UNLIMITED):
         """Admit a system with a step budget; returns its job slot."""
@@ -443,6 +450,15 @@ class Pool:
         r = _u64()
         self._check(self.lib.pd_job_remaining(self.ctx, job, C.byref(r)), "job_remaining")
         return r.value
+
+    def status(self, job):
+        """(steps remaining, None) or, for a job the library stopped, (0, why)."""
+        r = _u64()
+        rc = self.lib.pd_job_remaining(self.ctx, job, C.byref(r))
+        if rc == PD_E_OVERFLOW:
+            return 0, (self.f["last_error"](self.ctx) or b"").decode(errors="replace")
+        self._check(rc, "job_remaining")
+        return r.value, None
 
     def step(self, n=1):
         self._check(self.f["step"](self.ctx, n), "step")

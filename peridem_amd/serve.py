@@ -77,6 +77,7 @@ class Done:
     nsteps: int
     nodes: int
     digest: str | None = None
+    error: str | None = None      # why the library stopped the job (no results then)
 
     @property
     def latency(self):
@@ -231,24 +232,36 @@ class Server:
         self.peak_live = max(self.peak_live, len(self.live))
 
     def advance(self):
-        """One pd_step call over the resident jobs; returns the steps taken."""
-        rem = [self.pool.remaining(j) for j in self.live]
+        """One pd_step call over the resident jobs; returns the steps taken.
+        A job the library stops during the call (PD_E_OVERFLOW: contact
+        candidates beyond what a list can hold) does not stop the others: the
+        call completes for them and harvest() retires the stopped one."""
+        rem = [r for r, _ in (self.pool.status(j) for j in self.live) if r > 0]
+        if not rem:
+            return 0
         n = int(min(min(rem), self.chunk))
-        self.pool.step(n)
+        try:
+            self.pool.step(n)
+        except abi.PDError as e:
+            if e.code != abi.PD_E_OVERFLOW:
+                raise
         self.steps_run += n
         self.device_ms += self.pool.last_step_ms()
         return n
 
     def harvest(self, now_fn):
-        for job in [j for j in self.live if self.pool.remaining(j) == 0]:
+        for job in list(self.live):
+            left, why = self.pool.status(job)
+            if left:
+                continue
             rid, arrival, admitted, system, nsteps = self.live.pop(job)
             res = None
-            if self.digests:
+            if self.digests and why is None:
                 res = read_results(lambda p: self.pool.state(job, p),
                                    lambda p: self.pool.connectivity(job, p),
                                    lambda: self.pool.wall(job), len(system.particles))
             self.pool.remove_job(job)
-            self.done.append(Done(rid, arrival, admitted, now_fn(), nsteps, system.nodes))
+            self.done.append(Done(rid, arrival, admitted, now_fn(), nsteps, system.nodes, error=why))
             self.results.append(res)
 
     def serve(self, requests, clock=time.perf_counter):

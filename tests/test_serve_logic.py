@@ -28,11 +28,22 @@ class FakePool:
     def remaining(self, j):
         return self.jobs[j][1]
 
+    def status(self, j):
+        return (0, self.jobs[j][2]) if len(self.jobs[j]) > 2 else (self.jobs[j][1], None)
+
     def step(self, n):
         assert n > 0 and all(v[1] >= n for v in self.jobs.values() if v[1]), "stepped past a budget"
-        for v in self.jobs.values():
+        stopped = []
+        for j, v in self.jobs.items():
+            if getattr(v[0], "stop_after", None) is not None and len(v) == 2:
+                # the library stops this job inside the call (PD_E_OVERFLOW)
+                v[1] = 0
+                v.append(f"job {j} stopped")
+                stopped.append(j)
             v[1] = max(0, v[1] - n)
         self.calls.append(("step", n))
+        if stopped:
+            raise serve.abi.PDError("pd_step: PD_E_OVERFLOW: job stopped", serve.abi.PD_E_OVERFLOW)
 
     def last_step_ms(self):
         return 1.0
@@ -98,3 +109,17 @@ def test_latency_bookkeeping_and_fifo_reference():
     assert serve.fifo_latency([0.0, 0.0, 5.0], [2.0, 3.0, 1.0]) == [2.0, 5.0, 1.0]
     st = serve.latency_stats([1.0, 2.0, 3.0, 4.0])
     assert st["p50_s"] == 2.5 and st["max_s"] == 4.0
+
+
+def test_a_stopped_job_is_retired_and_serving_goes_on():
+    """A job the library stops (PD_E_OVERFLOW from pd_step) is retired with
+    its reason and no results; the other jobs run to their budgets."""
+    srv = make_server(capacity=100, chunk=8)
+    bad = Sys("bad", 1)
+    bad.stop_after = 0
+    done = srv.serve([(0.0, Sys("a", 1), 24), (0.0, bad, 30), (0.0, Sys("c", 1), 12)],
+                     clock=iter(np.arange(0, 1000, 0.5)).__next__)
+    by = {d.rid: d for d in done}
+    assert sorted(by) == [0, 1, 2]
+    assert by[1].error == "job 1 stopped" and by[0].error is None and by[2].error is None
+    assert sum(c[1] for c in srv.pool.calls if c[0] == "step") == 24
