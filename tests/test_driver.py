@@ -177,3 +177,137 @@ def test_driver_torque_velocity_constraint_boundary_contact(tmp_path, oracle_lib
             assert "P_00001/applied_force_density" not in got
     finally:
         o.close()
+
+
+def _read_out(path, tmp_path, has_h5, name):
+    """An output container of the driver as a dict of arrays."""
+    if has_h5:
+        dst = tmp_path / f"{name}.pdbin"
+        run_driver("--convert", str(path), str(dst))
+        path = dst
+    return setupio.read_pdbin(path)
+
+
+def _compare_tc(got, states, conns, t, label):
+    for p in range(len(states)):
+        g = f"P_{p:05d}/"
+        for k, key in (("CurrPos", "currpos"), ("vel", "vel"), ("acc", "acc"), ("force", "force")):
+            assert_bitwise(got[g + k], states[p][key], f"{label} step {t} {g}{k}")
+        assert np.array_equal(got[g + "Connectivity"], conns[p]), f"{label} step {t} {g}Connectivity"
+
+
+def _wall_keys(s):
+    return dict(speed_wall_x_min=s.wall_speed[0], speed_wall_y_min=s.wall_speed[1],
+                speed_wall_z_min=s.wall_speed[2], speed_wall_x_max=s.wall_speed[3],
+                speed_wall_y_max=s.wall_speed[4], speed_wall_z_max=s.wall_speed[5])
+
+
+@pytest.mark.gpu
+def test_driver_resume_matches_the_oracle_fed_the_same_reload(tmp_path, oracle_lib):
+    """do_resume / wall_resume (timeloop.h:262-305): the driver reloads
+    tc_00001 -- disp = CurrPos - Pos, vel, acc, and the saved Connectivity
+    rebuilt into rows by conn2NArr -- and wall_00001's wall_info, restarts t
+    at 1 (timeloop.h:394) and writes counters 2 and 3.  The oracle is built
+    from the same files and stepped alike."""
+    s = small_impact(seed=23)
+    s.self_contact, s.enable_fracture = 1, 1
+    setup = tmp_path / "setup.pdbin"
+    setupio.write_setup(s, setup)
+    has_h5 = "hdf5 yes" in run_driver("--features").stdout
+    base = dict(modulo=10, dt=s.dt, enable_fracture=1, self_contact=1, allow_damping=1,
+                allow_friction=1, compute_wall_reaction=1, **_wall_keys(s))
+    out = tmp_path / "out"
+    cfg = tmp_path / "a.conf"
+    setupio.write_config(cfg, timesteps=20, **base)
+    run_driver("-c", str(cfg), "-i", str(setup), "-o", str(out))
+    tc1 = _read_out(out / "tc_00001.h5", tmp_path, has_h5, "tc1")
+    wl1 = _read_out(out / "wall_00001.h5", tmp_path, has_h5, "wl1")
+    cfg = tmp_path / "b.conf"
+    setupio.write_config(cfg, timesteps=20, do_resume=1, resume_ind=1, wall_resume=1, **base)
+    print("[driver] resume from tc_00001", flush=True)
+    r = run_driver("-c", str(cfg), "-i", str(setup), "-o", str(out))
+    assert "Resuming from counter 2" in r.stdout
+    # the oracle fed the reload
+    for p, P in enumerate(s.particles):
+        g = f"P_{p:05d}/"
+        P.disp = tc1[g + "CurrPos"] - P.pos
+        P.vel, P.acc = tc1[g + "vel"], tc1[g + "acc"]
+        P.conn = tc1[g + "Connectivity"].astype(np.uint32).reshape(-1, 2)
+    s.wall_lrtb = tuple(wl1["wall_info"].ravel())
+    o = oracle_lib.engine(s)
+    try:
+        for t in range(1, 21):
+            o.step(1)
+            if t % 10:
+                continue
+            c = 1 + t // 10
+            got = _read_out(out / f"tc_{c:05d}.h5", tmp_path, has_h5, f"tc{c}")
+            _compare_tc(got, [o.state(p) for p in range(2)], [o.connectivity(p) for p in range(2)], t,
+                        "resumed")
+            gw = _read_out(out / f"wall_{c:05d}.h5", tmp_path, has_h5, f"wl{c}")
+            lrtb, reac = o.wall()
+            assert_bitwise(gw["wall_info"].ravel(), lrtb, f"resumed step {t} wall_info")
+            assert_bitwise(gw["reaction"], reac, f"resumed step {t} reaction")
+    finally:
+        o.close()
+
+
+@pytest.mark.gpu
+def test_driver_time_switches_and_shuffled_connectivity(tmp_path, oracle_lib):
+    """set_fracture / set_movable / set_self_contact / set_stoppable /
+    set_damping / set_torque timesteps through the config (timeloop.h:405-495)
+    on a setup file whose Connectivity rows are shuffled and flipped, so the
+    rows are not ascending and self-contact's bond test takes its linear path
+    (load_hdf5.h:27-37, contact.h:783)."""
+    s = small_impact(seed=27)
+    rng = np.random.default_rng(5)
+    P0, P1 = s.particles
+    conn = P0.conn[rng.permutation(len(P0.conn))]
+    flip = rng.random(len(conn)) < 0.5
+    conn[flip] = conn[flip][:, ::-1]
+    P0.conn = np.ascontiguousarray(conn)
+    P1.movable, P1.stoppable = 0, 0
+    P0.torque_axis, P0.torque_val = 1, 1.5e9
+    P1.torque_axis, P1.torque_val = 2, -2.0e9
+    s.enable_fracture = s.self_contact = s.allow_damping = s.enable_torque = 0
+    setup = tmp_path / "setup.pdbin"
+    setupio.write_setup(s, setup)
+    sw = dict(set_fracture_timestep=4, set_movable_index=1, set_movable_timestep=6,
+              set_self_contact_timestep=8, set_stoppable_index=1, set_stoppable_timestep=9,
+              set_damping_timestep=11, set_torque_timestep=13)
+    T, modulo = 24, 8
+    cfg = tmp_path / "main.conf"
+    setupio.write_config(cfg, timesteps=T, modulo=modulo, dt=s.dt, enable_fracture=0, self_contact=0,
+                         allow_damping=0, allow_friction=1, compute_wall_reaction=1, enable_torque=0,
+                         **sw, **_wall_keys(s))
+    out = tmp_path / "out"
+    r = run_driver("-c", str(cfg), "-i", str(setup), "-o", str(out))
+    for msg in ("Setting fracture to 1 on timestep 4", "Setting particle 1 to movable on timestep 6",
+                "Setting self_contact to 1 on timestep 8", "Setting particle 1 to stoppable on timestep 9",
+                "Setting damping to 1 on timestep 11", "Setting torque to 1 on timestep 13"):
+        assert msg in r.stdout, msg
+    has_h5 = "hdf5 yes" in run_driver("--features").stdout
+    o = oracle_lib.engine(s)
+    try:
+        for t in range(1, T + 1):
+            if t == 4:
+                o.set_flag(abi.PD_FLAG_ENABLE_FRACTURE, 1)
+            if t == 6:
+                o.set_particle_flag(1, abi.PD_PFLAG_MOVABLE, 1)
+            if t == 8:
+                o.set_flag(abi.PD_FLAG_SELF_CONTACT, 1)
+            if t == 9:
+                o.set_particle_flag(1, abi.PD_PFLAG_STOPPABLE, 1)
+            if t == 11:
+                o.set_flag(abi.PD_FLAG_ALLOW_DAMPING, 1)
+            if t == 13:
+                o.set_flag(abi.PD_FLAG_ENABLE_TORQUE, 1)
+            o.step(1)
+            if t % modulo:
+                continue
+            got = _read_out(out / f"tc_{t // modulo:05d}.h5", tmp_path, has_h5, "tc")
+            _compare_tc(got, [o.state(p) for p in range(2)], [o.connectivity(p) for p in range(2)], t,
+                        "switched")
+        assert o.counters()["alive_bonds"] < o.counters()["total_bonds"]     # bonds broke after step 4
+    finally:
+        o.close()
