@@ -97,15 +97,16 @@ def fp64_instr(kernel):
         return None
 
 
-def shape_name(k, nodes):
+def shape_name(k, nodes, tol=False):
     """Kernel symbol of bond shape k (pools below 2^27 node slots address
-    node records with 32-bit offsets)."""
+    node records with 32-bit offsets; tol: the tolerance-arithmetic form)."""
     from peridem_amd import abi
     u, pipe, staged = abi.bond_shapes()[k]
+    t = ", true" if tol else ""
     if staged:
-        return f"pd::k_bond_lds<{u}, {pipe}>"
+        return f"pd::k_bond_lds<{u}, {pipe}{t}>"
     off32 = "true" if nodes < (1 << 27) else "false"
-    return f"pd::k_bond_t<{u}, {pipe}, {off32}>"
+    return f"pd::k_bond_t<{u}, {pipe}, {off32}{t}>"
 
 
 def cpu_model():
@@ -235,6 +236,71 @@ def serving(args):
     return out
 
 
+def crush(args):
+    """The crushing aggregate at full size: 512 mixed cubes, spheres and convex
+    polyhedra (2k-12k nodes each, h = 6e-5 m, self-contact, friction,
+    damping, wall reaction), the top wall moving down at 5 m/s until 30% strain
+    of the box height, then all wall speeds set to zero
+    (set_zero_wall_speed_timestep, timeloop.h:447-462) and the packing left to
+    relax.  Every chunk of steps records the top wall's travel, the reaction
+    on the top and bottom walls (contact.h:255-272: -sum V f over the nodes in
+    contact), the fraction of directed bonds broken, the particle pairs that
+    passed the box test per step and the device time per step."""
+    from peridem_amd import abi, gen
+    s = gen.crushing_aggregate(512)
+    H0 = s.wall_lrtb[5] - s.wall_lrtb[2]
+    speed = -s.wall_speed[5]
+    n30 = int(round(0.3 * H0 / (speed * s.dt)))
+    end = n30 + args.crush_relax
+    log(f"[crush] {len(s.particles)} particles, {s.nodes} nodes, {s.bonds} directed bonds; box height "
+        f"{H0:.4e} m; 30% strain after {n30} steps, then {args.crush_relax} steps at rest")
+    e = abi.gpu_engine(s)
+    rows, t, dev_ms = [], 0, 0.0
+    c0 = e.counters()
+    t_start = time.perf_counter()
+    try:
+        while t < end:
+            if t == n30:
+                e.set_wall_speed([0.0] * 6)
+            n = min(args.crush_chunk, (n30 if t < n30 else end) - t)
+            e.step(n)
+            t += n
+            dev_ms += e.last_step_ms()
+            lrtb, reac = e.wall()
+            c = e.counters()
+            row = {"step": t, "wall_travel_m": s.wall_lrtb[5] - lrtb[5],
+                   "strain": (s.wall_lrtb[5] - lrtb[5]) / H0,
+                   "reaction_top_z_N": float(reac[5][2]), "reaction_bottom_z_N": float(reac[2][2]),
+                   "broken_fraction": 1.0 - c["alive_bonds"] / c["total_bonds"],
+                   "pairs_per_step": (c["pair_computations"] - c0["pair_computations"]) / n,
+                   "ms_per_step": e.last_step_ms() / n, "list_capacity": c["list_capacity"]}
+            c0 = c
+            rows.append(row)
+            log(f"[crush] step {t}: strain {row['strain']:.4f}, top reaction {row['reaction_top_z_N']:.4e} N, "
+                f"broken {row['broken_fraction']:.3e}, pairs/step {row['pairs_per_step']:.0f}, "
+                f"{row['ms_per_step']:.3f} ms/step, list capacity {row['list_capacity']}")
+    finally:
+        cnt = e.counters()
+        e.close()
+    wall_s = time.perf_counter() - t_start
+    out = {"mode": "crush", "workload": "crushing aggregate, 512 cubes/spheres/convex polyhedra of 2k-12k "
+                                        "nodes, h=6e-5 m, delta=3h, self-contact, top wall -5 m/s to 30% "
+                                        "strain then at rest, FP64, dt=1e-8",
+           "nodes": s.nodes, "directed_bonds": s.bonds, "box_height_m": H0, "steps_to_30pct": n30,
+           "relax_steps": args.crush_relax, "steps": t, "wall_s": wall_s,
+           "device_s": dev_ms * 1e-3, "bond_updates_per_s": s.movable_bonds() * t / (dev_ms * 1e-3),
+           "list_regrows": cnt["list_regrows"], "cell_rebuilds": cnt["cell_rebuilds"], "trace": rows}
+    print(json.dumps(out), flush=True)
+    if args.crush_md:
+        with open(args.crush_md, "w") as f:
+            f.write("| step | top wall travel (m) | strain | top wall reaction z (N) | bottom wall reaction z (N) "
+                    "| broken bonds | pairs/step | ms/step |\n|---|---|---|---|---|---|---|---|\n")
+            for r in rows:
+                f.write(f"| {r['step']} | {r['wall_travel_m']:.4e} | {r['strain']:.4f} | "
+                        f"{r['reaction_top_z_N']:.4e} | {r['reaction_bottom_z_N']:.4e} | "
+                        f"{r['broken_fraction']:.4e} | {r['pairs_per_step']:.0f} | {r['ms_per_step']:.3f} |\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +319,8 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tolerance", action="store_true",
+                    help="skip the tolerance-arithmetic line")
     ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--serve-rates", type=lambda v: [float(x) for x in v.split(",")],
                     default=[2.0, 8.0, 50.0])
@@ -265,12 +333,22 @@ def main():
     ap.add_argument("--bond-shape", type=int, default=-1,
                     help="force this bond-kernel shape instead of the autotuner's choice "
                          "(profiling passes pin one so that every pass times the same kernel)")
+    ap.add_argument("--crush", action="store_true",
+                    help="instead of the benchmark: the crushing aggregate at full size to 30%% "
+                         "strain and a relaxation, printing its force-displacement trace")
+    ap.add_argument("--crush-chunk", type=int, default=1000)
+    ap.add_argument("--crush-relax", type=int, default=10000)
+    ap.add_argument("--crush-md", default="", help="also write the trace as a markdown table here")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "note only (particles shard by spatial block with a surface "
                                    "halo exchange)", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.crush:
+        crush(args)
+        return
 
     probe = None
     if args.copy_probe_mib:
@@ -326,6 +404,38 @@ def main():
     cnt = e.counters()
     kname = shape_name(e.bond_shape(), N)
     fi = fp64_instr(kname)
+    tol = None
+    if not args.no_tolerance:
+        # the same pool in tolerance arithmetic (its shapes tuned again at the
+        # first step), timed the same way
+        e.set_arithmetic(abi.PD_ARITH_TOLERANCE)
+        e.step(args.warmup)
+        e.set_profiling(abi.PD_PROFILE_BOND)
+        c0 = e.counters()
+        t0 = time.perf_counter()
+        e.step(args.steps)
+        el = time.perf_counter() - t0
+        c1 = e.counters()
+        pt = e.profile()
+        e.set_profiling(0)
+        tms = pt["bond"]["ms"] / max(pt["bond"]["launches"], 1)
+        ta = force_bytes / (tms * 1e-3) / 1e9
+        tk = shape_name(e.bond_shape(), N, tol=True)
+        tfi = fp64_instr(tk)
+        tol = {"arithmetic": "PD_ARITH_TOLERANCE: |e| and |xi| from v_rsq_f64 with one Newton-Raphson "
+                             "step, divisions as products with those reciprocals",
+               "tolerance": "per step and node |f - f_ref| <= 1e-12 * c0 * sum_j V_j (a stretch error of "
+                            "1e-12); broken set exact for |s - s0| > 1e-12 (tests/test_gpu_tolerance.py)",
+               "value": (c1["bond_updates"] - c0["bond_updates"]) / el, "unit": "bond-updates/s",
+               "ms_per_step": el * 1e3 / args.steps,
+               "step_hbm_fraction": step_bytes / (el / args.steps) / (HBM_PEAK_GBS * 1e9),
+               "roofline": {"bound": "hbm", "achieved": ta, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": ta / HBM_PEAK_GBS, "kernel": tk, "bytes_per_launch": force_bytes,
+                            "avg_launch_ms": tms, "target_frac": 0.6},
+               "fp64_issue": None if tfi is None else {
+                   "wave_instr_per_launch": tfi, "frac": tfi * 64 / (tms * 1e-3) / FP64_LANE_INSTR_PEAK}}
+        log(f"[bench] tolerance arithmetic: {tol['ms_per_step']:.4f} ms/step, bond kernel {tms:.4f} ms "
+            f"({tk}), {ta:.0f} GB/s = {ta / HBM_PEAK_GBS:.3f} of HBM peak")
     e.close()
 
     if probe is not None:
@@ -372,6 +482,7 @@ def main():
             "wave_instr_per_launch": fi, "achieved": fi * 64 / (force_ms * 1e-3) / 1e12,
             "peak": FP64_LANE_INSTR_PEAK / 1e12, "unit": "T FP64 lane-instructions/s",
             "frac": fi * 64 / (force_ms * 1e-3) / FP64_LANE_INSTR_PEAK},
+        "tolerance_mode": tol,
         "cpu_baseline": cpu,
         "batched": bat,
         "serving": srv,

@@ -154,7 +154,9 @@ int pd_get_applied_force(pd_ctx* ctx, uint32_t p, double* afd);
  * Arrays are [nnodes*3].  force is the force density, as in the reference. */
 int pd_get_state(pd_ctx* ctx, uint32_t p, double* disp, double* currpos,
                  double* vel, double* acc, double* force);
-/* Overwrite state (resume, timeloop.h:262-291).  Any pointer may be NULL. */
+/* Overwrite state (resume, timeloop.h:262-291).  Any pointer may be NULL.
+ * A new disp also resets the particle's lagged box to the min/max of
+ * pos + disp, as a resumed run starts (timeloop.h:389-393). */
 int pd_set_state(pd_ctx* ctx, uint32_t p, const double* disp,
                  const double* vel, const double* acc);
 /* Intact bonds as NbdArr2conn emits them (timeloop.h:20-44): pairs (i,q)
@@ -245,6 +247,24 @@ int pd_set_graphs(pd_ctx* ctx, int on);
  * at the next step); pd_get_bond_shape reports the shape launches use.
  * Shape index pd_bond_shape_count() (never tuned) is the unstaged kernel with
  * 64-bit record offsets that large pools use; it can be forced on any pool. */
+/* Bond-term arithmetic.  No counterpart in the reference (a CPU code using
+ * IEEE sqrt and division).  PD_ARITH_BITWISE (default): correctly rounded
+ * square roots and divisions in the reference's operation order, results
+ * bitwise the reference's.  PD_ARITH_TOLERANCE: |e| and |xi| come from the
+ * hardware reciprocal square root refined by one Newton-Raphson step and the
+ * divisions become multiplications by those reciprocals, so every stretch is
+ * within ~1e-14 (absolute) of the bitwise one: per step, each node's force
+ * density agrees with the reference's to 1e-12 x c0 sum_j V_j over its row
+ * (a stretch error of 1e-12), and the broken-bond set is exact for every
+ * bond whose stretch lies farther than 1e-12 from s0.  Contact, wall,
+ * external-force and Verlet arithmetic stay bitwise.  Changing the mode
+ * re-tunes the bond shape at the next step (a forced shape is dropped);
+ * pools of 2^27 node slots or more keep the bitwise kernel. */
+#define PD_ARITH_BITWISE 0
+#define PD_ARITH_TOLERANCE 1
+int pd_set_arithmetic(pd_ctx* ctx, int mode);
+int pd_get_arithmetic(pd_ctx* ctx, int* mode);
+
 int pd_bond_shape_count(void);
 int pd_bond_shape_info(int shape, int* unroll, int* pipelined, int* staged);
 int pd_set_bond_shape(pd_ctx* ctx, int shape);

@@ -29,6 +29,8 @@ def load():
         lib = C.CDLL(LIB)
         lib.or_set_ranks.argtypes = [C.c_void_p, C.c_int]
         lib.or_set_ranks.restype = C.c_int
+        lib.or_set_prune.argtypes = [C.c_void_p, C.c_int]
+        lib.or_set_prune.restype = C.c_int
         lib.or_get_stretch.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_double), C.c_uint64,
                                        C.POINTER(C.c_uint64)]
         lib.or_get_stretch.restype = C.c_int
@@ -38,12 +40,16 @@ def load():
     return _LIB
 
 
-def engine(system, ranks=1):
+def engine(system, ranks=1, prune=False):
+    """prune: contact candidates from a cell grid (bitwise the same results,
+    for large dense systems); off: the reference's all-pairs loops."""
     from peridem_amd.abi import Engine
     lib, fns = load()
     e = Engine(lib, fns, system, "or_")
     if ranks != 1:
         e._check(lib.or_set_ranks(e.ctx, ranks), "set_ranks")
+    if prune:
+        e._check(lib.or_set_prune(e.ctx, 1), "set_prune")
     return e
 
 

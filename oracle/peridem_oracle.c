@@ -69,6 +69,7 @@ struct or_ctx {
   double* gfr;    /* per-rank buffers when nranks > 1 */
   uint32_t t;
   int committed, nranks;
+  int prune;      /* contact candidates from a cell grid (or_set_prune) */
   pd_counters cnt;
   char err[256];
   /* prescribed velocity (timeloop.h:685-764): CFGV.particle_w_fixed_velocity,
@@ -299,83 +300,171 @@ static int row_has(const orow* r, uint32_t j) {
   return 0;
 }
 
+/* update_contact_force_by_boundary (contact.h:746-942): the terms of one
+ * node pair, i of P_C = C and j of P_N = N, added to giant_f.  The distance
+ * and self-contact tests come first, as in the reference's loop body. */
+static void contact_pair(const or_ctx* c, const opart* C, const opart* N, double* gf, uint64_t ioff,
+                         uint64_t joff, int self, uint32_t i, uint32_t j) {
+  const pd_run_desc* R = &c->run;
+  const double Rc = R->contact_rad, dt = R->dt;
+  const double* Xi = C->pos + 3 * (size_t)i;
+  const double* Ui = C->disp + 3 * (size_t)i;
+  const double* Xj = N->pos + 3 * (size_t)j;
+  const double* Uj = N->disp + 3 * (size_t)j;
+  double xi[3], d[3];
+  for (int k = 0; k < 3; k++) xi[k] = Xj[k] - Xi[k];
+  for (int k = 0; k < 3; k++) d[k] = (Uj[k] - Ui[k]) + xi[k];
+  double dn = nrm3(d[0], d[1], d[2]);
+  if (!(dn <= Rc)) return;
+  int do_compute = 1;
+  double xin = nrm3(xi[0], xi[1], xi[2]);
+  double len = Rc;
+  double K = R->normal_stiffness * Rc;
+  if (self) {
+    if (i == j) do_compute = 0;
+    else if (row_has(&C->rows[i], j)) do_compute = 0;
+    else if (xin < Rc) {
+      len = xin;
+      K = N->cnot;
+    } else
+      len = Rc;
+  }
+  if (!(do_compute && dn < len)) return;
+  double Vn = N->vol[j];
+  double u[3] = {d[0] / dn, d[1] / dn, d[2] / dn};
+  double s = (dn - len) / len;
+  double fc[3];
+  for (int k = 0; k < 3; k++) fc[k] = ((K * s) * Vn) * u[k];
+  double fcn = nrm3(fc[0], fc[1], fc[2]);
+  double* Fi = gf + 3 * (ioff + i);
+  double* Fj = gf + 3 * (joff + j);
+  if (C->stoppable)
+    for (int k = 0; k < 3; k++) Fi[k] += fc[k];
+  if (!self && N->stoppable)
+    for (int k = 0; k < 3; k++) Fj[k] -= (fc[k] / Vn) * C->vol[i];
+  if (R->allow_damping) {
+    double rv[3];
+    for (int k = 0; k < 3; k++) rv[k] = C->vel[3 * (size_t)i + k] - N->vel[3 * (size_t)j + k];
+    double pr = dot3(rv, u);
+    double dc = ((2 * R->damping_ratio) * sqrt(K * C->rho)) * sqrt(Vn);
+    double fd[3];
+    for (int k = 0; k < 3; k++) fd[k] = (-dc) * (pr * u[k]);
+    if (C->stoppable)
+      for (int k = 0; k < 3; k++) Fi[k] += fd[k];
+    if (!self && N->stoppable)
+      for (int k = 0; k < 3; k++) Fj[k] -= (fd[k] / sqrt(Vn)) * sqrt(C->vol[i]);
+  }
+  if (R->allow_friction) {
+    double rv[3];
+    for (int k = 0; k < 3; k++) rv[k] = C->vel[3 * (size_t)i + k] - N->vel[3 * (size_t)j + k];
+    double pr = dot3(rv, u);
+    double tv[3];
+    for (int k = 0; k < 3; k++) tv[k] = rv[k] - pr * u[k];
+    double tn = nrm3(tv[0], tv[1], tv[2]);
+    if (tn > 0) {
+      double tu[3], ff[3];
+      for (int k = 0; k < 3; k++) tu[k] = tv[k] / tn;
+      for (int k = 0; k < 3; k++) ff[k] = ((-R->friction_coefficient) * fcn) * tu[k];
+      if (tn < ((R->friction_coefficient * fcn) * dt) / C->rho)
+        for (int k = 0; k < 3; k++) ff[k] = (((-tn) / C->rho) / dt) * tu[k];
+      if (C->stoppable)
+        for (int k = 0; k < 3; k++) Fi[k] += ff[k];
+      if (!self && N->stoppable)
+        for (int k = 0; k < 3; k++) Fj[k] -= ff[k];
+    }
+  }
+}
+
+/* Candidate pruning (or_set_prune, off by default): the nodes of N are put
+ * in cells of edge R_c (1 + 1e-6) by their current position, and node i
+ * visits only the members of its 27 neighbouring cells, in the full loop's
+ * own order (position in N's node list or boundary list).  Every pair within
+ * R_c -- rounding moves computed distances by ~1e-17 m against a margin of
+ * 1e-6 R_c -- is met, in the same order, so the sums are bitwise those of
+ * the full loop; only pairs the distance test would reject are skipped. */
+typedef struct {
+  int64_t x, y, z;
+  uint32_t q;
+} ocell;
+
+static int ocell_cmp(const void* a, const void* b) {
+  const ocell *u = (const ocell*)a, *v = (const ocell*)b;
+  if (u->x != v->x) return u->x < v->x ? -1 : 1;
+  if (u->y != v->y) return u->y < v->y ? -1 : 1;
+  if (u->z != v->z) return u->z < v->z ? -1 : 1;
+  return u->q < v->q ? -1 : (u->q > v->q);
+}
+
+static int u32_cmp(const void* a, const void* b) {
+  const uint32_t u = *(const uint32_t*)a, v = *(const uint32_t*)b;
+  return u < v ? -1 : (u > v);
+}
+
+static void contact_pruned(const or_ctx* c, const opart* C, const opart* N, double* gf, uint64_t ioff,
+                           uint64_t joff, int self) {
+  const pd_run_desc* R = &c->run;
+  const double cell = R->contact_rad * (1 + 1e-6);
+  const uint32_t tni = R->nl_bdry_only ? C->nbdry : C->n;
+  const uint32_t tnj = R->nl_bdry_only ? N->nbdry : N->n;
+  if (!tni || !tnj) return;
+  ocell* tab = (ocell*)malloc(tnj * sizeof(ocell));
+  uint32_t* cand = (uint32_t*)malloc(tnj * sizeof(uint32_t));
+  for (uint32_t qq = 0; qq < tnj; qq++) {
+    const uint32_t j = R->nl_bdry_only ? N->bdry[qq] : qq;
+    const double* X = N->pos + 3 * (size_t)j;
+    const double* U = N->disp + 3 * (size_t)j;
+    tab[qq].x = (int64_t)floor((X[0] + U[0]) / cell);
+    tab[qq].y = (int64_t)floor((X[1] + U[1]) / cell);
+    tab[qq].z = (int64_t)floor((X[2] + U[2]) / cell);
+    tab[qq].q = qq;
+  }
+  qsort(tab, tnj, sizeof(ocell), ocell_cmp);
+  for (uint32_t pp = 0; pp < tni; pp++) {
+    const uint32_t i = R->nl_bdry_only ? C->bdry[pp] : pp;
+    const double* X = C->pos + 3 * (size_t)i;
+    const double* U = C->disp + 3 * (size_t)i;
+    const int64_t cx = (int64_t)floor((X[0] + U[0]) / cell);
+    const int64_t cy = (int64_t)floor((X[1] + U[1]) / cell);
+    const int64_t cz = (int64_t)floor((X[2] + U[2]) / cell);
+    uint32_t nc = 0;
+    for (int dx = -1; dx <= 1; dx++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dz = -1; dz <= 1; dz++) {
+          const ocell key = {cx + dx, cy + dy, cz + dz, 0};
+          size_t lo = 0, hi = tnj;   /* first entry >= key */
+          while (lo < hi) {
+            const size_t mid = (lo + hi) / 2;
+            if (ocell_cmp(&tab[mid], &key) < 0) lo = mid + 1;
+            else hi = mid;
+          }
+          for (; lo < tnj && tab[lo].x == key.x && tab[lo].y == key.y && tab[lo].z == key.z; lo++)
+            cand[nc++] = tab[lo].q;
+        }
+    qsort(cand, nc, sizeof(uint32_t), u32_cmp);
+    for (uint32_t k = 0; k < nc; k++) {
+      const uint32_t j = R->nl_bdry_only ? N->bdry[cand[k]] : cand[k];
+      contact_pair(c, C, N, gf, ioff, joff, self, i, j);
+    }
+  }
+  free(cand);
+  free(tab);
+}
+
 /* update_contact_force_by_boundary (contact.h:746-942).  C = P_C, N = P_N. */
 static void contact(const or_ctx* c, const opart* C, const opart* N, double* gf,
                     uint64_t ioff, uint64_t joff, int self) {
+  if (c->prune) {
+    contact_pruned(c, C, N, gf, ioff, joff, self);
+    return;
+  }
   const pd_run_desc* R = &c->run;
-  const double Rc = R->contact_rad, dt = R->dt;
   uint32_t tni = R->nl_bdry_only ? C->nbdry : C->n;
   uint32_t tnj = R->nl_bdry_only ? N->nbdry : N->n;
   for (uint32_t pp = 0; pp < tni; pp++) {
     uint32_t i = R->nl_bdry_only ? C->bdry[pp] : pp;
-    const double* Xi = C->pos + 3 * (size_t)i;
-    const double* Ui = C->disp + 3 * (size_t)i;
     for (uint32_t qq = 0; qq < tnj; qq++) {
       uint32_t j = R->nl_bdry_only ? N->bdry[qq] : qq;
-      const double* Xj = N->pos + 3 * (size_t)j;
-      const double* Uj = N->disp + 3 * (size_t)j;
-      double xi[3], d[3];
-      for (int k = 0; k < 3; k++) xi[k] = Xj[k] - Xi[k];
-      for (int k = 0; k < 3; k++) d[k] = (Uj[k] - Ui[k]) + xi[k];
-      double dn = nrm3(d[0], d[1], d[2]);
-      if (!(dn <= Rc)) continue;
-      int do_compute = 1;
-      double xin = nrm3(xi[0], xi[1], xi[2]);
-      double len = Rc;
-      double K = R->normal_stiffness * Rc;
-      if (self) {
-        if (i == j) do_compute = 0;
-        else if (row_has(&C->rows[i], j)) do_compute = 0;
-        else if (xin < Rc) {
-          len = xin;
-          K = N->cnot;
-        } else
-          len = Rc;
-      }
-      if (!(do_compute && dn < len)) continue;
-      double Vn = N->vol[j];
-      double u[3] = {d[0] / dn, d[1] / dn, d[2] / dn};
-      double s = (dn - len) / len;
-      double fc[3];
-      for (int k = 0; k < 3; k++) fc[k] = ((K * s) * Vn) * u[k];
-      double fcn = nrm3(fc[0], fc[1], fc[2]);
-      double* Fi = gf + 3 * (ioff + i);
-      double* Fj = gf + 3 * (joff + j);
-      if (C->stoppable)
-        for (int k = 0; k < 3; k++) Fi[k] += fc[k];
-      if (!self && N->stoppable)
-        for (int k = 0; k < 3; k++) Fj[k] -= (fc[k] / Vn) * C->vol[i];
-      if (R->allow_damping) {
-        double rv[3];
-        for (int k = 0; k < 3; k++) rv[k] = C->vel[3 * (size_t)i + k] - N->vel[3 * (size_t)j + k];
-        double pr = dot3(rv, u);
-        double dc = ((2 * R->damping_ratio) * sqrt(K * C->rho)) * sqrt(Vn);
-        double fd[3];
-        for (int k = 0; k < 3; k++) fd[k] = (-dc) * (pr * u[k]);
-        if (C->stoppable)
-          for (int k = 0; k < 3; k++) Fi[k] += fd[k];
-        if (!self && N->stoppable)
-          for (int k = 0; k < 3; k++) Fj[k] -= (fd[k] / sqrt(Vn)) * sqrt(C->vol[i]);
-      }
-      if (R->allow_friction) {
-        double rv[3];
-        for (int k = 0; k < 3; k++) rv[k] = C->vel[3 * (size_t)i + k] - N->vel[3 * (size_t)j + k];
-        double pr = dot3(rv, u);
-        double tv[3];
-        for (int k = 0; k < 3; k++) tv[k] = rv[k] - pr * u[k];
-        double tn = nrm3(tv[0], tv[1], tv[2]);
-        if (tn > 0) {
-          double tu[3], ff[3];
-          for (int k = 0; k < 3; k++) tu[k] = tv[k] / tn;
-          for (int k = 0; k < 3; k++) ff[k] = ((-R->friction_coefficient) * fcn) * tu[k];
-          if (tn < ((R->friction_coefficient * fcn) * dt) / C->rho)
-            for (int k = 0; k < 3; k++) ff[k] = (((-tn) / C->rho) / dt) * tu[k];
-          if (C->stoppable)
-            for (int k = 0; k < 3; k++) Fi[k] += ff[k];
-          if (!self && N->stoppable)
-            for (int k = 0; k < 3; k++) Fj[k] -= ff[k];
-        }
-      }
+      contact_pair(c, C, N, gf, ioff, joff, self, i, j);
     }
   }
 }
@@ -641,6 +730,12 @@ int or_step(or_ctx* c, uint32_t nsteps) {
   return PD_OK;
 }
 
+int or_set_prune(or_ctx* c, int on) {
+  if (!c) return PD_E_ARG;
+  c->prune = on != 0;
+  return PD_OK;
+}
+
 int or_set_ranks(or_ctx* c, int nranks) {
   if (!c || nranks < 1) return PD_E_ARG;
   if (nranks != c->nranks) {
@@ -712,7 +807,14 @@ int or_set_state(or_ctx* c, uint32_t pi, const double* disp, const double* vel, 
   if (!c || pi >= c->np) return PD_E_ARG;
   opart* p = &c->P[pi];
   size_t b = 3ull * p->n * sizeof(double);
-  if (disp) memcpy(p->disp, disp, b);
+  if (disp) {
+    memcpy(p->disp, disp, b);
+    /* a resumed run starts from lrtb = minmax(pos + disp) (timeloop.h:389-393) */
+    if (c->committed) {
+      for (uint64_t k = 0; k < 3ull * p->n; k++) p->cp[k] = p->pos[k] + p->disp[k];
+      minmax(p, c->lrtb[pi]);
+    }
+  }
   if (vel) memcpy(p->vel, vel, b);
   if (acc) memcpy(p->acc, acc, b);
   return PD_OK;

@@ -30,6 +30,11 @@ int or_step(or_ctx* ctx, uint32_t nsteps);
  * zeroed force buffer, buffers are summed in rank order (timeloop.h:544,
  * 637, 658-659).  nranks=1 (default) is the bitwise reference order. */
 int or_set_ranks(or_ctx* ctx, int nranks);
+/* Contact candidates from a cell grid over the current positions instead of
+ * every node pair, visited in the full loop's order: bitwise the same sums,
+ * far fewer distance tests on large dense systems.  Off by default (the
+ * reference's all-pairs loop, which the CPU baseline times). */
+int or_set_prune(or_ctx* ctx, int on);
 int or_set_flag(or_ctx* ctx, int flag, int value);
 int or_set_wall_speed(or_ctx* ctx, const double speed[6]);
 int or_set_wall(or_ctx* ctx, const double lrtb[6]);

@@ -95,6 +95,8 @@ class PDError(RuntimeError):
 
 
 PD_E_OVERFLOW = -5
+PD_ARITH_BITWISE = 0
+PD_ARITH_TOLERANCE = 1
 
 
 def _bind(lib, prefix):
@@ -178,6 +180,8 @@ def load_gpu_lib(path=GPU_LIB):
                 ("pd_job_get_connectivity", C.c_int, [P, _u32, _u32, _pu, _u64, C.POINTER(_u64)]),
                 ("pd_job_set_flag", C.c_int, [P, _u32, C.c_int, C.c_int]),
                 ("pd_set_graphs", C.c_int, [P, C.c_int]),
+                ("pd_set_arithmetic", C.c_int, [P, C.c_int]),
+                ("pd_get_arithmetic", C.c_int, [P, C.POINTER(C.c_int)]),
                 ("pd_bond_shape_count", C.c_int, []),
                 ("pd_bond_shape_info", C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                                  C.POINTER(C.c_int)]),
@@ -293,6 +297,9 @@ class Engine:
 
     def set_graphs(self, on=True):
         self._check(self.lib.pd_set_graphs(self.ctx, int(on)), "set_graphs")
+
+    def set_arithmetic(self, mode):
+        self._check(self.lib.pd_set_arithmetic(self.ctx, mode), "set_arithmetic")
 
     def set_bond_shape(self, shape):
         """Force a bond-kernel shape (see bond_shapes()); -1 = autotune."""

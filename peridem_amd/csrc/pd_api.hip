@@ -161,6 +161,7 @@ struct pd_ctx {
   bool use_graph = true, graph_dirty = true;
   uint64_t hwm_slices = 0;      // end of the highest node range in use
   int bond_shape = -1;          // index into bond_shapes() or kWideShape; -1: tune at the next step
+  int arith = PD_ARITH_BITWISE; // bond-term arithmetic (pd_set_arithmetic)
   uint32_t lds_bad = 0;         // resident jobs whose block tables exceed kLdsCap
   uint64_t tuned_nodes = 0;     // resident nodes when the shape was tuned
   double tune_ms[kBondShapes] = {};
@@ -1188,9 +1189,16 @@ static Dev step_view(const pd_ctx* c) {
 // Node records of pools this large are addressed with 64-bit offsets.
 static bool wide_pool(const pd_ctx* c) { return c->cap_slices * kSlice > kOff32MaxNodes; }
 
-// The staged (LDS) shapes need every resident job's block tables to fit.
+// The shape table of the context's arithmetic.
+static const BondShape* shapes(const pd_ctx* c) {
+  return c->arith == PD_ARITH_TOLERANCE ? bond_shapes_tol() : bond_shapes();
+}
+
+// A shape is usable if it is built in the context's arithmetic; the staged
+// (LDS) shapes also need every resident job's block tables to fit.
 static bool shape_usable(const pd_ctx* c, int v) {
-  return !bond_shapes()[v].lds || (c->lds_bad == 0 && !wide_pool(c));
+  const BondShape& b = shapes(c)[v];
+  return b.fn && (!b.lds || (c->lds_bad == 0 && !wide_pool(c)));
 }
 
 // The shape a launch uses: the selected one while usable, else the default.
@@ -1203,7 +1211,7 @@ static int effective_shape(const pd_ctx* c) {
 
 static BondShape launch_shape(const pd_ctx* c) {
   const int v = effective_shape(c);
-  return wide_pool(c) || v == kWideShape ? bond_wide() : bond_shapes()[v];
+  return wide_pool(c) || v == kWideShape ? bond_wide() : shapes(c)[v];
 }
 
 // Phase times of one profiled step (waits for its last mark).
@@ -1347,9 +1355,9 @@ static int autotune(pd_ctx* c, uint32_t reps) {
         c->tune_ms[v] = -1.0;   // not timed
         continue;
       }
-      hipLaunchKernelGGL(bond_shapes()[v].fn, grid, dim3(256), 0, c->stream, t);   // warm-up
+      hipLaunchKernelGGL(shapes(c)[v].fn, grid, dim3(256), 0, c->stream, t);   // warm-up
       HIPCHK(c, hipEventRecord(e0, c->stream));
-      for (uint32_t r = 0; r < reps; r++) hipLaunchKernelGGL(bond_shapes()[v].fn, grid, dim3(256), 0, c->stream, t);
+      for (uint32_t r = 0; r < reps; r++) hipLaunchKernelGGL(shapes(c)[v].fn, grid, dim3(256), 0, c->stream, t);
       HIPCHK(c, hipEventRecord(e1, c->stream));
       HIPCHK(c, hipGetLastError());
       HIPCHK(c, hipEventSynchronize(e1));
@@ -1578,6 +1586,23 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
 }
 
 int pd_bond_shape_count(void) { return kBondShapes; }
+
+int pd_set_arithmetic(pd_ctx* c, int mode) {
+  if (!c) return PD_E_ARG;
+  if (mode != PD_ARITH_BITWISE && mode != PD_ARITH_TOLERANCE) return fail(c, PD_E_ARG, "no arithmetic mode %d", mode);
+  if (mode != c->arith) {
+    c->arith = mode;
+    c->bond_shape = -1;   // the other table's shapes: tuned again at the next step
+    c->graph_dirty = true;
+  }
+  return PD_OK;
+}
+
+int pd_get_arithmetic(pd_ctx* c, int* mode) {
+  if (!c || !mode) return PD_E_ARG;
+  *mode = c->arith;
+  return PD_OK;
+}
 
 int pd_bond_shape_info(int shape, int* unroll, int* pipelined, int* staged) {
   if (shape < 0 || shape > kWideShape) return PD_E_ARG;
@@ -1907,6 +1932,17 @@ int pd_set_state(pd_ctx* c, uint32_t pi, const double* disp, const double* vel, 
     HIPCHK(c, hipMemcpy(c->d.U + p.off, U.data(), p.n * sizeof(double4), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_split, dim3(grid_of(p.n, 256)), dim3(256), 0, c->stream, c->d, p.off, (uint64_t)p.n);
     HIPCHK(c, hipGetLastError());
+    // a resumed run starts from lrtb = min/max of pos + disp (timeloop.h:389-393)
+    double o[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t v = 0; v < p.n; v++)
+      for (int k = 0; k < 3; k++) {
+        const double cp = p.pos[3 * v + k] + disp[3 * v + k];
+        if (v == 0) o[k] = o[3 + k] = cp;
+        o[k] = (cp < o[k]) ? cp : o[k];
+        o[3 + k] = (o[3 + k] < cp) ? cp : o[3 + k];
+      }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d.lrtb + 6 * (uint64_t)p.slot, o, sizeof o, hipMemcpyHostToDevice));
   }
   if (vel) HIPCHK(c, hipMemcpy(c->d.V + 3 * p.off, vel, 3ull * p.n * sizeof(double), hipMemcpyHostToDevice));
   if (acc) HIPCHK(c, hipMemcpy(c->d.A + 3 * p.off, acc, 3ull * p.n * sizeof(double), hipMemcpyHostToDevice));

@@ -1099,14 +1099,34 @@ struct LdsRows {
   }
 };
 
+// Tolerance arithmetic: 1/sqrt(x) from the hardware estimate (relative error
+// below 2^-24 on gfx950) refined by kTolNewton Newton-Raphson steps
+// y <- y + y (1/2 - (x/2) y^2), each taken with two fused multiply-adds; one
+// step leaves a relative error below 2^-47.  |e| = |e|^2 y_e and |xi| =
+// |xi|^2 y_xi then carry that error, so the stretch (|e| - |xi|) y_xi is
+// within ~1e-14 (absolute) of the correctly rounded one.
+constexpr int kTolNewton = 1;
+__device__ __forceinline__ double rsq_refined(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+#pragma unroll
+  for (int k = 0; k < kTolNewton; k++) {
+    const double t = __builtin_fma(-(h * y), y, 0.5);
+    y = __builtin_fma(y, t, y);
+  }
+  return y;
+}
+
 // Terms of one batch, added to (f0, f1, f2) in row order; breaks recorded.
+// TOL: the tolerance arithmetic above in the fast domain instead of the
+// correctly rounded square roots and divisions (the IEEE branch is shared).
 //
 // Fast domain (every lane of the wave): a dead entry carries the node's own
 // record, so with finite operands its e and xi are exactly +0 and |e|^2 =
 // |xi|^2 = +0; giving both the high word of 1.0 makes them exactly 1.0, the
 // stretch exactly +0 and the coefficient +-0 without a select on it.  The
 // IEEE branch (safe mode, or a lane outside the domain) selects explicitly.
-template <int U, class Src>
+template <int U, bool TOL, class Src>
 __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint32_t (&e)[U], const double4 (&xj)[U],
                                            const double4 (&uj)[U], const double4& xi, const double4& ui, bool fast,
                                            bool can_break, double cnot, double snot, double& f0, double& f1,
@@ -1131,14 +1151,25 @@ __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint3
   }
   double str[U], t[U][3];
   if (__all(fast & (over < kN2Span))) {
+    if (TOL) {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const SqrtRcp sn = sqrt_rcp_core(n2[u]);
-      const SqrtRcp sl = sqrt_rcp_core(l2[u]);
-      str[u] = div_core(sn.s - sl.s, sl.r);
-      double c = (cnot * str[u]) * xj[u].w;
+      for (int u = 0; u < U; u++) {
+        const double re = rsq_refined(n2[u]), rl = rsq_refined(l2[u]);
+        str[u] = (n2[u] * re - l2[u] * rl) * rl;
+        const double c = ((cnot * str[u]) * xj[u].w) * re;
 #pragma unroll
-      for (int q = 0; q < 3; q++) t[u][q] = c * div_core(ev[u][q], sn.r);
+        for (int q = 0; q < 3; q++) t[u][q] = c * ev[u][q];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const SqrtRcp sn = sqrt_rcp_core(n2[u]);
+        const SqrtRcp sl = sqrt_rcp_core(l2[u]);
+        str[u] = div_core(sn.s - sl.s, sl.r);
+        double c = (cnot * str[u]) * xj[u].w;
+#pragma unroll
+        for (int q = 0; q < 3; q++) t[u][q] = c * div_core(ev[u][q], sn.r);
+      }
     }
   } else {
 #pragma unroll
@@ -1168,7 +1199,7 @@ __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint3
 // evaluated, so the index stream (HBM) and the gathers (L2 or LDS) overlap the
 // FP64 arithmetic instead of stalling every batch; 2: the same, unrolled
 // twice so the records alternate between two register sets.
-template <int U, int PIPE, class Src>
+template <int U, int PIPE, bool TOL, class Src>
 __device__ __forceinline__ void bond_rows(const Src& S, uint32_t W, const double4& xi, const double4& ui, bool fast,
                                           bool can_break, double cnot, double snot, double& f0, double& f1,
                                           double& f2) {
@@ -1181,7 +1212,7 @@ __device__ __forceinline__ void bond_rows(const Src& S, uint32_t W, const double
       bond_idx<U>(R, k, W, pad, e);
       S.template resolve<U>(e, a);
       S.template gather<U>(e, a, xj, uj);
-      bond_terms<U>(S, k, e, xj, uj, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+      bond_terms<U, TOL>(S, k, e, xj, uj, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
     }
   } else if (PIPE == 2 && W) {
     // batch k: records in (xa, ua); k+U: entries resolved, records loading
@@ -1197,12 +1228,12 @@ __device__ __forceinline__ void bond_rows(const Src& S, uint32_t W, const double
       uint32_t ec[U], ac[U];
       bond_idx<U>(R, k + 2 * U, W, pad, ec);
       S.template gather<U>(eb, ab, xb, ub);
-      bond_terms<U>(S, k, ea, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+      bond_terms<U, TOL>(S, k, ea, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
       if (k + U >= W) break;
       S.template resolve<U>(ec, ac);
       bond_idx<U>(R, k + 3 * U, W, pad, ea);
       S.template gather<U>(ec, ac, xa, ua);
-      bond_terms<U>(S, k + U, eb, xb, ub, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+      bond_terms<U, TOL>(S, k + U, eb, xb, ub, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
       S.template resolve<U>(ea, ab);   // batch k + 3U
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -1223,7 +1254,7 @@ __device__ __forceinline__ void bond_rows(const Src& S, uint32_t W, const double
       double4 xb[U], ub[U];
       bond_idx<U>(R, k + 2 * U, W, pad, e2);
       S.template gather<U>(e1, a1, xb, ub);
-      bond_terms<U>(S, k, e0, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
+      bond_terms<U, TOL>(S, k, e0, xa, ua, xi, ui, fast, can_break, cnot, snot, f0, f1, f2);
       S.template resolve<U>(e2, a1);
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -1256,7 +1287,7 @@ __device__ __forceinline__ void slice_of(const Dev& d, uint64_t t, uint64_t& bas
 
 // Bond kernel over the u32 entries, records gathered from HBM/L2.
 // OFF32: see rec.
-template <int U, int PIPE, bool OFF32>
+template <int U, int PIPE, bool OFF32, bool TOL = false>
 __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
   if (halted(d)) return;
   const uint64_t t = xcd_block() * blockDim.x + threadIdx.x;   // slot
@@ -1281,7 +1312,7 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
     // each node's data is fetched from HBM once per launch
     double4 xi, ui;
     rec<false>(d, (uint32_t)g, xi, ui);
-    bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, H.frac, H.cnot, H.snot, f0, f1, f2);
+    bond_rows<U, PIPE, TOL>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, H.frac, H.cnot, H.snot, f0, f1, f2);
   }
   finish_node(d, g, p, cin, f0, f1, f2);
   d.F[3 * g] = f0;
@@ -1295,7 +1326,7 @@ __global__ __launch_bounds__(256) void k_bond_t(Dev d) {
 // only the displacement is gathered from L2.  Used when every job of the
 // pool has block tables of at most kLdsCap nodes.  The tables hold a CU to
 // two workgroups (two waves per SIMD), so registers are allotted for two.
-template <int U, int PIPE>
+template <int U, int PIPE, bool TOL = false>
 __global__ __launch_bounds__(256, 2) void k_bond_lds(Dev d) {
   if (halted(d)) return;
   const uint64_t blk = xcd_block();
@@ -1351,7 +1382,7 @@ __global__ __launch_bounds__(256, 2) void k_bond_lds(Dev d) {
     const double4 xi = make_double4(xy.x, xy.y, zv.x, zv.y);
     double4 ui;
     rec_u<true>(d, (uint32_t)g, ui);
-    bond_rows<U, PIPE>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, H.frac, H.cnot, H.snot, f0, f1, f2);
+    bond_rows<U, PIPE, TOL>(S, W, xi, ui, (*d.flags & kFlagSafeMath) == 0, H.frac, H.cnot, H.snot, f0, f1, f2);
   }
   finish_node(d, g, p, cin, f0, f1, f2);
   d.F[3 * g] = f0;
@@ -1377,6 +1408,17 @@ __host__ inline const BondShape* bond_shapes() {
       {k_bond_t<3, 2, true>, 3, 2, 0}, {k_bond_t<4, 2, true>, 4, 2, 0}, {k_bond_t<2, 2, true>, 2, 2, 0},
       {k_bond_lds<3, 2>, 3, 2, 1},     {k_bond_lds<4, 2>, 4, 2, 1},     {k_bond_lds<2, 2>, 2, 2, 1},
       {k_bond_lds<4, 1>, 4, 1, 1},     {k_bond_lds<5, 2>, 5, 2, 1}};
+  return t;
+}
+// The same shapes in tolerance arithmetic (the default unstaged shape and the
+// staged ones that fit the two-wave register budget; null: not built).
+__host__ inline const BondShape* bond_shapes_tol() {
+  static const BondShape t[kBondShapes] = {
+      {nullptr, 8, 0, 0}, {nullptr, 4, 0, 0}, {nullptr, 6, 0, 0},
+      {nullptr, 3, 1, 0}, {k_bond_t<4, 1, true, true>, 4, 1, 0}, {nullptr, 5, 1, 0},
+      {nullptr, 3, 2, 0}, {nullptr, 4, 2, 0}, {nullptr, 2, 2, 0},
+      {k_bond_lds<3, 2, true>, 3, 2, 1}, {k_bond_lds<4, 2, true>, 4, 2, 1}, {k_bond_lds<2, 2, true>, 2, 2, 1},
+      {k_bond_lds<4, 1, true>, 4, 1, 1}, {nullptr, 5, 2, 1}};
   return t;
 }
 // The default shape with 64-bit record addressing, for pools too large for

@@ -12,7 +12,12 @@ properties that hold at any size, plus bitwise parity of a sample:
 * a sample: two of the 125 spheres, stepped inside the full pool, equal the
   oracle stepping those two spheres alone, bit for bit (they are far from
   the others and from the walls for these few steps).
+
+The crushing aggregate (2.85M nodes) is checked in dense contact by teacher
+forcing: the oracle is built from the GPU's state and both take one step.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -22,6 +27,12 @@ from peridem_amd import abi, gen
 pytestmark = pytest.mark.gpu
 
 NSPHERES = 125
+# checkpoints of the crushing aggregate's teacher-forced steps: the top wall
+# has driven the top layers into contact with each other (A) and the packing
+# is in contact through most of its height (B); at least this many particle
+# pairs pass the box test in the checked step
+DENSE_STEPS_A, DENSE_STEPS_B = 20000, 60000
+MIN_DENSE_PAIRS = 50
 
 
 def full_pool(**kw):
@@ -139,3 +150,69 @@ def test_crushing_aggregate_full_size_sample_matches_oracle(oracle_lib):
     finally:
         g.close()
         o.close()
+
+
+def state_as_system(s, g, first_step):
+    """The system as the GPU holds it after some steps -- every particle's
+    disp, vel, acc and intact bonds (Connectivity as NbdArr2conn emits it,
+    timeloop.h:20-44) and the wall planes: what a tc_/wall_ checkpoint holds
+    (timeloop.h:262-305), with the step counter continued."""
+    parts = []
+    for p, P in enumerate(s.particles):
+        st = g.state(p)
+        Q = gen.Particle(**P.__dict__)
+        Q.disp, Q.vel, Q.acc = st["disp"], st["vel"], st["acc"]
+        Q.conn = g.connectivity(p)
+        parts.append(Q)
+    lrtb, _ = g.wall()
+    return gen.System(**{**s.__dict__, "particles": parts, "wall_lrtb": tuple(lrtb),
+                         "first_step": first_step})
+
+
+@pytest.mark.parametrize("checkpoints", [(DENSE_STEPS_A, DENSE_STEPS_B)])
+def test_crushing_aggregate_dense_contact_teacher_forced(oracle_lib, checkpoints):
+    """The full crushing aggregate (512 particles, 2.85M nodes, 256M directed
+    bonds) stepped on the GPU while the top wall drives the packing into
+    contact; at each checkpoint the oracle is built from the GPU's state and
+    both take one step, which must agree bit for bit on every node, bond,
+    wall plane, wall reaction and on the number of particle pairs in contact
+    (contact.h:746-942, 1060-1124, timeloop.h:636-651).  The oracle prunes
+    its contact candidates with a cell grid, which leaves its sums bitwise
+    those of its all-pairs loops (test_pruned_contact_is_bitwise_the_all_pairs_loop)."""
+    s = gen.crushing_aggregate(512)
+    g = abi.gpu_engine(s)
+    done = 0
+    try:
+        for k in checkpoints:
+            g.step(k - done)
+            done = k
+            t0 = time.perf_counter()
+            st = state_as_system(s, g, first_step=done + 1)
+            o = oracle_lib.engine(st, prune=True)
+            pw0 = g.counters()["pair_computations"]
+            print(f"[fullsize] teacher-forced step {done + 1}: oracle built in "
+                  f"{time.perf_counter() - t0:.1f} s", flush=True)
+            try:
+                g.step(1)
+                o.step(1)
+                done += 1
+                pairs = g.counters()["pair_computations"] - pw0
+                assert pairs == o.counters()["pair_computations"], (pairs, o.counters())
+                for p in range(len(s.particles)):
+                    sg, so = g.state(p), o.state(p)
+                    for f in FIELDS:
+                        assert_bitwise(sg[f], so[f], f"step {done} particle {p} {f}")
+                    assert np.array_equal(g.connectivity(p), o.connectivity(p)), f"step {done} particle {p} bonds"
+                lg, rg = g.wall()
+                lo, ro = o.wall()
+                assert_bitwise(lg, lo, f"step {done} wall planes")
+                assert_bitwise(rg, ro, f"step {done} wall reaction")
+                assert np.any(rg[5] != 0), "the top wall touches the packing"
+                print(f"[fullsize] step {done}: {pairs} particle pairs in contact, "
+                      f"{s.bonds - g.counters()['alive_bonds']} directed bonds broken, bitwise equal",
+                      flush=True)
+                assert pairs >= MIN_DENSE_PAIRS
+            finally:
+                o.close()
+    finally:
+        g.close()

@@ -603,3 +603,41 @@ def test_self_contact_beyond_contact_radius_uses_the_contact_spring(oracle_lib):
     _, st = force_after_one_step(oracle_lib, system([p], self_contact=1, enable_fracture=1,
                                                     normal_stiffness=Kn))
     assert np.all(st[0]["force"] == 0)
+
+
+# ---------------------------------------------------------------- candidate pruning
+@pytest.mark.parametrize("case", ["impact", "mixed", "bdry"])
+def test_pruned_contact_is_bitwise_the_all_pairs_loop(oracle_lib, case):
+    """or_set_prune visits only cell-grid neighbours, in the loop's own
+    order: the run is bit for bit the all-pairs one, with self, pair and wall
+    contact, fracture, and boundary-list order (nl_bdry_only)."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from helpers import mixed_system, small_impact
+    if case == "impact":
+        s = small_impact(seed=4)
+    elif case == "mixed":
+        s = mixed_system(seed=3)
+    else:
+        s = _impact_with_walls()
+        s.nl_bdry_only = 1
+        rng = np.random.default_rng(8)
+        for p in s.particles:
+            p.bdry = rng.permutation(p.n).astype(np.uint32)[: (3 * p.n) // 4]
+    a, b = oracle_lib.engine(s), oracle_lib.engine(s, prune=True)
+    try:
+        for n in ((40, 60) if case != "bdry" else (120, 120)):
+            a.step(n)
+            b.step(n)
+            for p in range(len(s.particles)):
+                sa, sb = a.state(p), b.state(p)
+                for k in ("disp", "vel", "acc", "force"):
+                    assert np.array_equal(sa[k].view(np.uint64), sb[k].view(np.uint64)), (case, p, k)
+                assert np.array_equal(a.connectivity(p), b.connectivity(p))
+            assert np.array_equal(a.wall()[1], b.wall()[1])
+            assert a.counters()["pair_computations"] == b.counters()["pair_computations"]
+        assert a.counters()["pair_computations"] > 0
+    finally:
+        a.close()
+        b.close()
