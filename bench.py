@@ -120,13 +120,31 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(s, nspheres, steps, threads, steps_1core):
+def gpu_state_system(s, e, first_step):
+    """The workload as the GPU holds it now: every particle's disp, vel, acc
+    and intact bonds and the wall planes (what a tc_/wall_ checkpoint holds)."""
+    from peridem_amd import gen
+    parts = []
+    for p, P in enumerate(s.particles):
+        st = e.state(p)
+        Q = gen.Particle(**P.__dict__)
+        Q.disp, Q.vel, Q.acc = st["disp"], st["vel"], st["acc"]
+        Q.conn = e.connectivity(p)
+        parts.append(Q)
+    lrtb, _ = e.wall()
+    return gen.System(**{**s.__dict__, "particles": parts, "wall_lrtb": tuple(lrtb), "first_step": first_step})
+
+
+def cpu_baseline(s, nspheres, steps, threads, steps_1core, mid=None, mid_step=0):
     """The oracle on the first `nspheres` spheres of the workload `s`, on
-    `threads` host threads and (a shorter window) on one."""
+    `threads` host threads and (a shorter window) on one; with `mid`, the same
+    window again from that GPU-produced mid-run state."""
     from oracle import oracle
     from peridem_amd import gen
     if nspheres < len(s.particles):
         s = gen.System(**{**s.__dict__, "particles": s.particles[:nspheres]})
+        if mid is not None:
+            mid = gen.System(**{**mid.__dict__, "particles": mid.particles[:nspheres]})
 
     def timed(ranks, n):
         o = oracle.engine(s, ranks=ranks)
@@ -147,6 +165,11 @@ def cpu_baseline(s, nspheres, steps, threads, steps_1core):
         v1, dt1 = timed(1, steps_1core)
         out["single_core"] = {"value": v1, "sample": f"{steps_1core} steps of the same spheres on one "
                                                      f"thread, {dt1:.1f} s"}
+    if mid is not None:
+        s = mid
+        vm, dtm = timed(threads, steps)
+        out["mid_run"] = {"value": vm, "sample": f"{steps} steps (after 2 untimed) from the GPU's state after "
+                                                 f"{mid_step} steps, {threads} threads, {dtm:.1f} s"}
     return out
 
 
@@ -161,6 +184,11 @@ def batched(args):
     wall_b, dev_ms, dig_b, adm = serve.run_batch(systems, args.batch_steps, digests=True)
     log(f"[bench] batched: {wall_b:.2f} s wall ({adm:.2f} s admission), "
         f"{dev_ms / args.batch_steps:.3f} ms device per step")
+    if args.no_sequential:   # profiling the batch alone
+        return {"workload": "batched impacts (no sequential comparison)", "jobs": len(systems),
+                "steps_per_job": args.batch_steps, "nodes": nodes, "directed_bonds": bonds,
+                "batch_wall_s": wall_b, "batch_ms_per_step": dev_ms / args.batch_steps,
+                "batch_bond_updates_per_s": bonds * args.batch_steps / (dev_ms * 1e-3)}
     t_seq, dig_s = serve.run_sequential(systems, args.batch_steps, digests=True)
     wall_s = sum(t_seq)
     same = sum(a == b for a, b in zip(dig_b, dig_s))
@@ -244,8 +272,9 @@ def crush(args):
     (set_zero_wall_speed_timestep, timeloop.h:447-462) and the packing left to
     relax.  Every chunk of steps records the top wall's travel, the reaction
     on the top and bottom walls (contact.h:255-272: -sum V f over the nodes in
-    contact), the fraction of directed bonds broken, the particle pairs that
-    passed the box test per step and the device time per step."""
+    contact; the reaction of one step, averaged over samples taken every
+    `crush_sample` steps of the chunk), the fraction of directed bonds broken
+    and the particle pairs that passed the box test per step."""
     from peridem_amd import abi, gen
     s = gen.crushing_aggregate(512)
     H0 = s.wall_lrtb[5] - s.wall_lrtb[2]
@@ -263,29 +292,40 @@ def crush(args):
             if t == n30:
                 e.set_wall_speed([0.0] * 6)
             n = min(args.crush_chunk, (n30 if t < n30 else end) - t)
-            e.step(n)
+            # the reaction of single steps, sampled every `sample` steps of the chunk
+            top, bottom, done = [], [], 0
+            while done < n:
+                m = min(args.crush_sample, n - done)
+                e.step(m)
+                dev_ms += e.last_step_ms()
+                done += m
+                _, reac = e.wall()
+                top.append(float(reac[5][2]))
+                bottom.append(float(reac[2][2]))
             t += n
-            dev_ms += e.last_step_ms()
-            lrtb, reac = e.wall()
+            lrtb, _ = e.wall()
             c = e.counters()
             row = {"step": t, "wall_travel_m": s.wall_lrtb[5] - lrtb[5],
                    "strain": (s.wall_lrtb[5] - lrtb[5]) / H0,
-                   "reaction_top_z_N": float(reac[5][2]), "reaction_bottom_z_N": float(reac[2][2]),
+                   "reaction_top_z_N": sum(top) / len(top), "reaction_top_z_max_N": max(top, key=abs),
+                   "reaction_bottom_z_N": sum(bottom) / len(bottom),
                    "broken_fraction": 1.0 - c["alive_bonds"] / c["total_bonds"],
                    "pairs_per_step": (c["pair_computations"] - c0["pair_computations"]) / n,
-                   "ms_per_step": e.last_step_ms() / n, "list_capacity": c["list_capacity"]}
+                   "ms_per_step": dev_ms / max(t, 1), "list_capacity": c["list_capacity"]}
             c0 = c
             rows.append(row)
-            log(f"[crush] step {t}: strain {row['strain']:.4f}, top reaction {row['reaction_top_z_N']:.4e} N, "
-                f"broken {row['broken_fraction']:.3e}, pairs/step {row['pairs_per_step']:.0f}, "
-                f"{row['ms_per_step']:.3f} ms/step, list capacity {row['list_capacity']}")
+            log(f"[crush] step {t}: strain {row['strain']:.4f}, top reaction mean {row['reaction_top_z_N']:.4e} N "
+                f"(largest {row['reaction_top_z_max_N']:.4e}), bottom {row['reaction_bottom_z_N']:.4e} N, broken "
+                f"{row['broken_fraction']:.3e}, pairs/step {row['pairs_per_step']:.0f}, list capacity "
+                f"{row['list_capacity']}")
     finally:
         cnt = e.counters()
         e.close()
     wall_s = time.perf_counter() - t_start
     out = {"mode": "crush", "workload": "crushing aggregate, 512 cubes/spheres/convex polyhedra of 2k-12k "
-                                        "nodes, h=6e-5 m, delta=3h, self-contact, top wall -5 m/s to 30% "
-                                        "strain then at rest, FP64, dt=1e-8",
+                                        "nodes placed at random (bounding spheres 35% of the box), h=1e-4 m, "
+                                        "delta=3h, self-contact, contact stiffness of 2e9 Pa, top wall -5 m/s to "
+                                        "30% strain then at rest, FP64, dt=1e-8",
            "nodes": s.nodes, "directed_bonds": s.bonds, "box_height_m": H0, "steps_to_30pct": n30,
            "relax_steps": args.crush_relax, "steps": t, "wall_s": wall_s,
            "device_s": dev_ms * 1e-3, "bond_updates_per_s": s.movable_bonds() * t / (dev_ms * 1e-3),
@@ -293,12 +333,14 @@ def crush(args):
     print(json.dumps(out), flush=True)
     if args.crush_md:
         with open(args.crush_md, "w") as f:
-            f.write("| step | top wall travel (m) | strain | top wall reaction z (N) | bottom wall reaction z (N) "
-                    "| broken bonds | pairs/step | ms/step |\n|---|---|---|---|---|---|---|---|\n")
+            f.write("| step | top wall travel (m) | strain | top wall reaction z, mean (N) | largest (N) "
+                    "| bottom wall reaction z, mean (N) | broken bonds | pairs/step |\n"
+                    "|---|---|---|---|---|---|---|---|\n")
             for r in rows:
                 f.write(f"| {r['step']} | {r['wall_travel_m']:.4e} | {r['strain']:.4f} | "
-                        f"{r['reaction_top_z_N']:.4e} | {r['reaction_bottom_z_N']:.4e} | "
-                        f"{r['broken_fraction']:.4e} | {r['pairs_per_step']:.0f} | {r['ms_per_step']:.3f} |\n")
+                        f"{r['reaction_top_z_N']:.4e} | {r['reaction_top_z_max_N']:.4e} | "
+                        f"{r['reaction_bottom_z_N']:.4e} | {r['broken_fraction']:.4e} | "
+                        f"{r['pairs_per_step']:.0f} |\n")
 
 
 def main():
@@ -318,6 +360,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batched leg without the one-by-one comparison runs (for profiling)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tolerance", action="store_true",
                     help="skip the tolerance-arithmetic line")
@@ -333,10 +377,14 @@ def main():
     ap.add_argument("--bond-shape", type=int, default=-1,
                     help="force this bond-kernel shape instead of the autotuner's choice "
                          "(profiling passes pin one so that every pass times the same kernel)")
+    ap.add_argument("--no-headline", action="store_true",
+                    help="run only the batched and served legs (their JSON, not the benchmark line)")
     ap.add_argument("--crush", action="store_true",
                     help="instead of the benchmark: the crushing aggregate at full size to 30%% "
                          "strain and a relaxation, printing its force-displacement trace")
-    ap.add_argument("--crush-chunk", type=int, default=1000)
+    ap.add_argument("--crush-chunk", type=int, default=2000)
+    ap.add_argument("--crush-sample", type=int, default=100,
+                    help="steps between the wall reactions averaged over a chunk")
     ap.add_argument("--crush-relax", type=int, default=10000)
     ap.add_argument("--crush-md", default="", help="also write the trace as a markdown table here")
     args = ap.parse_args()
@@ -348,6 +396,12 @@ def main():
 
     if args.crush:
         crush(args)
+        return
+    if args.no_headline:   # the batched and served legs alone (profiling, long serving sweeps)
+        out = {"mode": "batched/served legs only",
+               "batched": None if args.no_batch else batched(args),
+               "serving": None if args.no_serve else serving(args)}
+        print(json.dumps(out), flush=True)
         return
 
     probe = None
@@ -404,6 +458,9 @@ def main():
     cnt = e.counters()
     kname = shape_name(e.bond_shape(), N)
     fi = fp64_instr(kname)
+    # a GPU-produced mid-run state for the CPU baseline's second window
+    mid = None if args.no_cpu else gpu_state_system(s, e, cnt["steps"] + 1)
+    mid_step = cnt["steps"]
     tol = None
     if not args.no_tolerance:
         # the same pool in tolerance arithmetic (its shapes tuned again at the
@@ -452,7 +509,8 @@ def main():
     if not args.no_cpu:
         log(f"[bench] CPU baseline: {args.cpu_spheres} spheres x {args.cpu_steps} steps, "
             f"{args.cpu_threads} threads")
-        cpu = cpu_baseline(s, args.cpu_spheres, args.cpu_steps, args.cpu_threads, args.cpu_steps_1core)
+        cpu = cpu_baseline(s, args.cpu_spheres, args.cpu_steps, args.cpu_threads, args.cpu_steps_1core,
+                           mid=mid, mid_step=mid_step)
 
     out = {
         "metric": "bond-updates/s & % HBM peak; batched sims/hr + p99 latency vs plain GPU",

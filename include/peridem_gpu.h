@@ -225,6 +225,36 @@ int pd_job_get_state(pd_ctx* ctx, uint32_t job, uint32_t p, double* disp, double
 int pd_job_get_connectivity(pd_ctx* ctx, uint32_t job, uint32_t p, uint32_t* pairs,
                             uint64_t cap, uint64_t* m);
 int pd_job_set_flag(pd_ctx* ctx, uint32_t job, int flag, int value);
+/* Served mode: a request loop over a reserved pool, for C and C++ hosts.
+ * No counterpart in the reference (one system per process,
+ * run/simulate3d.cpp:142).  pd_serve_submit copies a request (run
+ * parameters, particles, step budget) and lays it out on a worker thread;
+ * each pd_serve_poll call admits waiting requests in arrival order while
+ * their footprint fits the pool (pd_add_prepared), advances every resident
+ * job by one chunk (at most `chunk` steps, never past the earliest budget end,
+ * so finished jobs leave and waiting ones enter at call boundaries) and
+ * returns the requests that finished.  A finished request's results stay in
+ * its pool slot (pd_job_get_state, pd_job_get_connectivity, pd_get_wall with
+ * that job) until pd_serve_release.  status is PD_OK, or why the request
+ * produced no results: PD_E_NOMEM (larger than the empty pool), PD_E_OVERFLOW
+ * (stopped by the library, see pd_step), a layout error.  Times are seconds
+ * since pd_serve_create.  A server is used from one host thread; the pool
+ * must not be stepped by anything else while it serves. */
+typedef struct pd_server pd_server;
+typedef struct pd_serve_event {
+  uint64_t request;   /* id from pd_serve_submit */
+  uint32_t job;       /* pool job slot holding the results (UINT32_MAX: none) */
+  int32_t status;
+  double t_submit, t_admit, t_done;
+} pd_serve_event;
+int pd_serve_create(pd_ctx* pool, uint32_t chunk, uint32_t prepare_threads, pd_server** out);
+int pd_serve_submit(pd_server* server, const pd_run_desc* run, const pd_particle_desc* parts,
+                    uint32_t nparts, uint64_t nsteps, uint64_t* request);
+int pd_serve_poll(pd_server* server, pd_serve_event* events, uint32_t cap, uint32_t* n);
+int pd_serve_pending(pd_server* server, uint64_t* queued, uint64_t* resident, uint64_t* unreported);
+int pd_serve_release(pd_server* server, uint32_t job);
+void pd_serve_destroy(pd_server* server);
+
 /* Runs of 8 steps replay one captured HIP graph (default on); off: every
  * step is launched kernel by kernel.  Results are the same either way. */
 int pd_set_graphs(pd_ctx* ctx, int on);

@@ -14,7 +14,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-GPU_LIB = os.path.join(HERE, "libperidem_gpu.so")
+# PERIDEM_GPU_LIB points the binding at another build of the same library
+# (the sanitizer builds of peridem_amd/sanitize.py)
+GPU_LIB = os.environ.get("PERIDEM_GPU_LIB") or os.path.join(HERE, "libperidem_gpu.so")
 
 PD_OK = 0
 ERRORS = {-1: "PD_E_ARG", -2: "PD_E_STATE", -3: "PD_E_HIP", -4: "PD_E_UNSUPPORTED",
@@ -84,6 +86,11 @@ class Profile(C.Structure):
     def as_dict(self):
         return {n: {"ms": self.ms[i], "launches": int(self.launches[i])}
                 for i, n in enumerate(self.PHASES)}
+
+
+class ServeEvent(C.Structure):
+    _fields_ = [("request", _u64), ("job", _u32), ("status", C.c_int32), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
 
 
 class PDError(RuntimeError):
@@ -193,7 +200,14 @@ def load_gpu_lib(path=GPU_LIB):
                 ("pd_job_blob_footprint", C.c_int, [P, C.POINTER(_u64), C.POINTER(_u64),
                                                     C.POINTER(_u64)]),
                 ("pd_add_prepared", C.c_int, [P, P, _u64, _pu]),
-                ("pd_job_blob_free", None, [P])):
+                ("pd_job_blob_free", None, [P]),
+                ("pd_serve_create", C.c_int, [P, _u32, _u32, C.POINTER(P)]),
+                ("pd_serve_submit", C.c_int, [P, C.POINTER(RunDesc), C.POINTER(ParticleDesc), _u32, _u64,
+                                              C.POINTER(_u64)]),
+                ("pd_serve_poll", C.c_int, [P, C.POINTER(ServeEvent), _u32, C.POINTER(_u32)]),
+                ("pd_serve_pending", C.c_int, [P, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
+                ("pd_serve_release", C.c_int, [P, _u32]),
+                ("pd_serve_destroy", None, [P])):
             f = getattr(lib, name)
             f.restype, f.argtypes = res, args
         _LIBS[path] = (lib, _bind(lib, "pd_"))
@@ -595,3 +609,53 @@ def selftest_math_noalloc(n, seed=1):
 def gpu_engine(system):
     lib, fns = load_gpu_lib()
     return Engine(lib, fns, system, "pd_")
+
+
+class CServer:
+    """The library's own request loop (pd_serve_*) over a Pool: submit
+    requests, poll until they finish, read a finished one's results from its
+    pool slot (Pool.state / connectivity / wall), release it."""
+
+    def __init__(self, pool, chunk=256, prepare_threads=2):
+        self.pool = pool
+        self.lib = pool.lib
+        self.ptr = C.c_void_p()
+        pool._check(self.lib.pd_serve_create(pool.ctx, chunk, prepare_threads, C.byref(self.ptr)), "serve_create")
+        self.systems = {}
+
+    def submit(self, system, nsteps):
+        keep = []
+        arr = _descs(system, keep)
+        rd = system.run_desc()
+        rid = _u64()
+        self.pool._check(self.lib.pd_serve_submit(self.ptr, C.byref(rd), arr, len(system.particles), nsteps,
+                                                  C.byref(rid)), "serve_submit")
+        self.systems[rid.value] = system
+        return rid.value
+
+    def poll(self, cap=64):
+        ev = (ServeEvent * cap)()
+        n = _u32()
+        self.pool._check(self.lib.pd_serve_poll(self.ptr, ev, cap, C.byref(n)), "serve_poll")
+        out = []
+        for k in range(n.value):
+            e = ev[k]
+            d = {f: getattr(e, f) for f, _ in ServeEvent._fields_}
+            if e.job != 0xFFFFFFFF:
+                self.pool.systems[e.job] = self.systems[e.request]
+            out.append(d)
+        return out
+
+    def pending(self):
+        q, r, u = _u64(), _u64(), _u64()
+        self.pool._check(self.lib.pd_serve_pending(self.ptr, C.byref(q), C.byref(r), C.byref(u)), "serve_pending")
+        return q.value, r.value, u.value
+
+    def release(self, job):
+        self.pool._check(self.lib.pd_serve_release(self.ptr, job), "serve_release")
+        self.pool.systems.pop(job, None)
+
+    def close(self):
+        if self.ptr:
+            self.lib.pd_serve_destroy(self.ptr)
+            self.ptr = C.c_void_p()

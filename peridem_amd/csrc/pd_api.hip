@@ -605,6 +605,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(ccnt, N);
   ALLOC(UBXY, N);
   ALLOC(UBZ, N);
+  ALLOC(cpos, N);
+  ALLOC(cps, N);
   ALLOC(ckey, N);
   ALLOC(cstart, (size_t)T + 1);
   ALLOC(ccur, T);
@@ -2197,3 +2199,6 @@ void pd_destroy(pd_ctx* c) {
 }
 
 }  // extern "C"
+
+// served mode's request loop (host code over the pool entry points above)
+#include "pd_serve.hip"

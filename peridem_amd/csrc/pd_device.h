@@ -161,6 +161,8 @@ struct Dev {
   double2* UBXY;     // [N] displacement when the node's list was built: x, y
   double* UBZ;       //     and z
   // contact cell list
+  double4* cpos;     // [N] CurrPos of the nodes of rebuilding jobs, this step (k_cellkey)
+  double4* cps;      // [N] the same in cell-list order, beside csorted (k_scatter)
   uint32_t* ckey;    // [N]
   uint32_t* cstart;  // [T+1]
   uint32_t* ccur;    // [T]

@@ -330,12 +330,38 @@ __global__ __launch_bounds__(64) void k_centroid(Dev d) {
 // ---------------------------------------------------------------------------
 // Pair box test on the lagged boxes, pair list in the reference's pair order
 // restricted to one instance (timeloop.h:243-255, 636-653).
+//
+// In a step where the pair's job rebuilds its candidate lists, bit 3 marks
+// both particles when their boxes of this step (min/max of the CurrPos the
+// position kernel has just accumulated) come within rb (1 + 1e-6) of each
+// other: a node of a particle with no self-contact and no such pair has no
+// candidate within rb, so its list is empty and its cell walk is skipped.
+// Until the next rebuild no node moves half a skin, so such a particle
+// cannot come within R_c of another one before its lists are built again.
+// Non-movable particles have no box of this step: their pairs count as near.
+__device__ __forceinline__ void list_test(Dev& d, uint32_t i, uint32_t j, const DevInst& I) {
+  bool near = !(d.part[i].movable && d.part[j].movable);
+  if (!near) {
+    double A[6], B[6];
+    for (int k = 0; k < 6; k++) {
+      A[k] = okey_inv(d.lrtb_next[6 * i + k]);
+      B[k] = okey_inv(d.lrtb_next[6 * j + k]);
+    }
+    near = intersects_box_fext(A, B, I.rb * (1.0 + 1e-6));
+  }
+  if (near) {
+    atomicOr(&d.pactive[i], 8u);
+    atomicOr(&d.pactive[j], 8u);
+  }
+}
+
 __device__ __forceinline__ void pair_test(Dev& d, uint64_t q) {
   uint32_t i = d.pairs[2 * q], j = d.pairs[2 * q + 1];
   const DevPart& Pi = d.part[i];
   const DevPart& Pj = d.part[j];
   const DevInst& I = d.inst[Pi.inst];
   if (!live(I) || !(Pi.movable || Pj.movable)) return;
+  if (d.lists && rebuilds(d, Pi.inst)) list_test(d, i, j, I);
   if (!intersects_box_fext(d.lrtb + 6 * i, d.lrtb + 6 * j, I.Rc)) return;
   atomicAdd(&d.counters[3], 1ull);   // moved into the total by k_finalize
   // bit 1: the particle's nodes are contact candidates this step;
@@ -387,6 +413,7 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
     cell_of(cp, I.cell, c);
     key = cell_hash(c[0], c[1], c[2], P.inst, d.T);
     atomicAdd(&d.cstart[key], 1u);
+    d.cpos[g] = make_double4(cp[0], cp[1], cp[2], 0.0);
   }
   d.ckey[g] = key;
 }
@@ -443,6 +470,7 @@ __global__ __launch_bounds__(256) void k_scatter(Dev d) {
   if (key == kNoKey) return;
   uint32_t slot = atomicAdd(&d.ccur[key], 1u);
   d.csorted[slot] = (uint32_t)g;
+  d.cps[slot] = d.cpos[g];
 }
 
 // Candidate list of node g: the nodes j != g of its job within rb, same
@@ -460,15 +488,21 @@ __global__ __launch_bounds__(256) void k_scatter(Dev d) {
 // and an insertion sort in private memory would be a long chain of dependent
 // memory round trips.  With nl_bdry_only the order is the boundary lists'
 // (k_build_bdry).
+//
+// A candidate's distance is taken between the CurrPos of this step that
+// k_cellkey stored (cpos, and cps in cell-list order, read sequentially with
+// the bucket's entries): it differs from the reference's (u_j - u_i) +
+// (X_j - X_i) only by rounding, far inside the margins of rb.  A node whose
+// particle has no self-contact and no other particle near it (bit 3 of
+// pactive, pair_test) gets an empty list without walking the cells.
 template <bool BDRY>
-__device__ __forceinline__ bool build_head(const Dev& d, uint64_t g, uint32_t& p, double4& xi, double4& ui,
-                                           bool& selfc, int64_t* c) {
+__device__ __forceinline__ bool build_head(const Dev& d, uint64_t g, uint32_t& p, double4& ci, bool& selfc,
+                                           int64_t* c) {
   p = d.pid[g];
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
   if (!rebuilds_node(d, P.inst) || (I.nl_bdry_only != 0) != BDRY) return false;
-  xi = d.X[g];
-  ui = disp_of(d, g);
+  const double4 ui = disp_of(d, g);
   d.UBXY[g] = make_double2(ui.x, ui.y);
   d.UBZ[g] = ui.z;
   if (BDRY && d.brank[g] == kNoKey) {   // not a boundary node: no contact
@@ -476,7 +510,12 @@ __device__ __forceinline__ bool build_head(const Dev& d, uint64_t g, uint32_t& p
     return false;
   }
   selfc = self_active(P, I);
-  const double cpi[3] = {xi.x + ui.x, xi.y + ui.y, xi.z + ui.z};
+  if (!selfc && !(d.pactive[p] & 8u)) {   // nothing within rb
+    d.ccnt[g] = 0;
+    return false;
+  }
+  ci = d.cpos[g];
+  const double cpi[3] = {ci.x, ci.y, ci.z};
   cell_of(cpi, I.cell, c);
   return true;
 }
@@ -495,10 +534,10 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
   uint32_t p;
-  double4 xi, ui;
+  double4 ci;
   bool selfc;
   int64_t c[3];
-  if (!build_head<false>(d, g, p, xi, ui, selfc, c)) return;
+  if (!build_head<false>(d, g, p, ci, selfc, c)) return;
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
   constexpr uint32_t kEmpty = ~0u;
@@ -525,9 +564,14 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
         nb1 = d.cstart[key + 1];
       }
       uint32_t nj = b0 < b1 ? d.csorted[b0] : 0u;
+      double4 ncj = b0 < b1 ? d.cps[b0] : ci;
       for (uint32_t it = b0; it < b1; it++) {
         const uint32_t j = nj;
-        if (it + 1 < b1) nj = d.csorted[it + 1];
+        const double4 cj = ncj;
+        if (it + 1 < b1) {
+          nj = d.csorted[it + 1];
+          ncj = d.cps[it + 1];
+        }
         if (j == g) continue;
         // jobs and particles hold contiguous node ranges: membership is a
         // range test (hash buckets are shared by every job)
@@ -536,10 +580,7 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
         if (same && !selfc) continue;
         uint32_t x = (same ? 0u : kBroken) | j;
         if (!first && x <= lb) continue;   // placed by an earlier pass
-        const double4 xj = d.X[j];
-        const double4 uj = disp_of(d, j);
-        const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
-                               (uj.z - ui.z) + (xj.z - xi.z));
+        const double dn = nrm3(cj.x - ci.x, cj.y - ci.y, cj.z - ci.z);
         if (!(dn <= I.rb)) continue;
 #pragma unroll
         for (int i0 = 0; i0 < kMaxCand; i0 += 8) {
@@ -580,10 +621,10 @@ __global__ __launch_bounds__(256) void k_build_bdry(Dev d) {
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
   uint32_t p;
-  double4 xi, ui;
+  double4 ci;
   bool selfc;
   int64_t c[3];
-  if (!build_head<true>(d, g, p, xi, ui, selfc, c)) return;
+  if (!build_head<true>(d, g, p, ci, selfc, c)) return;
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
   uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * d.ccap) + (g & 63);
@@ -613,14 +654,12 @@ __global__ __launch_bounds__(256) void k_build_bdry(Dev d) {
         if (j < I.node0 || j >= I.node1) continue;
         const bool same = j >= P.node_off && j < P.node_off + P.nnodes;
         if (same && !selfc) continue;
+        const double4 cj = d.cps[it];
+        const double dn = nrm3(cj.x - ci.x, cj.y - ci.y, cj.z - ci.z);
+        if (!(dn <= I.rb)) continue;
         const uint64_t key2 =
             ((uint64_t)!same << 63) | ((uint64_t)d.part[d.pid[j]].node_off << 32) | (uint64_t)d.brank[j];
         if (!first && key2 <= lbkey) continue;   // placed by an earlier pass
-        const double4 xj = d.X[j];
-        const double4 uj = disp_of(d, j);
-        const double dn = nrm3((uj.x - ui.x) + (xj.x - xi.x), (uj.y - ui.y) + (xj.y - xi.y),
-                               (uj.z - ui.z) + (xj.z - xi.z));
-        if (!(dn <= I.rb)) continue;
         if (ncand == kMaxCand) {   // keep the kMaxCand smallest keys
           over = true;
           if (key2 > ckey[kMaxCand - 1]) continue;

@@ -13,7 +13,8 @@ Material constants follow the 3-D PMB closed forms of material_dict.py:266-277
 (peridem_3d): c0 = 18K/(pi delta^4), s0 = sqrt(5 Gc / (9 K delta)), with the
 bulk modulus K taken from the particle's own E and nu.  The contact normal
 stiffness is K_n = 18K/(pi R_c^5), i.e. c0 evaluated at delta = R_c divided by
-R_c, the convention stated at contact.h:775-776.
+R_c (contact.h:775-776), with K = 2e9 Pa as the reference's aggregate example
+takes it (see contact_stiffness).
 """
 from __future__ import annotations
 
@@ -39,7 +40,26 @@ def pmb3d(E, nu, Gc, delta):
 
 
 def normal_stiffness(E, nu, Rc):
+    """18 K / (pi R_c^5) with the particle's own bulk modulus (c0 at delta = R_c,
+    over R_c, contact.h:775-776)."""
     return 18.0 * bulk_modulus(E, nu) / (math.pi * Rc ** 5)
+
+
+# The reference's aggregate example takes the contact stiffness from the
+# peridem_3d material whatever the particles are made of:
+#   normal_stiffness = material_dict.peridem_3d(contact_radius).cnot / contact_radius
+# (examples/3D-HPC-aggregate-particle-breakage/setup.py:160; peridem_3d has a
+# bulk modulus of 2e9 Pa, material_dict.py:266-277).  The workloads below use
+# that convention: with the particles' own 48e9 Pa the spring is 24x stiffer,
+# and two spheres meeting at 10 m/s blow up within a few hundred steps of
+# dt = 1e-8 s (in the reference algorithm as in this library), while with
+# 2e9 Pa they bounce (tests/test_generator.py).
+CONTACT_BULK_MODULUS = 2e9
+
+
+def contact_stiffness(Rc, K=CONTACT_BULK_MODULUS):
+    """peridem_3d(R_c).cnot / R_c = 18 K / (pi R_c^5), K = 2e9 Pa."""
+    return 18.0 * K / (math.pi * Rc ** 5)
 
 
 # ---------------------------------------------------------------------------
@@ -247,7 +267,7 @@ def two_sphere_impact(R=1e-3, h=1e-4, v=20.0, seed=1, E=72e9, nu=0.25, rho=2440.
     A.vel = np.zeros((A.n, 3))
     A.vel[:, 0] = v
     Rc = 0.9 * h
-    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(E, nu, Rc),
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=contact_stiffness(Rc),
               friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=0,
               name="two_sphere_impact")
     kw.update(sys_kw)
@@ -297,7 +317,7 @@ def gravity_settling(nspheres=125, R=1e-3, nodes_per_sphere=8000, box=1.2e-2, se
             p.vel = (vrng.uniform(-sphere_speed, sphere_speed, size=(1, 3)) +
                      vrng.uniform(-node_speed, node_speed, size=(p.n, 3)))
         parts.append(p)
-    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(E, nu, Rc),
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=contact_stiffness(Rc),
               friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=0,
               allow_wall=1, wall_lrtb=(0, 0, 0, box, box, box), name="gravity_settling")
     kw.update(sys_kw)
@@ -306,9 +326,49 @@ def gravity_settling(nspheres=125, R=1e-3, nodes_per_sphere=8000, box=1.2e-2, se
     return s
 
 
+def _rsa_centres(radii, frac, rng, margin, gap):
+    """Random sequential placement, largest first, of spheres of the given
+    radii in a cube whose side puts their volume at `frac` of it: each centre
+    is drawn uniformly at least radius + margin from the walls and kept if it
+    is at least r_i + r_j + gap from every placed one (the first free one of
+    batches of draws); the cube grows by 2% and placement restarts when a
+    sphere finds no room in 20000 draws."""
+    radii = np.asarray(radii, dtype=float)
+    order = np.argsort(-radii, kind="stable")
+    side = (4.0 / 3.0 * math.pi * np.sum(radii ** 3) / frac) ** (1.0 / 3.0)
+    while True:
+        placed = np.zeros((0, 3))
+        prad = np.zeros(0)
+        ok = True
+        for k in order:
+            r = radii[k]
+            lo, hi = r + margin, side - r - margin
+            for _ in range(40):   # 40 batches of 500 draws, the first free one kept
+                cand = rng.uniform(lo, hi, size=(500, 3))
+                if placed.shape[0]:
+                    dist = np.linalg.norm(cand[:, None, :] - placed[None, :, :], axis=2)
+                    free = np.all(dist >= prad[None, :] + r + gap, axis=1)
+                else:
+                    free = np.ones(len(cand), dtype=bool)
+                if free.any():
+                    placed = np.vstack([placed, cand[np.argmax(free)]])
+                    prad = np.append(prad, r)
+                    break
+            else:
+                ok = False
+                break
+        if ok:
+            out = np.zeros((len(radii), 3))
+            out[order] = placed
+            return out, side
+        side *= 1.02
+
+
 def crushing_aggregate(nparticles=512, nodes_range=(2000, 12000), seed=3, E=72e9, nu=0.25,
-                       rho=2440.0, Gc=135.0, h=1e-4, top_speed=-5.0, **sys_kw):
-    """Mixed cubes, spheres and convex polyhedra packed on a grid, top wall moving down.
+                       rho=2440.0, Gc=135.0, h=1e-4, top_speed=-5.0, packing=0.35, **sys_kw):
+    """Mixed cubes, spheres and convex polyhedra placed at random without
+    overlap (their bounding spheres fill `packing` of the box), top wall
+    moving down.
 
     h = 1e-4 m as in the other workloads: the wall contact spring (contact.h:385-502,
     K_n = 18K/(pi R_c^5), R_c = 0.9h) is stiffer per unit volume as R_c shrinks,
@@ -329,19 +389,16 @@ def crushing_aggregate(nparticles=512, nodes_range=(2000, 12000), seed=3, E=72e9
         else:
             L = lattice_convex((3 * target / (4 * math.pi)) ** (1 / 3) * 1.25, rng)
         sizes.append(L)
-    ext = max(int(np.abs(L).max()) for L in sizes) * 2 + 4
-    pitch = ext * h
-    m = int(math.ceil(nparticles ** (1 / 3)))
+    # bounding radius of each lattice (in h) plus the jitter
+    radii = np.array([float(np.sqrt((np.asarray(L, float) ** 2).sum(1)).max()) + 0.1 for L in sizes]) * h
     Rc = 0.9 * h
-    for k, L in enumerate(sizes):
-        ix, iy, iz = k % m, (k // m) % m, k // (m * m)
-        c = np.array([(ix + 0.5) * pitch, (iy + 0.5) * pitch, (iz + 0.5) * pitch])
+    centres, side = _rsa_centres(radii, packing, rng, margin=2 * h, gap=1.5 * Rc)
+    for L, c in zip(sizes, centres):
         parts.append(make_particle(L, h, c, rng, E, nu, rho, Gc))
-    top = (nparticles + m * m - 1) // (m * m)
-    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(E, nu, Rc),
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=contact_stiffness(Rc),
               friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=1,
               allow_wall=1, compute_wall_reaction=1,
-              wall_lrtb=(0, 0, 0, m * pitch, m * pitch, top * pitch),
+              wall_lrtb=(0, 0, 0, side, side, side),
               wall_speed=(0, 0, 0, 0, 0, top_speed), name="crushing_aggregate")
     kw.update(sys_kw)
     return System(particles=parts, **kw)
@@ -361,7 +418,7 @@ def impact_pair(nA, nB, v, E, seed=1, h=1e-4, nu=0.25, rho=2440.0, Gc=135.0, gap
     A.vel = np.zeros((A.n, 3))
     A.vel[:, 0] = v
     Rc = 0.9 * h
-    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(E, nu, Rc),
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=contact_stiffness(Rc),
               friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=1,
               name="impact_pair")
     kw.update(sys_kw)
@@ -437,7 +494,7 @@ def aggregate_job(job_id, nparticles_range=(2, 64), nodes_range=(2000, 8000),
         parts.append(p)
     Rc = 0.9 * h
     box = m * pitch
-    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=normal_stiffness(72e9, 0.25, Rc),
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=contact_stiffness(Rc),
               friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=1,
               allow_wall=1, wall_lrtb=(0, 0, 0, box, box, box), name="aggregate_job")
     kw.update(sys_kw)

@@ -39,7 +39,7 @@ def test_struct_layout_matches_header(tmp_path):
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "peridem_gpu.h"', "int main(){"]
     for cname, cls in (("pd_run_desc", abi.RunDesc), ("pd_particle_desc", abi.ParticleDesc),
                        ("pd_counters", abi.Counters), ("pd_pool_desc", abi.PoolDesc),
-                       ("pd_profile", abi.Profile)):
+                       ("pd_profile", abi.Profile), ("pd_serve_event", abi.ServeEvent)):
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for f, _ in cls._fields_:
             lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
@@ -53,7 +53,7 @@ def test_struct_layout_matches_header(tmp_path):
         cname, f, v = line.split()
         cls = {"pd_run_desc": abi.RunDesc, "pd_particle_desc": abi.ParticleDesc,
                "pd_counters": abi.Counters, "pd_pool_desc": abi.PoolDesc,
-               "pd_profile": abi.Profile}[cname]
+               "pd_profile": abi.Profile, "pd_serve_event": abi.ServeEvent}[cname]
         want = C.sizeof(cls) if f == "size" else getattr(cls, f).offset
         assert int(v) == want, (cname, f, v, want)
 

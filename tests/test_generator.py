@@ -50,3 +50,36 @@ def test_headline_workload_sizes():
     s = gen.gravity_settling(nspheres=2)
     assert [q.n for q in s.particles] == [8025, 8025]
     assert [q.nbonds for q in s.particles] == [741090, 740742]
+
+
+def test_contact_stiffness_is_the_reference_examples():
+    """normal_stiffness = peridem_3d(R_c).cnot / R_c
+    (examples/3D-HPC-aggregate-particle-breakage/setup.py:160): bulk modulus
+    2e9 Pa whatever the particles' material (material_dict.py:266-277)."""
+    Rc = 9e-5
+    assert math.isclose(gen.contact_stiffness(Rc), 18 * 2e9 / (math.pi * Rc ** 4) / Rc, rel_tol=1e-14)
+    s = gen.two_sphere_impact(R=4e-4, h=1e-4, v=20.0, seed=1)
+    assert s.normal_stiffness == gen.contact_stiffness(s.contact_rad)
+
+
+def test_workload_contact_is_stable_at_the_workloads_time_step(oracle_lib):
+    """Two spheres meeting at 10 m/s from just outside R_c, dt = 1e-8 s (the
+    reference algorithm, in the oracle): with the workloads' contact stiffness
+    they bounce -- speeds stay near 10 m/s and no bond breaks; with the 24x
+    stiffer spring of the particles' own bulk modulus the same impact blows
+    up (speeds beyond 1e3 m/s, bonds broken)."""
+    out = {}
+    for label, Kn in (("workload", None), ("particle K", gen.normal_stiffness(72e9, 0.25, 9e-5))):
+        s = gen.two_sphere_impact(R=4e-4, h=1e-4, v=10.0, seed=3, gap_h=1.0, b_movable=1, self_contact=1)
+        if Kn is not None:
+            s.normal_stiffness = Kn
+        o = oracle_lib.engine(s, prune=True)
+        o.step(600)
+        vmax = max(float(np.abs(o.state(p)["vel"]).max()) for p in range(2))
+        c = o.counters()
+        out[label] = (vmax, c["total_bonds"] - c["alive_bonds"], c["pair_computations"])
+        o.close()
+    v, broken, pairs = out["workload"]
+    assert pairs > 0 and v < 20.0 and broken == 0, out
+    v, broken, _ = out["particle K"]
+    assert v > 1e3 and broken > 0, out

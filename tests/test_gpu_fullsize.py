@@ -126,7 +126,7 @@ def test_crushing_aggregate_full_size_sample_matches_oracle(oracle_lib):
         p.vel = rng.uniform(-1.0, 1.0, size=(p.n, 3))
     print(f"[fullsize] crushing aggregate: {len(s.particles)} particles, {s.nodes} nodes, "
           f"{s.bonds} directed bonds", flush=True)
-    assert s.nodes == 2852640 and s.bonds == 256487986
+    assert s.nodes == 2852640 and s.bonds == 256481230
     sample = gen.System(**{**s.__dict__, "particles": s.particles[:2]})
     g = abi.gpu_engine(s)
     o = oracle_lib.engine(sample)
