@@ -140,6 +140,17 @@ int pd_set_wall(pd_ctx* ctx, const double lrtb[6]);
 int pd_set_particle_flag(pd_ctx* ctx, uint32_t p, int which, int value);
 /* TL.new_snot override (timeloop.h:314-318). */
 int pd_set_snot(pd_ctx* ctx, uint32_t p, double snot);
+/* Wheel reset (TL.reset_partzero_y at reset_partzero_y_timestep,
+ * timeloop.h:496-535): call it before the pd_step whose first step is that
+ * timestep.  Particle 0's reference positions move in y so that its mean
+ * current position sits wheel_rad + contact_rad above the highest node of the
+ * other particles (of its instance) lying within wheel_rad of it in x.  The
+ * bond vectors keep the positions the particle was added with (gen_xi,
+ * particle2.h:241-257); CurrPos, contact and walls see the moved ones.
+ * wheel_rad is CFGV.wheel_rad (read_config.h:58,333-334).  Particle 1 must
+ * have at least two nodes: the reference starts its running maximum from
+ * particle 1's node 1. */
+int pd_reset_partzero_y(pd_ctx* ctx, double wheel_rad);
 /* Prescribed velocity of one particle (timeloop.h:685-764): the particle
  * CFGV.particle_w_fixed_velocity with fix_velocity_mean_x /
  * prescribed_velocity_mean_x and fix_velocity_angular /

@@ -790,6 +790,50 @@ int or_set_snot(or_ctx* c, uint32_t p, double snot) {
   return PD_OK;
 }
 
+/* Wheel reset (timeloop.h:496-535), applied before the position half of the
+ * step it precedes.  Particle 0's mean current position m = mean_CurrPos()
+ * (particle2.h:94-103: sum of pos + disp in node order, then / nnodes); the
+ * bulk height is the largest pos_y + disp_y over the nodes of particles
+ * 1.. whose |m_x - (pos_x + disp_x)| <= wheel_rad, the first such node in
+ * particle and node order starting the running maximum; with no such node
+ * it stays at particle 1's pos[0]_y + disp[1]_y (node 1's displacement, as
+ * the reference initialises it).  Particle 0's reference positions move in y
+ * by (bulk + wheel_rad + contact_rad) - m_y.  Its bond vectors xi keep their
+ * values from gen_xi (particle2.h:241-257): only pos moves. */
+int or_reset_partzero_y(or_ctx* c, double wheel_rad) {
+  if (!c) return PD_E_ARG;
+  if (!c->committed) return fail(c, PD_E_STATE, "reset_partzero_y before commit");
+  if (c->np < 2 || c->P[1].n < 2)
+    return fail(c, PD_E_ARG, "reset_partzero_y reads node 1 of particle 1");
+  const opart* P0 = &c->P[0];
+  double m[3] = {0.0, 0.0, 0.0};
+  for (uint32_t i = 0; i < P0->n; i++)
+    for (int k = 0; k < 3; k++) m[k] += (P0->pos[3 * (size_t)i + k] + P0->disp[3 * (size_t)i + k]);
+  for (int k = 0; k < 3; k++) m[k] /= (double)P0->n;
+  double top = c->P[1].pos[1] + c->P[1].disp[3 + 1];
+  int init = 1;
+  for (uint32_t q = 1; q < c->np; q++) {
+    const opart* p = &c->P[q];
+    for (uint32_t j = 0; j < p->n; j++) {
+      const double now_x = p->pos[3 * (size_t)j] + p->disp[3 * (size_t)j];
+      const double now_y = p->pos[3 * (size_t)j + 1] + p->disp[3 * (size_t)j + 1];
+      if (fabs(m[0] - now_x) <= wheel_rad) {
+        if (init) {
+          top = now_y;
+          init = 0;
+        } else if (now_y > top) {
+          top = now_y;
+        }
+      }
+    }
+  }
+  const double dest = top + wheel_rad + c->run.contact_rad;
+  const double move = dest - m[1];
+  opart* W = &c->P[0];
+  for (uint32_t j = 0; j < W->n; j++) W->pos[3 * (size_t)j + 1] += move;
+  return PD_OK;
+}
+
 int or_get_state(or_ctx* c, uint32_t pi, double* disp, double* currpos, double* vel,
                  double* acc, double* force) {
   if (!c || pi >= c->np) return PD_E_ARG;

@@ -40,6 +40,9 @@ int or_set_wall_speed(or_ctx* ctx, const double speed[6]);
 int or_set_wall(or_ctx* ctx, const double lrtb[6]);
 int or_set_particle_flag(or_ctx* ctx, uint32_t p, int which, int value);
 int or_set_snot(or_ctx* ctx, uint32_t p, double snot);
+/* Wheel reset of particle 0's reference positions (timeloop.h:496-535),
+ * called before the step it precedes. */
+int or_reset_partzero_y(or_ctx* ctx, double wheel_rad);
 int or_get_state(or_ctx* ctx, uint32_t p, double* disp, double* currpos,
                  double* vel, double* acc, double* force);
 int or_set_state(or_ctx* ctx, uint32_t p, const double* disp,

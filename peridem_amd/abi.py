@@ -118,6 +118,7 @@ def _bind(lib, prefix):
         "set_wall": (C.c_int, [P, _pd]),
         "set_particle_flag": (C.c_int, [P, _u32, C.c_int, C.c_int]),
         "set_snot": (C.c_int, [P, _u32, _dbl]),
+        "reset_partzero_y": (C.c_int, [P, _dbl]),
         "get_state": (C.c_int, [P, _u32, _pd, _pd, _pd, _pd, _pd]),
         "set_state": (C.c_int, [P, _u32, _pd, _pd, _pd]),
         "get_connectivity": (C.c_int, [P, _u32, _pu, _u64, C.POINTER(_u64)]),
@@ -258,6 +259,10 @@ class Engine:
 
     def set_snot(self, p, snot):
         self._check(self.f["set_snot"](self.ctx, p, float(snot)), "set_snot")
+
+    def reset_partzero_y(self, wheel_rad):
+        """Wheel reset of particle 0 before the next step (timeloop.h:496-535)."""
+        self._check(self.f["reset_partzero_y"](self.ctx, float(wheel_rad)), "reset_partzero_y")
 
     def set_velocity_constraint(self, p, fix_mean_x=0, mean_x=0.0, fix_angular=0, angular_vel=0.0):
         self._check(self.f["set_velocity_constraint"](self.ctx, p, int(fix_mean_x), float(mean_x),

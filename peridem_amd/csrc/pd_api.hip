@@ -1020,7 +1020,7 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
 #define UP(dst, src, n) HIPCHK(c, hipMemcpy((dst), (src), (n) * sizeof(*(dst)), hipMemcpyHostToDevice))
   UP(d.X + g0, B->X.data(), Np);
   UP(d.U + g0, B->U.data(), Np);
-  hipLaunchKernelGGL(k_split, dim3(grid_of(Np, 256)), dim3(256), 0, c->stream, d, g0, Np);
+  hipLaunchKernelGGL(k_split, dim3(grid_of(Np, 256)), dim3(256), 0, c->stream, d, g0, Np, 1);
   HIPCHK(c, hipGetLastError());
   UP(d.V + 3 * g0, B->V.data(), 3 * Np);
   UP(d.A + 3 * g0, B->A.data(), 3 * Np);
@@ -1861,6 +1861,30 @@ int pd_set_snot(pd_ctx* c, uint32_t pi, double snot) {
   return rc ? rc : edit_part(c, j, p, 0, 0, snot, true);
 }
 
+int pd_reset_partzero_y(pd_ctx* c, double wheel_rad) {
+  if (!c) return PD_E_ARG;
+  uint32_t job, lp;
+  int rc = plain(c, 0, &job, &lp);
+  if (rc) return rc;
+  Job& J = c->jobs[job];
+  if (lp != 0 || J.parts.size() < 2 || J.parts[1].n < 2)
+    return fail(c, PD_E_ARG, "reset_partzero_y needs particle 0 and a particle 1 of at least two nodes");
+  if (!(wheel_rad >= 0.0)) return fail(c, PD_E_ARG, "wheel_rad must be >= 0");
+  HostPart& W = J.parts[0];
+  const HostPart& B = J.parts[1];
+  // the move's bits come back through the last counter slot (unused by steps)
+  double* out = (double*)(c->d.counters + 7);
+  hipLaunchKernelGGL(k_partzero, dim3(1), dim3(kResetThreads), 0, c->stream, c->d, W.slot, B.off,
+                     J.node0 + J.nodes, wheel_rad, J.run.contact_rad, out);
+  HIPCHK(c, hipGetLastError());
+  double move = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&move, out, sizeof move, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (uint32_t i = 0; i < W.n; i++) W.pos[3 * (size_t)i + 1] += move;
+  // particle 0 jumped: its job's candidate lists are rebuilt in the next step
+  return edit_inst(c, job, [](DevInst& I) { I.dirty = 1; });
+}
+
 int pd_job_get_state(pd_ctx* c, uint32_t job, uint32_t pi, double* disp, double* currpos, double* vel,
                      double* acc, double* force) {
   if (!c) return PD_E_ARG;
@@ -1932,7 +1956,7 @@ int pd_set_state(pd_ctx* c, uint32_t pi, const double* disp, const double* vel, 
     std::vector<double4> U(p.n);
     for (uint32_t i = 0; i < p.n; i++) U[i] = make_double4(disp[3 * i], disp[3 * i + 1], disp[3 * i + 2], 0.0);
     HIPCHK(c, hipMemcpy(c->d.U + p.off, U.data(), p.n * sizeof(double4), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_split, dim3(grid_of(p.n, 256)), dim3(256), 0, c->stream, c->d, p.off, (uint64_t)p.n);
+    hipLaunchKernelGGL(k_split, dim3(grid_of(p.n, 256)), dim3(256), 0, c->stream, c->d, p.off, (uint64_t)p.n, 0);
     HIPCHK(c, hipGetLastError());
     // a resumed run starts from lrtb = min/max of pos + disp (timeloop.h:389-393)
     double o[6] = {0, 0, 0, 0, 0, 0};

@@ -54,15 +54,21 @@ __device__ __forceinline__ bool reaction_step(const Dev& d, const DevInst& I) {
 }
 
 // ---------------------------------------------------------------------------
-// The bond kernel's neighbour copies of nodes [g0, g0+n) from X and U (after
-// an upload; the position kernel keeps the displacement copies current).
-__global__ __launch_bounds__(256) void k_split(Dev d, uint64_t g0, uint64_t n) {
+// The bond kernel's neighbour copies of nodes [g0, g0+n) from U and, when a
+// job is placed (pos), from X (after an upload; the position kernel keeps the
+// displacement copies current).  XY/ZV are the reference positions the bond
+// vectors are formed from and are written only at placement: a wheel reset
+// moves X alone.
+__global__ __launch_bounds__(256) void k_split(Dev d, uint64_t g0, uint64_t n, int pos) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint64_t g = g0 + i;
-  const double4 x = d.X[g], u = d.U[g];
-  d.XY[g] = make_double2(x.x, x.y);
-  d.ZV[g] = make_double2(x.z, x.w);
+  if (pos) {
+    const double4 x = d.X[g];
+    d.XY[g] = make_double2(x.x, x.y);
+    d.ZV[g] = make_double2(x.z, x.w);
+  }
+  const double4 u = d.U[g];
   d.UXY[g] = make_double2(u.x, u.y);
   d.UZ[g] = u.z;
 }
@@ -325,6 +331,82 @@ __global__ __launch_bounds__(64) void k_centroid(Dev d) {
   d.cent[3 * q] = m0 / n;
   d.cent[3 * q + 1] = m1 / n;
   d.cent[3 * q + 2] = m2 / n;
+}
+
+// ---------------------------------------------------------------------------
+// Wheel reset (timeloop.h:496-535) of a job's particle 0 (slot p0), before the
+// position half of the step it precedes.  One workgroup:
+//  * mean_CurrPos of particle 0 (particle2.h:94-103), one thread per
+//    component summing pos + disp in node order;
+//  * the bulk height: the largest pos_y + disp_y over the nodes [b0, b1)
+//    (the job's particles 1.. in order) with |m_x - (pos_x + disp_x)| <=
+//    wheel_rad.  The reference keeps the first node of largest value in that
+//    order (a later one replaces it only when strictly greater), so each
+//    thread keeps its first best in increasing node order and the threads'
+//    results combine by (larger value, then smaller node index);
+//  * with no such node, particle 1's pos[0]_y + disp[1]_y (node 1's
+//    displacement, as the reference starts its maximum);
+//  * move = ((bulk + wheel_rad) + Rc) - m_y is added to X.y of particle 0.
+// Only X moves: the bond kernels read the reference positions from XY/ZV,
+// which keep the values gen_xi formed the bond vectors from
+// (particle2.h:241-257).  The move is returned in *out for the host's copy.
+constexpr uint32_t kResetThreads = 256;
+__global__ __launch_bounds__(kResetThreads) void k_partzero(Dev d, uint32_t p0, uint64_t b0, uint64_t b1,
+                                                             double wheel_rad, double Rc, double* out) {
+  __shared__ double sm[3];
+  __shared__ double sv[kResetThreads];
+  __shared__ uint64_t si[kResetThreads];
+  __shared__ double smove;
+  constexpr uint64_t kNone = ~0ull;
+  const DevPart& P = d.part[p0];
+  const uint32_t tid = threadIdx.x;
+  if (tid < 3) {
+    double m = 0.0;
+    for (uint64_t g = P.node_off; g < P.node_off + P.nnodes; g++) {
+      const double4 x = d.X[g];
+      const double4 u = disp_of(d, g);
+      m += tid == 0 ? (x.x + u.x) : tid == 1 ? (x.y + u.y) : (x.z + u.z);
+    }
+    sm[tid] = m / (double)P.nnodes;
+  }
+  __syncthreads();
+  const double mx = sm[0];
+  double best = 0.0;
+  uint64_t bi = kNone;
+  for (uint64_t g = b0 + tid; g < b1; g += kResetThreads) {
+    const double4 x = d.X[g];
+    const double4 u = disp_of(d, g);
+    const double now_x = x.x + u.x, now_y = x.y + u.y;
+    if (fabs(mx - now_x) <= wheel_rad && (bi == kNone || now_y > best)) {
+      best = now_y;
+      bi = g;
+    }
+  }
+  sv[tid] = best;
+  si[tid] = bi;
+  __syncthreads();
+  if (tid == 0) {
+    for (uint32_t k = 1; k < kResetThreads; k++) {
+      const uint64_t oi = si[k];
+      if (oi == kNone) continue;
+      if (bi == kNone || sv[k] > best || (sv[k] == best && oi < bi)) {
+        best = sv[k];
+        bi = oi;
+      }
+    }
+    double top;
+    if (bi != kNone) {
+      top = best;
+    } else {
+      top = d.X[b0].y + disp_of(d, b0 + 1).y;
+    }
+    const double dest = top + wheel_rad + Rc;
+    smove = dest - sm[1];
+    *out = smove;
+  }
+  __syncthreads();
+  const double move = smove;
+  for (uint64_t g = P.node_off + tid; g < P.node_off + P.nnodes; g += kResetThreads) d.X[g].y += move;
 }
 
 // ---------------------------------------------------------------------------

@@ -268,11 +268,6 @@ static int run(Config& C) {
   // Wall.speed_bottom, which a 3-D RectWall never reads (contact.h:221-238):
   // in 3-D it changes nothing
   if (C.brazil) std::cout << "brazil: acts on the 2-D wall members only, no effect in 3-D" << std::endl;
-  // reset_partzero_y (timeloop.h:496-535) moves particle 0's reference
-  // positions while the bond terms keep the stored xi
-  if (C.reset_partzero_y)
-    throw std::runtime_error("reset_partzero_y (wheel reset of particle 0's reference positions) is not supported");
-
   // resume (timeloop.h:262-312)
   unsigned counter = 1, first_counter = 1, last_counter = C.timesteps / C.modulo;
   if (C.do_resume) {
@@ -390,7 +385,7 @@ static int run(Config& C) {
                   C.set_self_contact_timestep, C.set_damping_timestep, C.set_zero_wall_speed_timestep,
                   C.set_movable_timestep, C.set_stoppable_timestep})
       if (s != -1 && (unsigned)s == t) return true;
-    return false;
+    return C.reset_partzero_y && C.reset_partzero_y_timestep == t;
   };
   auto apply_switches = [&](unsigned t) {
     if (C.set_torque_timestep != -1 && t == (unsigned)C.set_torque_timestep) {
@@ -425,6 +420,12 @@ static int run(Config& C) {
     if (C.set_stoppable_index != -1 && (unsigned)C.set_stoppable_timestep == t) {
       check(ctx, pd_set_particle_flag(ctx, C.set_stoppable_index, PD_PFLAG_STOPPABLE, 1), "stoppable");
       std::cout << "Setting particle " << C.set_stoppable_index << " to stoppable on timestep " << t << std::endl;
+    }
+    // the wheel reset (timeloop.h:496-535): particle 0's reference positions
+    // move in y; its bond vectors keep the setup positions
+    if (C.reset_partzero_y && C.reset_partzero_y_timestep == t) {
+      check(ctx, pd_reset_partzero_y(ctx, C.wheel_rad), "pd_reset_partzero_y");
+      std::cout << "Resetting particle zero's mean y on timestep " << t << std::endl;
     }
   };
 

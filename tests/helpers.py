@@ -160,3 +160,28 @@ def passing_spheres(Rc_h=1.8, v=1500.0, seed=21):
     s.contact_rad = Rc_h * h
     s.normal_stiffness = 1e20
     return s
+
+
+def wheel_on_bulk(seed=31, wheel_y_h=9.0, wheel_vy=0.0, walls=False, self_contact=0):
+    """A small sphere (particle 0, the wheel) above a cube (particle 1, the
+    bulk), and a taller block (particle 2) off to the side in x: the setting
+    of the wheel reset (timeloop.h:496-535).  Every node carries a seeded
+    displacement; velocities are zero unless wheel_vy is set."""
+    h = 1e-4
+    rng = np.random.default_rng(seed)
+    W = gen.make_particle(gen.lattice_sphere(3.0), h, np.array([0.0, wheel_y_h * h, 0.0]), rng)
+    B = gen.make_particle(gen.lattice_cube(3.0), h, np.array([0.0, 0.0, 0.0]), rng)
+    T = gen.make_particle(gen.lattice_cube(2.0), h, np.array([12 * h, 2 * h, 0.0]), rng)
+    for p in (W, B, T):
+        p.disp = rng.uniform(-2e-8, 2e-8, size=(p.n, 3))
+        p.vel = np.zeros((p.n, 3))
+    W.vel[:, 1] = wheel_vy
+    Rc = 0.9 * h
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=gen.contact_stiffness(Rc),
+              friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=self_contact,
+              allow_damping=1, allow_friction=1, name="wheel_on_bulk")
+    if walls:
+        kw.update(allow_wall=1, compute_wall_reaction=1,
+                  wall_lrtb=(-5 * h, -4 * h, -4 * h, 15 * h, 14 * h, 4 * h),
+                  wall_speed=(0.0, 0.0, 0.0, 0.0, -2.0, 0.0))
+    return gen.System(particles=[W, B, T], **kw)

@@ -311,3 +311,39 @@ def test_driver_time_switches_and_shuffled_connectivity(tmp_path, oracle_lib):
         assert o.counters()["alive_bonds"] < o.counters()["total_bonds"]     # bonds broke after step 4
     finally:
         o.close()
+
+
+@pytest.mark.gpu
+def test_driver_wheel_reset_matches_the_oracle(tmp_path, oracle_lib):
+    """reset_partzero_y / reset_partzero_y_timestep / wheel_rad through the
+    config (read_config.h:327-335, timeloop.h:496-535): the driver moves the
+    wheel before step 5, the tc_ files after it match an oracle given the
+    same reset bit for bit."""
+    from helpers import wheel_on_bulk
+    s = wheel_on_bulk(wheel_vy=-3.0, walls=True, self_contact=1)
+    setup = tmp_path / "setup.pdbin"
+    setupio.write_setup(s, setup)
+    T, modulo, at, wheel_rad = 16, 4, 5, 2e-4
+    cfg = tmp_path / "main.conf"
+    setupio.write_config(cfg, timesteps=T, modulo=modulo, dt=s.dt, enable_fracture=1, self_contact=1,
+                         allow_damping=1, allow_friction=1, compute_wall_reaction=1,
+                         reset_partzero_y=1, reset_partzero_y_timestep=at, wheel_rad=wheel_rad,
+                         **_wall_keys(s))
+    out = tmp_path / "out"
+    r = run_driver("-c", str(cfg), "-i", str(setup), "-o", str(out))
+    assert f"Resetting particle zero's mean y on timestep {at}" in r.stdout
+    has_h5 = "hdf5 yes" in run_driver("--features").stdout
+    o = oracle_lib.engine(s)
+    try:
+        for t in range(1, T + 1):
+            if t == at:
+                o.reset_partzero_y(wheel_rad)
+            o.step(1)
+            if t % modulo:
+                continue
+            got = _read_out(out / f"tc_{t // modulo:05d}.h5", tmp_path, has_h5, "tc")
+            _compare_tc(got, [o.state(p) for p in range(3)], [o.connectivity(p) for p in range(3)], t,
+                        "wheel")
+        assert o.counters()["pair_computations"] > 0
+    finally:
+        o.close()

@@ -116,8 +116,10 @@ def test_crushing_aggregate_full_size_sample_matches_oracle(oracle_lib):
     and convex polyhedra of 2k-12k nodes, 2.85M nodes, 256M directed bonds,
     self-contact, walls with the top one moving, wall reaction on), every
     node with a seeded velocity so every bond carries stretch.  The particles
-    sit in separate grid cells (gaps of at least 4h > R_c, 2h to the walls),
-    so for these few steps no contact exists and two of them, stepped inside
+    are placed with their bounding spheres at least 1.5 R_c apart and clear
+    of the walls, so particle pairs pass the box test (the boxes are extended
+    by R_c) but for these few steps (nodes move < 1e-7 m) no node comes within
+    R_c of another particle's node or of a wall: two of them, stepped inside
     the full pool, must equal the oracle stepping those two alone with the
     same walls, bit for bit."""
     s = gen.crushing_aggregate(512)
@@ -146,7 +148,9 @@ def test_crushing_aggregate_full_size_sample_matches_oracle(oracle_lib):
             assert_bitwise(lg, lo, f"step {c} wall planes")
             assert not np.any(rg), "no node touches a wall yet"
         cnt = g.counters()
-        assert cnt["alive_bonds"] == s.bonds and cnt["pair_computations"] == 0
+        assert cnt["alive_bonds"] == s.bonds
+        print(f"[fullsize] {cnt['pair_computations']} particle pairs passed the box test in 10 steps "
+              f"(none within R_c)", flush=True)
     finally:
         g.close()
         o.close()

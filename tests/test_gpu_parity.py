@@ -338,3 +338,39 @@ def test_pair_activity_lags_one_step_like_the_oracle(oracle_lib):
     s = gen.System(particles=[A, B], dt=1e-8, contact_rad=Rc, normal_stiffness=4.6e31,
                    friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=0)
     run_pair(s, (1, 2, 3), oracle_lib, "lagged pair boxes")
+
+
+@pytest.mark.parametrize("first_reset", [1, 12])
+def test_wheel_reset_matches_oracle(oracle_lib, first_reset):
+    """reset_partzero_y (timeloop.h:496-535): the wheel is moved before step
+    first_reset and again 20 steps later; wheel_rad = 2h drops it into the
+    bulk below, so both resets are followed by wheel-bulk contact, self-contact,
+    walls and bond breaks.  The bond kernel keeps the setup positions (xi)
+    while contact and CurrPos use the moved ones: bitwise against the oracle
+    after each reset and at later checkpoints."""
+    from helpers import wheel_on_bulk
+    s = wheel_on_bulk(wheel_vy=-3.0, walls=True, self_contact=1)
+    wheel_rad = 2e-4
+    print(f"[parity] wheel reset at {first_reset}: nodes={s.nodes} bonds={s.bonds}", flush=True)
+    g, o = abi.gpu_engine(s), oracle_lib.engine(s)
+    try:
+        done = 0
+        plan = [(first_reset - 1, True), (1, False), (19, True), (1, False), (30, False)]
+        pairs0 = 0
+        for n, reset in plan:
+            if n:
+                g.step(n)
+                o.step(n)
+                done += n
+            if reset:
+                g.reset_partzero_y(wheel_rad)
+                o.reset_partzero_y(wheel_rad)
+            else:
+                rg, ro = compare_engines(g, o, s, f"wheel step {done}")
+                assert_bitwise(rg, ro, f"wheel step {done} wall reaction")
+            if done == first_reset:
+                pairs0 = g.counters()["pair_computations"]
+        assert g.counters()["pair_computations"] > pairs0, "the wheel lands in contact with the bulk"
+    finally:
+        g.close()
+        o.close()

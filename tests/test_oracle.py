@@ -13,6 +13,8 @@ sign or a swapped operand fails at least one check:
                closed form, force sign per plane, reaction = -sum V f
   Verlet       free flight, constant-force parabola, clamped nodes
   torque       tangential direction and magnitude on a ring
+  wheel reset  height above the bulk within the x window, bond vectors
+               kept, the initial running maximum from particle 1's node 1
 """
 import math
 
@@ -641,3 +643,96 @@ def test_pruned_contact_is_bitwise_the_all_pairs_loop(oracle_lib, case):
     finally:
         a.close()
         b.close()
+
+
+# ---------------------------------------------------------------- wheel reset
+def _wheel_top(s, wheel_rad):
+    """Highest pos_y + disp_y among the nodes of particles 1.. within
+    wheel_rad in x of particle 0's mean current position."""
+    W = s.particles[0]
+    mx = np.mean(W.pos[:, 0] + W.disp[:, 0])
+    ys = []
+    for p in s.particles[1:]:
+        cx, cy = p.pos[:, 0] + p.disp[:, 0], p.pos[:, 1] + p.disp[:, 1]
+        ys.append(cy[np.abs(mx - cx) <= wheel_rad])
+    ys = np.concatenate(ys)
+    return ys.max() if ys.size else None
+
+
+def test_wheel_reset_sits_the_wheel_on_the_bulk_below_it(oracle_lib):
+    """timeloop.h:496-535: particle 0's mean current position ends
+    wheel_rad + contact_rad above the highest node of the other particles
+    lying within wheel_rad of it in x.  Particle 2 is taller than the bulk but
+    10h away in x (outside the window); particle 0 itself is higher than the
+    bulk and inside it (the scan starts at particle 1).  With zero velocity
+    and acceleration the step's position half leaves disp unchanged, so
+    CurrPos = moved pos + disp."""
+    from helpers import wheel_on_bulk
+    h = 1e-4
+    s = wheel_on_bulk()
+    wheel_rad = 4 * h
+    top = _wheel_top(s, wheel_rad)
+    assert top is not None and top < np.max(s.particles[2].pos[:, 1])
+    e = oracle_lib.engine(s)
+    e.reset_partzero_y(wheel_rad)
+    e.step(1)
+    W = s.particles[0]
+    cp = e.state(0)["currpos"]
+    assert np.mean(cp[:, 1]) == pytest.approx(top + wheel_rad + s.contact_rad, abs=1e-15)
+    # only y moves, by one amount for every node
+    assert np.array_equal(cp[:, [0, 2]], W.pos[:, [0, 2]] + W.disp[:, [0, 2]])
+    move = cp[:, 1] - (W.pos[:, 1] + W.disp[:, 1])
+    assert np.ptp(move) < 1e-18 and abs(move[0] - (top + wheel_rad + s.contact_rad - 9 * h)) < 2e-6
+    # the other particles keep their positions
+    for q in (1, 2):
+        P = s.particles[q]
+        assert np.array_equal(e.state(q)["currpos"], P.pos + P.disp)
+
+
+def test_wheel_reset_keeps_the_bond_vectors(oracle_lib):
+    """The bond terms use xi from gen_xi (particle2.h:241-257), formed before
+    the move: after the reset the wheel's bond forces are bitwise those of
+    the same displacements without the move (the wheel lands clear of the
+    bulk), and differ from those of a wheel whose reference positions were
+    moved before its bonds were formed (xi re-rounded)."""
+    from helpers import wheel_on_bulk
+    h = 1e-4
+    wheel_rad = 4 * h
+    moved, plain = oracle_lib.engine(wheel_on_bulk()), oracle_lib.engine(wheel_on_bulk())
+    moved.reset_partzero_y(wheel_rad)
+    moved.step(1)
+    plain.step(1)
+    f_moved, f_plain = moved.state(0)["force"], plain.state(0)["force"]
+    assert np.any(f_plain != 0)
+    assert np.array_equal(f_moved.view(np.uint64), f_plain.view(np.uint64))
+    # the same move applied to the setup positions re-rounds xi
+    s = wheel_on_bulk()
+    dy = moved.state(0)["currpos"][:, 1] - (s.particles[0].pos[:, 1] + s.particles[0].disp[:, 1])
+    s.particles[0].pos = s.particles[0].pos.copy()
+    s.particles[0].pos[:, 1] += dy[0]
+    pre = oracle_lib.engine(s)
+    pre.step(1)
+    assert not np.array_equal(pre.state(0)["force"].view(np.uint64), f_plain.view(np.uint64))
+
+
+def test_wheel_reset_without_bulk_below_starts_from_particle_one_node_one(oracle_lib):
+    """With no node of particles 1.. within wheel_rad in x, the running
+    maximum keeps its initial value pos[0]_y + disp[1]_y of particle 1
+    (node 0's position, node 1's displacement: timeloop.h:504)."""
+    from helpers import wheel_on_bulk
+    h = 1e-4
+    s = wheel_on_bulk()
+    W, B = s.particles[0], s.particles[1]
+    W.pos = W.pos.copy()
+    W.pos[:, 0] += 40 * h   # far from every other node in x
+    B.disp = B.disp.copy()
+    B.disp[0, 1], B.disp[1, 1] = -7e-7, 5e-7
+    wheel_rad = 1e-9
+    assert _wheel_top(s, wheel_rad) is None
+    e = oracle_lib.engine(s)
+    e.reset_partzero_y(wheel_rad)
+    e.step(1)
+    cp = e.state(0)["currpos"]
+    expect = (B.pos[0, 1] + B.disp[1, 1]) + wheel_rad + s.contact_rad
+    assert np.mean(cp[:, 1]) == pytest.approx(expect, abs=1e-15)
+    assert abs(np.mean(cp[:, 1]) - ((B.pos[0, 1] + B.disp[0, 1]) + wheel_rad + s.contact_rad)) > 1e-6
