@@ -348,6 +348,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle-s", type=float, default=0.5,
+                    help="untimed steps for this long after the first (tuning) step, before the warm-up")
     ap.add_argument("--phase-steps", type=int, default=40,
                     help="steps after the timed ones with a mark around every phase")
     ap.add_argument("--spheres", type=int, default=125)
@@ -422,6 +424,14 @@ def main():
     if args.bond_shape >= 0:
         e.set_bond_shape(args.bond_shape)
     log(f"[bench] committed to the GPU in {time.perf_counter() - t0:.1f} s")
+    # the first step tunes the bond kernel and builds the candidate lists;
+    # then untimed steps for settle_s seconds bring the GPU to the clocks it
+    # runs a long simulation at, before the warm-up steps
+    e.step(1)
+    settled, t0 = 1, time.perf_counter()
+    while time.perf_counter() - t0 < args.settle_s:
+        e.step(50)
+        settled += 50
     e.step(args.warmup)
     # same number of steps replayed from the step graphs, no marks: logged
     # beside the timed rate so the cost of the marks stays visible
@@ -443,6 +453,17 @@ def main():
     e.set_profiling(abi.PD_PROFILE_PHASES)
     e.step(args.phase_steps)
     phases = e.profile()
+    # full rebuilds of every node's candidate list, which the timed window
+    # does not hold (lists are rebuilt once some node has moved half the
+    # skin): a particle flag set to the value it has marks the job for a
+    # rebuild in its next step, timed by that step's phase marks
+    rebuilds_seen, steps_seen = e.counters()["cell_rebuilds"], e.counters()["steps"]
+    rebuild_ms = []
+    for _ in range(3):
+        e.set_profiling(abi.PD_PROFILE_PHASES)
+        e.set_particle_flag(0, abi.PD_PFLAG_MOVABLE, 1)
+        e.step(1)
+        rebuild_ms.append(e.profile()["contact_list"]["ms"])
     e.set_profiling(0)
     bu = c1["bond_updates"] - c0["bond_updates"]
     log(f"[bench] {args.steps} steps: {elapsed * 1e3:.2f} ms wall, {dev_ms:.2f} ms device "
@@ -455,6 +476,9 @@ def main():
     log(f"[bench] bond kernel over the timed steps: {force_ms:.4f} ms per launch; phases per step over "
         f"{args.phase_steps} more steps (ms): " +
         ", ".join(f"{k}={v['ms'] / max(v['launches'], 1):.4f}" for k, v in phases.items()))
+    log(f"[bench] full candidate-list rebuild: {', '.join(f'{x:.3f}' for x in rebuild_ms)} ms "
+        f"(contact-list phase of a step that rebuilds every list); {rebuilds_seen} rebuilds in the "
+        f"first {steps_seen} steps")
     cnt = e.counters()
     kname = shape_name(e.bond_shape(), N)
     fi = fp64_instr(kname)
@@ -531,7 +555,11 @@ def main():
                    "nodes": N, "directed_bonds": B, "movable_bonds": Bm,
                    "alive_bonds": cnt["alive_bonds"], "parallelism": "single GPU",
                    "step_hbm_fraction": step_bytes / (step_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
-                   "phase_ms": {k: v["ms"] / max(v["launches"], 1) for k, v in phases.items()}},
+                   "phase_ms": {k: v["ms"] / max(v["launches"], 1) for k, v in phases.items()},
+                   "list_rebuild": {"full_rebuild_ms": sum(rebuild_ms) / len(rebuild_ms),
+                                    "samples_ms": rebuild_ms,
+                                    "rebuilds_before_forced": rebuilds_seen, "steps_before_forced": steps_seen,
+                                    "settle_steps": settled}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(N, Bm, kname),
                      "kernel": kname, "bytes_per_launch": force_bytes,

@@ -43,12 +43,12 @@ constexpr int kGraphSteps = 8;  // steps per captured graph
 constexpr int kProfRing = 16;   // event sets of profiled steps in flight
 constexpr int kProfLag = 8;     // a profiled step's marks are read this many steps later
 // device bytes per node slot (X, U, V, Vold, A, F, E, pid, nflag, deg, wmask, ckey,
-// csorted, ~2 hash buckets) and per bond entry
+// csorted, kCellsPerSlot cell-table keys) and per bond entry
 // node arrays + the bond kernel's record copies and both displacement
 // buffers + block tables (kLdsCap per 256 nodes); per entry the u32 and u16
 // forms.  The candidate lists (4 B per node and list entry) are sized for the
 // pool as a whole.
-constexpr uint64_t kBytesPerNode = 32 + 32 + 5 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 16 + 80 + 36 + 2;
+constexpr uint64_t kBytesPerNode = 32 + 32 + 5 * 24 + 4 + 1 + 4 + 1 + 4 + 4 + 8 * kCellsPerSlot + 80 + 36 + 2;
 constexpr uint64_t kBytesPerEntry = 6;
 
 struct HostPart {
@@ -244,6 +244,38 @@ static void fill_inst(const pd_run_desc& r, DevInst& D) {
   D.enable_torque = r.enable_torque;
   D.nl_bdry_only = r.nl_bdry_only;
   D.vc_slot = kNoKey;
+}
+
+// Cells per axis of a job's contact grid: its nodes' box at placement (the
+// particles' boxes) widened by an eighth of its extent and four cells on
+// each side, and no wider than the walls' box (plus a cell and a half) when
+// the job has walls, in cells of edge `cell`; scaled down together until the
+// grid fits `share` keys.  Any size is correct (positions wrap around the axes); one that spans
+// the job keeps its cells on distinct keys.
+static void job_grid(const double* lrtb, uint32_t np, const pd_run_desc& r, double cell, uint64_t share,
+                     uint32_t gn[3]) {
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (uint32_t i = 0; i < np; i++)
+    for (int k = 0; k < 3; k++) {
+      lo[k] = std::min(lo[k], lrtb[6 * i + k]);
+      hi[k] = std::max(hi[k], lrtb[6 * i + 3 + k]);
+    }
+  double e[3];
+  for (int k = 0; k < 3; k++) {
+    double n = 1.25 * (hi[k] - lo[k]) / cell + 8.0;
+    if (r.allow_wall) n = std::min(n, (r.wall_lrtb[3 + k] - r.wall_lrtb[k]) / cell + 3.0);
+    if (!(n >= 1.0)) n = 1.0;
+    e[k] = std::min(n, 1.0e6);
+  }
+  uint64_t n[3];
+  for (int k = 0; k < 3; k++) n[k] = (uint64_t)e[k];
+  while (n[0] * n[1] * n[2] > share) {
+    const double f = std::cbrt((double)share / (double)(n[0] * n[1] * n[2])) * 0.999;
+    for (int k = 0; k < 3; k++) n[k] = std::max<uint64_t>(1, (uint64_t)((double)n[k] * f));
+    if (n[0] * n[1] * n[2] > share)   // rounding: shrink the longest axis
+      *std::max_element(n, n + 3) -= 1;
+  }
+  for (int k = 0; k < 3; k++) gn[k] = (uint32_t)n[k];
 }
 
 static int check_run(pd_ctx* c, const pd_run_desc& r) {
@@ -541,10 +573,11 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   c->cap_parts = pool->max_particles;
   c->cap_jobs = pool->max_jobs;
   const uint64_t P = c->cap_parts + 1, I = c->cap_jobs + 1;  // + the void slots
-  // cell hash table: a power of two >= the node slots (occupied cells of edge
-  // rb hold ~1-2 nodes each, so it is at most about half full)
-  uint32_t T = 1024;
-  while (T < N && T < (1u << 31)) T <<= 1;
+  // contact cell table: kCellsPerSlot keys per node slot, each job's grid in
+  // the keys of its own node slots
+  if (N * kCellsPerSlot >= 0xFFFFFFFFull)
+    return fail(c, PD_E_ARG, "more than %u node slots", (unsigned)(0xFFFFFFFFull / kCellsPerSlot));
+  const uint32_t T = (uint32_t)(N * kCellsPerSlot);
   Dev& d = c->d;
   d.N = N;
   d.S = S;
@@ -612,6 +645,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   ALLOC(ccur, T);
   ALLOC(csorted, N);
   ALLOC(scan_tmp, T / kScanBlock + 2);
+  ALLOC(rjobs, I);
+  ALLOC(jkeyed, I);
 #undef ALLOC
   hipStream_t st = c->stream;
   // every slot starts void: node -> void particle, void particle -> void job
@@ -636,6 +671,8 @@ int pd_reserve(pd_ctx* c, const pd_pool_desc* pool) {
   HIPCHK(c, hipMemsetAsync(d.soff, 0, S * sizeof(uint64_t), st));
   HIPCHK(c, hipMemsetAsync(d.swid, 0, S * sizeof(uint32_t), st));
   HIPCHK(c, hipMemsetAsync(d.cstart, 0, ((size_t)T + 1) * sizeof(uint32_t), st));
+  HIPCHK(c, hipMemsetAsync(d.rjobs, 0, I * sizeof(uint32_t), st));
+  HIPCHK(c, hipMemsetAsync(d.jkeyed, 0, I * sizeof(uint32_t), st));
   if (c->cap_entries) HIPCHK(c, hipMemsetAsync(d.bond, 0xFF, c->cap_entries * sizeof(uint32_t), st));
   if (c->cap_entries) HIPCHK(c, hipMemsetAsync(d.bloc, 0xFF, c->cap_entries * sizeof(uint16_t), st));
   HIPCHK(c, hipMemsetAsync(d.btab, 0, S / kBlockSlices * kLdsCap * sizeof(uint32_t), st));
@@ -1014,6 +1051,8 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
   I.node0 = g0;
   I.node1 = g0 + B->nodes;
   I.t = B->run.first_step ? B->run.first_step : 1;
+  job_grid(B->lrtb.data(), np, B->run, I.cell, Np * kCellsPerSlot, I.gn);
+  I.gtb = (uint32_t)(g0 * kCellsPerSlot);
   const double zero18[18] = {0};
   Dev& d = c->d;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1176,15 +1215,13 @@ int pd_commit(pd_ctx* c) {
 // One explicit step of every resident job (timeloop.h:395-797).  Contains no
 // memset and no host synchronisation unless profiling, so it can be captured.
 // The pool as the step kernels see it: node slots up to the highest one in
-// use, and a cell hash table sized to them.
+// use, and the keys of the cell table that belong to them.
 static Dev step_view(const pd_ctx* c) {
   Dev d = c->d;
   d.S = c->hwm_slices;
   d.N = d.S * kSlice;
   d.lists = any_contact_possible(c) ? 1u : 0u;
-  uint32_t T = 1024;
-  while (T < d.N && T < c->d.T) T <<= 1;
-  d.T = T;
+  d.T = (uint32_t)(d.N * kCellsPerSlot);
   return d;
 }
 
@@ -1261,9 +1298,9 @@ static int launch_step(pd_ctx* c, int par, hipEvent_t* ev = nullptr) {
   if (any_torque(c)) hipLaunchKernelGGL(k_centroid, dim3(grid_of(d.P, 64)), dim3(64), 0, st, d);
   if (d.lists) {
     // every kernel of the list rebuild exits at once unless a job rebuilds
-    const uint32_t nb = grid_of(d.T, kScanBlock);
-    hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(kScanBlock), 0, st, d.ctl, d.cstart, d.T, d.scan_tmp);
-    hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(kScanBlock), 0, st, d.ctl, d.cstart, d.T, d.scan_tmp, d.ccur);
+    const uint32_t nb = d.T / kScanBlock;
+    hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(kScanThreads), 0, st, d);
+    hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(kScanThreads), 0, st, d);
     hipLaunchKernelGGL(k_scatter, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_build, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);
     if (any_bdry_only(c)) hipLaunchKernelGGL(k_build_bdry, dim3(grid_of(d.N, 256)), dim3(256), 0, st, d);

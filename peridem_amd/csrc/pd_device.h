@@ -29,6 +29,14 @@ static_assert(kLive16 <= kDead16, "table offsets must fit a u16 below the dead v
 constexpr int kMaxCand = 48;
 constexpr uint32_t kMaxCap = 255;
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+// keys of the contact cell table per node slot: a job's grid gets this many
+// cells per node slot it occupies (occupied cells of edge rb hold 1-2 nodes,
+// so this covers a bounding box about 2-6 times the volume of the nodes)
+constexpr uint32_t kCellsPerSlot = 4;
+// the cell table is scanned in blocks of kScanBlock keys: one block per
+// 256-node block, so each scan block belongs to one job
+constexpr int kScanBlock = 1024;
+static_assert(kScanBlock == 256 * kCellsPerSlot, "one scan block per node block");
 // state bits of Dev::flags
 constexpr uint32_t kFlagSafeMath = 1u;  // an operand left the fast-path range: use IEEE ops
 
@@ -77,6 +85,8 @@ struct DevInst {
   int32_t vc_on, vc_fix_mean_x, vc_fix_ang, pad2;
   double vc_mean_x, vc_omega;
   double vc_mean[3], vc_cent[3];   // its mean velocity and mean CurrPos of this step
+  uint32_t gn[3];           // contact cell grid: cells per axis (see cell_key)
+  uint32_t gtb;             // its first key in the cell table
 };
 
 // Per-launch control words shared by the kernels of a step.
@@ -90,6 +100,7 @@ struct DevCtl {
   // runs the step again.
   uint32_t halt;
   uint32_t need;            // longest list the halted build asked for
+  uint32_t nrj;             // jobs that rebuild their lists in this step (listed in Dev::rjobs)
 };
 
 struct Dev {
@@ -168,8 +179,10 @@ struct Dev {
   uint32_t* ccur;    // [T]
   uint32_t* csorted; // [N]
   uint32_t* scan_tmp;
+  uint32_t* rjobs;   // [ninst] the jobs that rebuild their lists in this step (DevCtl::nrj of them)
+  uint32_t* jkeyed;  // [ninst] nodes a rebuilding job put into its cells this step
   uint64_t N, S, P, npairs, ninst;   // N = S*64 node slots; P, ninst incl. the void slot
-  uint32_t T;        // hash table size (power of two)
+  uint32_t T;        // cell table size: kCellsPerSlot keys per node slot
   uint32_t lists;    // some job of the pool can have contact: candidate lists are kept
   uint32_t ccap;     // candidate list capacity per node (kMaxCand .. kMaxCap)
 };
@@ -280,13 +293,34 @@ __host__ __device__ inline double okey_inv(unsigned long long k) {
   return d;
 }
 
-__device__ __forceinline__ uint32_t cell_hash(int64_t cx, int64_t cy, int64_t cz, uint32_t inst, uint32_t T) {
-  uint64_t h = (uint64_t)cx * 0x9E3779B97F4A7C15ull;
-  h ^= (uint64_t)cy * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
-  h ^= (uint64_t)cz * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
-  h ^= (uint64_t)inst * 0xD6E8FEB86659FD93ull + (h << 6) + (h >> 2);
-  h ^= h >> 29;
-  return (uint32_t)h & (T - 1);
+// Contact cells.  Each job has a grid of gn[0] x gn[1] x gn[2] cells of edge
+// `cell` at keys gtb .. gtb + gn[0] gn[1] gn[2] - 1 of the cell table, x
+// fastest, so the three cells of a row of neighbours along x are adjacent
+// keys.  A position's cell is floor(p / cell) taken modulo gn per axis: the
+// map is one to one on any gn consecutive cells, so a job whose nodes span
+// at most gn cells per axis never puts two of its cells on one key; far-apart
+// cells of a wider job may share one, and the distance test sorts their nodes
+// apart.
+__device__ __forceinline__ uint32_t wrap_cell(double p, double cell, uint32_t n) {
+  double q = floor(p / cell);
+  if (!(q > -0x1p52 && q < 0x1p52)) q = 0.0;   // NaN or beyond exact integers: any cell
+  // q mod n in exact integer arithmetic on doubles: floor(q / n) may be one
+  // off after rounding, which leaves m in [-n, 2n)
+  const double dn = (double)n;
+  double m = q - dn * floor(q / dn);
+  if (m < 0.0) m += dn;
+  if (m >= dn) m -= dn;
+  return (uint32_t)m;
+}
+// w + dd (dd in -1..1) modulo n, w in [0, n)
+__device__ __forceinline__ uint32_t wrap_add(uint32_t w, int dd, uint32_t n) {
+  int64_t v = (int64_t)w + dd;
+  if (v < 0) v += n;
+  if (v >= (int64_t)n) v -= n;
+  return (uint32_t)v;
+}
+__device__ __forceinline__ uint32_t cell_key(const DevInst& I, uint32_t wx, uint32_t wy, uint32_t wz) {
+  return I.gtb + (wz * I.gn[1] + wy) * I.gn[0] + wx;
 }
 
 // intersects_box_fext<3> (contact.h:1143-1183): A extended by c.

@@ -108,7 +108,10 @@ __global__ __launch_bounds__(256) void k_position(Dev d) {
   // the block's wall-contact flag of this step starts clear, and so does the
   // step's rebuild flag (set by k_cellkey)
   if (threadIdx.x == 0 && g < d.N) d.bflag[g >> 8] = 0;
-  if (g == 0) d.ctl->any_rebuild = 0;
+  if (g == 0) {
+    d.ctl->any_rebuild = 0;
+    d.ctl->nrj = 0;
+  }
   double cp[3] = {0, 0, 0};
   uint32_t p = 0xFFFFFFFFu, jid = 0xFFFFFFFFu;
   unsigned long long dkey = 0ull;
@@ -465,8 +468,9 @@ __device__ __forceinline__ bool self_active(const DevPart& P, const DevInst& I) 
   return P.movable && I.enable_fracture && P.breakable && I.self_contact && P.stoppable;
 }
 
-__device__ __forceinline__ void cell_of(const double* cp, double cell, int64_t* c) {
-  for (int k = 0; k < 3; k++) c[k] = (int64_t)floor(cp[k] / cell);
+// The node's cell in its job's grid (wrapped coordinates).
+__device__ __forceinline__ void cell_of(const double* cp, const DevInst& I, uint32_t* w) {
+  for (int k = 0; k < 3; k++) w[k] = wrap_cell(cp[k], I.cell, I.gn[k]);
 }
 
 // One launch per step for the independent per-step tests (thread q: pair q,
@@ -479,7 +483,10 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
   if (q < d.npairs) pair_test(d, q);
   if (q < d.P) wall_test(d, q);
   if (!d.lists) return;   // no job of the pool can have contact: no lists
-  if (q < d.ninst && rebuilds(d, (uint32_t)q)) atomicOr(&d.ctl->any_rebuild, 1u);
+  if (q < d.ninst && rebuilds(d, (uint32_t)q)) {
+    atomicOr(&d.ctl->any_rebuild, 1u);
+    d.rjobs[atomicAdd(&d.ctl->nrj, 1u)] = (uint32_t)q;   // its cell blocks are cleared by k_finalize
+  }
   const uint64_t g = q;
   if (g >= d.N) return;
   const uint32_t p = d.pid[g];
@@ -491,68 +498,108 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
     double4 x = d.X[g];
     double4 u = disp_of(d, g);
     double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
-    int64_t c[3];
-    cell_of(cp, I.cell, c);
-    key = cell_hash(c[0], c[1], c[2], P.inst, d.T);
+    uint32_t w[3];
+    cell_of(cp, I, w);
+    key = cell_key(I, w[0], w[1], w[2]);
     atomicAdd(&d.cstart[key], 1u);
     d.cpos[g] = make_double4(cp[0], cp[1], cp[2], 0.0);
   }
   d.ckey[g] = key;
 }
 
-// exclusive scan of cstart[0..T) (counts) in place; cstart[T] = total.
-// Two phases: per-block scan (k_scan1), add-back (k_scan3).
-constexpr int kScanBlock = 1024;
-__global__ __launch_bounds__(kScanBlock) void k_scan1(const DevCtl* ctl, uint32_t* a, uint32_t n, uint32_t* sums) {
-  if (ctl->halt || !ctl->any_rebuild) return;
-  __shared__ uint32_t s[kScanBlock];
-  uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
-  uint32_t v = i < n ? a[i] : 0;
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
-    __syncthreads();
-    s[threadIdx.x] += t;
-    __syncthreads();
+// Exclusive scan of the cell counts of the rebuilding jobs, each job's
+// cells on their own: cstart[k] becomes node0 + the number of the job's
+// keyed nodes in cells before k, so a job's nodes in cell order fill the
+// slots [node0, node0 + jkeyed[job]) of csorted / cps, inside its own node
+// range.  Scan block b holds the keys of node block b, so it belongs to one
+// job; blocks of jobs that do not rebuild are left alone (their counts are
+// zero and nothing reads them).  Two phases: per-block scan (k_scan1) and
+// add-back (k_scan3).
+// 256 threads per scan block, four keys each (one 16-byte load).
+constexpr int kScanThreads = kScanBlock / 4;
+__global__ __launch_bounds__(kScanThreads) void k_scan1(Dev d) {
+  if (d.ctl->halt || !d.ctl->any_rebuild) return;
+  const uint32_t b = blockIdx.x;
+  const uint32_t job = d.part[ld_uniform(d.pid, b * 256)].inst;
+  if (!rebuilds_node(d, job)) return;
+  // the thread's four counts, then an inclusive scan of the threads' totals
+  // within each wavefront and the wavefronts' totals
+  __shared__ uint32_t wsum[kScanThreads / 64];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint4* a = reinterpret_cast<uint4*>(d.cstart + (uint64_t)b * kScanBlock) + threadIdx.x;
+  const uint4 v = *a;
+  const uint32_t t = v.x + v.y + v.z + v.w;
+  uint32_t x = t;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= (uint32_t)off) x += y;
   }
-  if (i < n) a[i] = s[threadIdx.x] - v;
-  if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = s[threadIdx.x];
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  for (uint32_t k = 0; k < wv; k++) x += wsum[k];
+  const uint32_t e = x - t;   // exclusive prefix of the thread's first key
+  *a = make_uint4(e, e + v.x, e + v.x + v.y, e + v.x + v.y + v.z);
+  if (threadIdx.x == kScanThreads - 1) d.scan_tmp[b] = x;
 }
-// Add-back: each block sums the totals of the blocks before it itself (exact
-// integer sums, L2-resident), so no separate pass scans the block totals;
-// the last block writes the grand total to a[n].
-__global__ __launch_bounds__(kScanBlock) void k_scan3(const DevCtl* ctl, uint32_t* a, uint32_t n, const uint32_t* sums,
-                                                      uint32_t* cur) {
-  if (ctl->halt || !ctl->any_rebuild) return;
-  __shared__ uint32_t red[kScanBlock / 64];
+// Add-back: each block sums the totals of its job's blocks before it itself
+// (exact integer sums, L2-resident); the job's last block records the job's
+// keyed node count.
+__global__ __launch_bounds__(kScanThreads) void k_scan3(Dev d) {
+  if (d.ctl->halt || !d.ctl->any_rebuild) return;
+  const uint32_t b = blockIdx.x;
+  const uint32_t job = d.part[ld_uniform(d.pid, b * 256)].inst;
+  if (!rebuilds_node(d, job)) return;
+  const DevInst& I = d.inst[job];
+  const uint32_t b0 = I.gtb / kScanBlock;
+  __shared__ uint32_t red[kScanThreads / 64];
   uint32_t part = 0;
-  for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanBlock) part += sums[j];
+  for (uint32_t j = b0 + threadIdx.x; j < b; j += kScanThreads) part += d.scan_tmp[j];
   for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
   __syncthreads();
   uint32_t prefix = 0;
-  for (int w = 0; w < kScanBlock / 64; w++) prefix += red[w];
-  uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
-  if (i < n) {
-    uint32_t v = a[i] + prefix;
-    a[i] = v;
-    cur[i] = v;
-  }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) a[n] = prefix + sums[blockIdx.x];
+  for (int w = 0; w < kScanThreads / 64; w++) prefix += red[w];
+  const uint32_t o = prefix + (uint32_t)I.node0;
+  uint4* a = reinterpret_cast<uint4*>(d.cstart + (uint64_t)b * kScanBlock) + threadIdx.x;
+  uint4 v = *a;
+  v.x += o;
+  v.y += o;
+  v.z += o;
+  v.w += o;
+  *a = v;
+  reinterpret_cast<uint4*>(d.ccur + (uint64_t)b * kScanBlock)[threadIdx.x] = v;
+  if (threadIdx.x == 0 && b == (uint32_t)((I.node1 - 1) / 256)) d.jkeyed[job] = prefix + d.scan_tmp[b];
 }
 
+// Counting sort of the keyed nodes into cell order (csorted, and cps: their
+// CurrPos with the node index in the w word), plus the per-node work of a
+// rebuild that needs no search: the displacement the list is built at (the
+// reference of the skin test), and an empty list for the nodes that cannot
+// have a candidate -- with nl_bdry_only the nodes outside the boundary list,
+// and the nodes of a particle with no self-contact and no other particle
+// within reach (bit 3 of pactive, pair_test).
 __global__ __launch_bounds__(256) void k_scatter(Dev d) {
   if (halted(d) || !d.ctl->any_rebuild) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
+  const uint32_t p = d.pid[g];
+  const DevPart& P = d.part[p];
   // only the nodes of rebuilding jobs hold a key of this step
-  if (!rebuilds_node(d, d.part[d.pid[g]].inst)) return;
-  uint32_t key = d.ckey[g];
-  if (key == kNoKey) return;
-  uint32_t slot = atomicAdd(&d.ccur[key], 1u);
-  d.csorted[slot] = (uint32_t)g;
-  d.cps[slot] = d.cpos[g];
+  if (!rebuilds_node(d, P.inst)) return;
+  const DevInst& I = d.inst[P.inst];
+  const double4 ui = disp_of(d, g);
+  d.UBXY[g] = make_double2(ui.x, ui.y);
+  d.UBZ[g] = ui.z;
+  const uint32_t key = d.ckey[g];
+  if (key != kNoKey) {
+    const uint32_t slot = atomicAdd(&d.ccur[key], 1u);
+    d.csorted[slot] = (uint32_t)g;
+    double4 cp = d.cpos[g];
+    cp.w = __longlong_as_double((long long)g);
+    d.cps[slot] = cp;
+  }
+  if (key == kNoKey || (!self_active(P, I) && !(d.pactive[p] & 8u))) d.ccnt[g] = 0;
 }
 
 // Candidate list of node g: the nodes j != g of its job within rb, same
@@ -560,47 +607,21 @@ __global__ __launch_bounds__(256) void k_scatter(Dev d) {
 // accumulation order (same particle first by node index, then the others by
 // global node index = particle order, then node order).
 //
-// Two forms.  Without nl_bdry_only a list entry, (other particle) << 31 |
-// node index, is its own order key (particle order is node order), and the
-// list is kept in registers: kMaxCand words, ascending, empty slots all
-// ones; a candidate is passed through every slot with a compare-exchange
+// Two forms.  Without nl_bdry_only (k_build) a list entry, (other particle)
+// << 31 | node index, is its own order key (particle order is node order),
+// and the list is kept in registers: kMaxCand words, ascending, empty slots
+// all ones; a candidate is passed through every slot with a compare-exchange
 // (static indices, no private memory), dropped if equal to one -- the
-// duplicate a hash bucket met twice gives -- and the pass stops once the
-// value carried is empty in every lane.  In crushed regions lists fill up,
-// and an insertion sort in private memory would be a long chain of dependent
-// memory round trips.  With nl_bdry_only the order is the boundary lists'
-// (k_build_bdry).
+// duplicate a wrapped grid axis of fewer than three cells gives -- and the
+// pass stops once the value carried is empty in every lane.  In crushed
+// regions lists fill up, and an insertion sort in private memory would be a
+// long chain of dependent memory round trips.  With nl_bdry_only the order is
+// the boundary lists' (k_build_bdry).
 //
 // A candidate's distance is taken between the CurrPos of this step that
-// k_cellkey stored (cpos, and cps in cell-list order, read sequentially with
-// the bucket's entries): it differs from the reference's (u_j - u_i) +
-// (X_j - X_i) only by rounding, far inside the margins of rb.  A node whose
-// particle has no self-contact and no other particle near it (bit 3 of
-// pactive, pair_test) gets an empty list without walking the cells.
-template <bool BDRY>
-__device__ __forceinline__ bool build_head(const Dev& d, uint64_t g, uint32_t& p, double4& ci, bool& selfc,
-                                           int64_t* c) {
-  p = d.pid[g];
-  const DevPart& P = d.part[p];
-  const DevInst& I = d.inst[P.inst];
-  if (!rebuilds_node(d, P.inst) || (I.nl_bdry_only != 0) != BDRY) return false;
-  const double4 ui = disp_of(d, g);
-  d.UBXY[g] = make_double2(ui.x, ui.y);
-  d.UBZ[g] = ui.z;
-  if (BDRY && d.brank[g] == kNoKey) {   // not a boundary node: no contact
-    d.ccnt[g] = 0;
-    return false;
-  }
-  selfc = self_active(P, I);
-  if (!selfc && !(d.pactive[p] & 8u)) {   // nothing within rb
-    d.ccnt[g] = 0;
-    return false;
-  }
-  ci = d.cpos[g];
-  const double cpi[3] = {ci.x, ci.y, ci.z};
-  cell_of(cpi, I.cell, c);
-  return true;
-}
+// k_cellkey stored (cpos, and cps in cell-list order): it differs from the
+// reference's (u_j - u_i) + (X_j - X_i) only by rounding, far inside the
+// margins of rb.
 
 // A node's list is longer than the pool's capacity: the pool halts (the
 // host grows the lists and runs the step again) and the job records the
@@ -611,17 +632,36 @@ __device__ __forceinline__ void list_overflow(const Dev& d, uint32_t job, uint32
   atomicOr(&d.ctl->halt, 1u);
 }
 
+// Thread t builds the list of the node in cell-list slot t, so the lanes of a
+// wavefront hold nodes of neighbouring cells of one job's grid and walk
+// nearly the same cells.  The 27 cells around a node are 9 rows of three
+// cells adjacent in key order: each row is one run of cell-list slots,
+// [cstart[k - 1], cstart[k + 2]), unless the row wraps around its axis (then
+// three runs of one cell).
 __global__ __launch_bounds__(256) void k_build(Dev d) {
   if (halted(d) || !d.ctl->any_rebuild) return;
-  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= d.N) return;
-  uint32_t p;
-  double4 ci;
-  bool selfc;
-  int64_t c[3];
-  if (!build_head<false>(d, g, p, ci, selfc, c)) return;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.N) return;
+  // slot t of a rebuilding job's cell-ordered nodes lies in the job's node
+  // range: the slot's own node tells the job
+  const uint32_t jt = d.part[d.pid[t]].inst;
+  if (!rebuilds_node(d, jt) || t - d.inst[jt].node0 >= d.jkeyed[jt]) return;
+  const double4 ci = d.cps[t];
+  const uint64_t g = (uint64_t)__double_as_longlong(ci.w);
+  const uint32_t p = d.pid[g];
   const DevPart& P = d.part[p];
   const DevInst& I = d.inst[P.inst];
+  if (I.nl_bdry_only) return;   // k_build_bdry
+  const bool selfc = self_active(P, I);
+  if (!selfc && !(d.pactive[p] & 8u)) return;   // empty list (k_scatter)
+  const double cpi[3] = {ci.x, ci.y, ci.z};
+  uint32_t w[3];
+  cell_of(cpi, I, w);
+  const bool xrun = I.gn[0] >= 3 && w[0] >= 1 && w[0] + 1 < I.gn[0];
+  // candidates within rb (a superset filter: the contact terms apply the
+  // exact tests, and rb's margins dwarf the rounding of its square)
+  const double rb2 = I.rb * I.rb;
+  constexpr uint32_t kChunk = 2;
   constexpr uint32_t kEmpty = ~0u;
   uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * d.ccap) + (g & 63);
   // passes of kMaxCand entries: pass k keeps the kMaxCand smallest entries
@@ -629,63 +669,75 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
   uint32_t total = 0, lb = 0;
   bool more = true, first = true;
   while (more) {
-    uint32_t w[kMaxCand];
+    uint32_t wl[kMaxCand];
 #pragma unroll
-    for (int i = 0; i < kMaxCand; i++) w[i] = kEmpty;
+    for (int i = 0; i < kMaxCand; i++) wl[i] = kEmpty;
     bool over = false;
-    // each cell's bucket bounds are loaded one cell ahead, each member's index
-    // one member ahead, so their round trips overlap the work on the previous
-    uint32_t key = cell_hash(c[0] - 1, c[1] - 1, c[2] - 1, P.inst, d.T);
-    uint32_t nb0 = d.cstart[key], nb1 = d.cstart[key + 1];
-    for (int s = 0; s < 27; s++) {
-      const uint32_t b0 = nb0, b1 = nb1;
-      if (s < 26) {
-        const int t = s + 1, dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-        key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
-        nb0 = d.cstart[key];
-        nb1 = d.cstart[key + 1];
-      }
-      uint32_t nj = b0 < b1 ? d.csorted[b0] : 0u;
-      double4 ncj = b0 < b1 ? d.cps[b0] : ci;
-      for (uint32_t it = b0; it < b1; it++) {
-        const uint32_t j = nj;
-        const double4 cj = ncj;
-        if (it + 1 < b1) {
-          nj = d.csorted[it + 1];
-          ncj = d.cps[it + 1];
+    // 9 rows of three cells; a row's bounds are loaded one row ahead.  A row
+    // that wraps around the x axis is walked as three runs of one cell.
+    uint32_t wy = wrap_add(w[1], -1, I.gn[1]), wz = wrap_add(w[2], -1, I.gn[2]);
+    uint32_t row = cell_key(I, 0, wy, wz);
+    uint32_t nb0 = xrun ? d.cstart[row + w[0] - 1] : 0, nb1 = xrun ? d.cstart[row + w[0] + 2] : 0;
+    for (int r = 0; r < 9; r++) {
+      const uint32_t cur = row, rb0 = nb0, rb1 = nb1;
+      if (r < 8) {
+        wy = wrap_add(w[1], (r + 1) % 3 - 1, I.gn[1]);
+        wz = wrap_add(w[2], (r + 1) / 3 - 1, I.gn[2]);
+        row = cell_key(I, 0, wy, wz);
+        if (xrun) {
+          nb0 = d.cstart[row + w[0] - 1];
+          nb1 = d.cstart[row + w[0] + 2];
         }
-        if (j == g) continue;
-        // jobs and particles hold contiguous node ranges: membership is a
-        // range test (hash buckets are shared by every job)
-        if (j < I.node0 || j >= I.node1) continue;
-        const bool same = j >= P.node_off && j < P.node_off + P.nnodes;
-        if (same && !selfc) continue;
-        uint32_t x = (same ? 0u : kBroken) | j;
-        if (!first && x <= lb) continue;   // placed by an earlier pass
-        const double dn = nrm3(cj.x - ci.x, cj.y - ci.y, cj.z - ci.z);
-        if (!(dn <= I.rb)) continue;
+      }
+      for (int part = 0; part < (xrun ? 1 : 3); part++) {
+        uint32_t b0 = rb0, b1 = rb1;
+        if (!xrun) {
+          const uint32_t k = cur + wrap_add(w[0], part - 1, I.gn[0]);
+          b0 = d.cstart[k];
+          b1 = d.cstart[k + 1];
+        }
+        // the run's records, kChunk loads in flight at a time
+        for (uint32_t it = b0; it < b1; it += kChunk) {
+          double4 cj[kChunk];
 #pragma unroll
-        for (int i0 = 0; i0 < kMaxCand; i0 += 8) {
-          if (__builtin_amdgcn_ballot_w64(x != kEmpty) == 0) continue;   // nothing left to place
+          for (uint32_t k = 0; k < kChunk; k++) cj[k] = d.cps[it + k < b1 ? it + k : b0];
 #pragma unroll
-          for (int i = i0; i < i0 + 8; i++) {
-            const uint32_t y = w[i];
-            x = x == y ? kEmpty : x;
-            w[i] = x < y ? x : y;
-            x = x < y ? y : x;
+          for (uint32_t k = 0; k < kChunk; k++) {
+            // the candidate's list word, or kEmpty: jobs and particles hold
+            // contiguous node ranges, so membership is a range test (wrapped
+            // cells of a wide job may hold far nodes of it)
+            const uint32_t j = (uint32_t)__double_as_longlong(cj[k].w);
+            const bool same = j >= P.node_off && j < P.node_off + P.nnodes;
+            uint32_t x = (same ? 0u : kBroken) | j;
+            const double ex = cj[k].x - ci.x, ey = cj[k].y - ci.y, ez = cj[k].z - ci.z;
+            const bool take = it + k < b1 && j != g && j >= I.node0 && j < I.node1 && (selfc || !same) &&
+                              (first || x > lb) && (ex * ex + ey * ey) + ez * ez <= rb2;
+            x = take ? x : kEmpty;
+            // through the sorted registers
+#pragma unroll
+            for (int i0 = 0; i0 < kMaxCand; i0 += 8) {
+              if (__builtin_amdgcn_ballot_w64(x != kEmpty) == 0) continue;   // nothing left to place
+#pragma unroll
+              for (int i = i0; i < i0 + 8; i++) {
+                const uint32_t y = wl[i];
+                x = x == y ? kEmpty : x;
+                wl[i] = x < y ? x : y;
+                x = x < y ? y : x;
+              }
+            }
+            over |= x != kEmpty;   // a candidate fell off the end: another pass
           }
         }
-        over |= x != kEmpty;   // a candidate fell off the end: another pass
       }
     }
 #pragma unroll
     for (int i = 0; i < kMaxCand; i++)
-      if (w[i] != kEmpty) {
-        if (total < d.ccap) cl[(uint64_t)total * kSlice] = w[i];
+      if (wl[i] != kEmpty) {
+        if (total < d.ccap) cl[(uint64_t)total * kSlice] = wl[i];
         total++;
       }
     more = over;
-    lb = w[kMaxCand - 1];
+    lb = wl[kMaxCand - 1];
     first = false;
   }
   if (total > d.ccap) {
@@ -702,20 +754,27 @@ __global__ __launch_bounds__(256) void k_build_bdry(Dev d) {
   if (halted(d) || !d.ctl->any_rebuild) return;
   uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.N) return;
-  uint32_t p;
-  double4 ci;
-  bool selfc;
-  int64_t c[3];
-  if (!build_head<true>(d, g, p, ci, selfc, c)) return;
+  const uint32_t p = d.pid[g];
   const DevPart& P = d.part[p];
+  if (!rebuilds_node(d, P.inst)) return;
   const DevInst& I = d.inst[P.inst];
+  if (!I.nl_bdry_only || d.brank[g] == kNoKey) return;   // other form, or an empty list (k_scatter)
+  const bool selfc = self_active(P, I);
+  if (!selfc && !(d.pactive[p] & 8u)) return;           // empty list (k_scatter)
+  const double4 ci = d.cpos[g];
+  const double cpi[3] = {ci.x, ci.y, ci.z};
+  uint32_t w[3];
+  cell_of(cpi, I, w);
   uint32_t* cl = d.clist + (g >> 6) * (uint64_t)(kSlice * d.ccap) + (g & 63);
+  // the 27 cells' keys, each once (an axis of fewer than three cells wraps
+  // onto itself)
   uint32_t seen[27];
   int nseen = 0;
   for (int dz = -1; dz <= 1; dz++)
     for (int dy = -1; dy <= 1; dy++)
       for (int dx = -1; dx <= 1; dx++) {
-        const uint32_t key = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, P.inst, d.T);
+        const uint32_t key = cell_key(I, wrap_add(w[0], dx, I.gn[0]), wrap_add(w[1], dy, I.gn[1]),
+                                      wrap_add(w[2], dz, I.gn[2]));
         bool dup = false;
         for (int q = 0; q < nseen; q++) dup |= (seen[q] == key);
         if (!dup) seen[nseen++] = key;
@@ -1669,12 +1728,18 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d) {
     d.counters[0] += d.counters[3];
     d.counters[3] = 0ull;
   }
-  // after a rebuild, the cell table is cleared for the next one (nothing
-  // reads it after k_build); the flag itself is reset by the next step's
-  // position kernel
+  // after a rebuild, the cell blocks of the jobs that rebuilt are cleared for
+  // the next one (nothing reads them after k_build); the list and the flag
+  // are reset by the next step's position kernel.  A job's blocks are those
+  // of its node blocks, from its grid's first key.
   if (d.ctl->any_rebuild) {
-    const uint64_t n = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = q; i <= d.T; i += n) d.cstart[i] = 0;
+    const uint32_t nr = ld_uniform(&d.ctl->nrj, 0u);
+    for (uint32_t k = 0; k < nr; k++) {
+      const DevInst& J = d.inst[ld_uniform(d.rjobs, k)];
+      const uint32_t jb0 = J.gtb / kScanBlock, jb1 = (uint32_t)((J.node1 - 1) / 256);
+      for (uint32_t b = jb0 + blockIdx.x; b <= jb1; b += gridDim.x)
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kScanBlock; i += blockDim.x) d.cstart[b * kScanBlock + i] = 0;
+    }
   }
 }
 
