@@ -207,6 +207,10 @@ def batched(args):
             "results_identical_to_solo": f"{same}/{len(systems)}"}
 
 
+# wave slots of the bond kernel on MI355X: 256 CUs x 4 SIMDs x 2 waves
+WAVE_SLOTS = 256 * 4 * 2
+
+
 def serving(args):
     from peridem_amd import abi, gen, serve
     jobs = []
@@ -216,8 +220,17 @@ def serving(args):
     nodes = [x.nodes for x, _ in jobs]
     log(f"[bench] serving: {len(jobs)} jobs, {min(nodes)}-{max(nodes)} nodes, steps "
         f"{min(n for _, n in jobs)}-{max(n for _, n in jobs)}")
-    t_seq, dig_s = serve.run_sequential([x for x, _ in jobs], [n for _, n in jobs], digests=True)
-    log(f"[bench] serving: sequential service {sum(t_seq):.2f} s total")
+    dev_seq = []
+    if args.no_sequential:   # profiling the served pool alone
+        t_seq, dig_s = [float("nan")] * len(jobs), [None] * len(jobs)
+    else:
+        t_seq, dig_s = serve.run_sequential([x for x, _ in jobs], [n for _, n in jobs], digests=True,
+                                            device_ms=dev_seq)
+        log(f"[bench] serving: sequential service {sum(t_seq):.2f} s total, {sum(dev_seq) * 1e-3:.2f} s of it "
+            f"stepping on the device")
+    # how much of the GPU one job fills: its bond kernel has one wavefront per
+    # 64 node slots and the GPU holds WAVE_SLOTS of them at two per SIMD
+    fill = [x.nodes / 64.0 / WAVE_SLOTS for x, _ in jobs]
     # pool sized by a device-memory budget; a job is admitted when its footprint fits
     fps = [abi.footprint(x) for x, _ in jobs]
     budget = args.serve_pool_gib * (1 << 30)
@@ -231,7 +244,14 @@ def serving(args):
                        f"{args.serve_step_scale}, FP64, dt=1e-8",
            "jobs": len(jobs), "pool_budget_gib": args.serve_pool_gib,
            "pool_node_slots": cap["max_nodes"], "pool_bond_entries": cap["max_bond_entries"],
-           "sequential_sims_per_hour": len(jobs) / sum(t_seq) * 3600.0, "rates": []}
+           "sequential_sims_per_hour": len(jobs) / sum(t_seq) * 3600.0,
+           "sequential_device_s": sum(dev_seq) * 1e-3, "sequential_wall_s": sum(t_seq),
+           "job_nodes": [x.nodes for x, _ in jobs], "job_steps": [n for _, n in jobs],
+           "job_gpu_fill": {"definition": f"nodes / 64 / {WAVE_SLOTS} (the bond kernel's wavefronts over the "
+                                          f"wave slots of 256 CUs x 4 SIMDs x 2)",
+                            "min": min(fill), "median": float(sorted(fill)[len(fill) // 2]), "max": max(fill),
+                            "jobs_at_least_one_gpu": sum(f >= 1.0 for f in fill)},
+           "rates": []}
     for rate in args.serve_rates:
         arr = serve.poisson_arrivals(len(jobs), rate, seed=int(rate * 1000) + 1)
         srv = serve.Server(chunk=args.serve_chunk, **cap)
@@ -254,12 +274,13 @@ def serving(args):
                "sequential_sims_per_hour_same_arrivals": len(jobs) / seq_span * 3600.0,
                "peak_resident_jobs": srv.peak_live, "results_identical_to_solo": f"{same}/{len(jobs)}",
                "pool_steps": srv.steps_run, "device_s": srv.device_ms * 1e-3,
+               "device_gain": sum(dev_seq) / max(srv.device_ms, 1e-9),
                "wall_s": time.perf_counter() - t0}
         log(f"[bench] serving rate {rate}/s: p50 {rec['served']['p50_s']:.3f} s, p99 "
             f"{rec['served']['p99_s']:.3f} s (sequential FIFO p50 {rec['sequential_fifo']['p50_s']:.3f}, "
             f"p99 {rec['sequential_fifo']['p99_s']:.3f}); {rec['sims_per_hour']:.0f} sims/h; "
             f"peak {srv.peak_live} resident; {same}/{len(jobs)} identical; {srv.steps_run} pool steps, "
-            f"{srv.device_ms * 1e-3:.2f} s on the device")
+            f"{srv.device_ms * 1e-3:.2f} s on the device (alone: {sum(dev_seq) * 1e-3:.2f} s)")
         out["rates"].append(rec)
     return out
 
@@ -363,13 +384,13 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-sequential", action="store_true",
-                    help="batched leg without the one-by-one comparison runs (for profiling)")
+                    help="batched and served legs without the one-by-one comparison runs (for profiling)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tolerance", action="store_true",
                     help="skip the tolerance-arithmetic line")
     ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--serve-rates", type=lambda v: [float(x) for x in v.split(",")],
-                    default=[2.0, 8.0, 50.0])
+                    default=[1.0, 2.0, 5.0, 10.0, 20.0, 50.0])
     ap.add_argument("--serve-step-scale", type=float, default=0.05)
     ap.add_argument("--serve-pool-gib", type=float, default=1.0)
     ap.add_argument("--serve-chunk", type=int, default=256)

@@ -281,10 +281,12 @@ int pd_set_graphs(pd_ctx* ctx, int on);
  * a selected shape that is not usable falls back to shape 4.  By default the
  * first pd_step of a pool with at least 2^18 resident nodes times every
  * usable shape on the resident state (scratch copies of the force and bond
- * arrays, so nothing changes) and keeps the fastest; smaller pools use shape
- * 10 (staged), or 4 while that is not usable.  pd_autotune re-runs that with reps timed launches per shape, returning
- * the choice and (when ms != NULL) the mean time of each shape [count], -1
- * for a shape not usable on this pool; pd_set_bond_shape forces one (-1: tune
+ * arrays, so nothing changes) and keeps the fastest: every usable shape is
+ * timed once, and the shapes within 15% of the fastest three more times in
+ * interleaved rounds, each keeping its smallest time; smaller pools use shape
+ * 10 (staged), or 4 while that is not usable.  pd_autotune re-runs that with
+ * reps timed launches per timing, returning the choice and (when ms != NULL)
+ * the time of each shape [count], -1 for a shape not usable on this pool; pd_set_bond_shape forces one (-1: tune
  * at the next step); pd_get_bond_shape reports the shape launches use.
  * Shape index pd_bond_shape_count() (never tuned) is the unstaged kernel with
  * 64-bit record offsets that large pools use; it can be forced on any pool. */

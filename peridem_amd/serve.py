@@ -128,12 +128,13 @@ def run_batch(systems, nsteps, digests=False):
         pool.close()
 
 
-def run_sequential(systems, nsteps, digests=False):
+def run_sequential(systems, nsteps, digests=False, device_ms=None):
     """Each system alone through the plain interface, one after another.
 
     Returns per-job wall seconds (context creation, upload, stepping and the
     read-back of every result array) and digests (computed after each job's
-    clock stops)."""
+    clock stops).  device_ms, when a list, receives each job's device time of
+    its steps (HIP events around the pd_step call)."""
     times, out = [], []
     budgets = nsteps if isinstance(nsteps, (list, tuple)) else [nsteps] * len(systems)
     for s, n in zip(systems, budgets):
@@ -141,6 +142,8 @@ def run_sequential(systems, nsteps, digests=False):
         e = abi.gpu_engine(s)
         try:
             e.step(n)
+            if device_ms is not None:
+                device_ms.append(e.last_step_ms())
             res = read_results(e.state, e.connectivity, e.wall, len(s.particles)) if digests else None
         finally:
             e.close()
