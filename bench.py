@@ -287,7 +287,7 @@ def serving(args):
 
 def crush(args):
     """The crushing aggregate at full size: 512 mixed cubes, spheres and convex
-    polyhedra (2k-12k nodes each, h = 6e-5 m, self-contact, friction,
+    polyhedra (2k-12k nodes each, h = 1e-4 m, self-contact, friction,
     damping, wall reaction), the top wall moving down at 5 m/s until 30% strain
     of the box height, then all wall speeds set to zero
     (set_zero_wall_speed_timestep, timeloop.h:447-462) and the packing left to
@@ -344,7 +344,7 @@ def crush(args):
         e.close()
     wall_s = time.perf_counter() - t_start
     out = {"mode": "crush", "workload": "crushing aggregate, 512 cubes/spheres/convex polyhedra of 2k-12k "
-                                        "nodes placed at random (bounding spheres 35% of the box), h=1e-4 m, "
+                                        "nodes (uniform) placed at random (bounding spheres 35% of the box), h=1e-4 m, "
                                         "delta=3h, self-contact, contact stiffness of 2e9 Pa, top wall -5 m/s to "
                                         "30% strain then at rest, FP64, dt=1e-8",
            "nodes": s.nodes, "directed_bonds": s.bonds, "box_height_m": H0, "steps_to_30pct": n30,

@@ -366,7 +366,8 @@ def _rsa_centres(radii, frac, rng, margin, gap):
 
 def crushing_aggregate(nparticles=512, nodes_range=(2000, 12000), seed=3, E=72e9, nu=0.25,
                        rho=2440.0, Gc=135.0, h=1e-4, top_speed=-5.0, packing=0.35, **sys_kw):
-    """Mixed cubes, spheres and convex polyhedra placed at random without
+    """Mixed cubes, spheres and convex polyhedra of node counts drawn uniformly
+    from nodes_range (512 of 2k-12k nodes: ~3.6M nodes) placed at random without
     overlap (their bounding spheres fill `packing` of the box), top wall
     moving down.
 
@@ -379,7 +380,7 @@ def crushing_aggregate(nparticles=512, nodes_range=(2000, 12000), seed=3, E=72e9
     parts = []
     sizes = []
     for k in range(nparticles):
-        target = float(np.exp(rng.uniform(np.log(nodes_range[0]), np.log(nodes_range[1]))))
+        target = float(rng.uniform(nodes_range[0], nodes_range[1]))
         kind = rng.integers(0, 3)
         if kind == 0:
             half = 0.5 * target ** (1 / 3) - 0.5

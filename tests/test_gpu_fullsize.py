@@ -13,7 +13,7 @@ properties that hold at any size, plus bitwise parity of a sample:
   oracle stepping those two spheres alone, bit for bit (they are far from
   the others and from the walls for these few steps).
 
-The crushing aggregate (2.85M nodes) is checked in dense contact by teacher
+The crushing aggregate (3.56M nodes) is checked in dense contact by teacher
 forcing: the oracle is built from the GPU's state and both take one step.
 """
 import time
@@ -113,7 +113,7 @@ def test_two_sphere_sample_matches_oracle_bitwise(oracle_lib):
 
 def test_crushing_aggregate_full_size_sample_matches_oracle(oracle_lib):
     """The crushing-aggregate configuration at full size (512 cubes, spheres
-    and convex polyhedra of 2k-12k nodes, 2.85M nodes, 256M directed bonds,
+    and convex polyhedra of 2k-12k nodes, 3.56M nodes, 324M directed bonds,
     self-contact, walls with the top one moving, wall reaction on), every
     node with a seeded velocity so every bond carries stretch.  The particles
     are placed with their bounding spheres at least 1.5 R_c apart and clear
@@ -128,7 +128,7 @@ def test_crushing_aggregate_full_size_sample_matches_oracle(oracle_lib):
         p.vel = rng.uniform(-1.0, 1.0, size=(p.n, 3))
     print(f"[fullsize] crushing aggregate: {len(s.particles)} particles, {s.nodes} nodes, "
           f"{s.bonds} directed bonds", flush=True)
-    assert s.nodes == 2852640 and s.bonds == 256481230
+    assert s.nodes == 3563127 and s.bonds == 324108448
     sample = gen.System(**{**s.__dict__, "particles": s.particles[:2]})
     g = abi.gpu_engine(s)
     o = oracle_lib.engine(sample)
@@ -175,7 +175,7 @@ def state_as_system(s, g, first_step):
 
 @pytest.mark.parametrize("checkpoints", [(DENSE_STEPS_A, DENSE_STEPS_B)])
 def test_crushing_aggregate_dense_contact_teacher_forced(oracle_lib, checkpoints):
-    """The full crushing aggregate (512 particles, 2.85M nodes, 256M directed
+    """The full crushing aggregate (512 particles, 3.56M nodes, 324M directed
     bonds) stepped on the GPU while the top wall drives the packing into
     contact; at each checkpoint the oracle is built from the GPU's state and
     both take one step, which must agree bit for bit on every node, bond,

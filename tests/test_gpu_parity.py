@@ -318,7 +318,7 @@ def test_crushing_aggregate_force_displacement_trace(oracle_lib):
     position and reaction), the progressive loss of bonds and every particle's
     state match the oracle bit for bit."""
     s = gen.crushing_aggregate(nparticles=8, nodes_range=(150, 500), h=6e-5, top_speed=-60.0,
-                               seed=5, dt=4e-9, packing=0.5)
+                               seed=13, dt=4e-9, packing=0.5)
     cg, co = run_pair(s, list(range(100, 801, 100)), oracle_lib, "crushing", reaction=True)
     broken = 1 - cg["alive_bonds"] / cg["total_bonds"]
     print(f"[parity] crushing: broken fraction {broken:.3f} after 800 steps", flush=True)
