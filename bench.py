@@ -102,7 +102,7 @@ def shape_name(k, nodes, tol=False):
     node records with 32-bit offsets; tol: the tolerance-arithmetic form)."""
     from peridem_amd import abi
     u, pipe, staged = abi.bond_shapes()[k]
-    t = ", true" if tol else ""
+    t = ", true" if tol else ", false"
     if staged:
         return f"pd::k_bond_lds<{u}, {pipe}{t}>"
     off32 = "true" if nodes < (1 << 27) else "false"

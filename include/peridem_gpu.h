@@ -294,8 +294,9 @@ int pd_set_graphs(pd_ctx* ctx, int on);
  * IEEE sqrt and division).  PD_ARITH_BITWISE (default): correctly rounded
  * square roots and divisions in the reference's operation order, results
  * bitwise the reference's.  PD_ARITH_TOLERANCE: |e| and |xi| come from the
- * hardware reciprocal square root refined by one Newton-Raphson step and the
- * divisions become multiplications by those reciprocals, so every stretch is
+ * hardware reciprocal square root refined by one Newton-Raphson step, the
+ * divisions become multiplications by those reciprocals and the squared norms
+ * and the row sums are contracted into fused multiply-adds, so every stretch is
  * within ~1e-14 (absolute) of the bitwise one: per step, each node's force
  * density agrees with the reference's to 1e-12 x c0 sum_j V_j over its row
  * (a stretch error of 1e-12), and the broken-bond set is exact for every

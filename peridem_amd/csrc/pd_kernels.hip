@@ -1284,7 +1284,9 @@ struct LdsRows {
 // y <- y + y (1/2 - (x/2) y^2), each taken with two fused multiply-adds; one
 // step leaves a relative error below 2^-47.  |e| = |e|^2 y_e and |xi| =
 // |xi|^2 y_xi then carry that error, so the stretch (|e| - |xi|) y_xi is
-// within ~1e-14 (absolute) of the correctly rounded one.
+// within ~1e-14 (absolute) of the correctly rounded one.  The squared norms,
+// the stretch's difference and the accumulation of the terms are fused
+// multiply-adds.
 constexpr int kTolNewton = 1;
 __device__ __forceinline__ double rsq_refined(double x) {
   double y = __builtin_amdgcn_rsq(x);
@@ -1323,8 +1325,11 @@ __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint3
     ev[u][0] = (uj[u].x - ui.x) + xv[u][0];
     ev[u][1] = (uj[u].y - ui.y) + xv[u][1];
     ev[u][2] = (uj[u].z - ui.z) + xv[u][2];
-    const double a2 = (ev[u][0] * ev[u][0] + ev[u][1] * ev[u][1]) + ev[u][2] * ev[u][2];
-    const double b2 = (xv[u][0] * xv[u][0] + xv[u][1] * xv[u][1]) + xv[u][2] * xv[u][2];
+    // the tolerance arithmetic contracts the squared norms into fused multiply-adds
+    const double a2 = TOL ? __builtin_fma(ev[u][2], ev[u][2], __builtin_fma(ev[u][1], ev[u][1], ev[u][0] * ev[u][0]))
+                          : (ev[u][0] * ev[u][0] + ev[u][1] * ev[u][1]) + ev[u][2] * ev[u][2];
+    const double b2 = TOL ? __builtin_fma(xv[u][2], xv[u][2], __builtin_fma(xv[u][1], xv[u][1], xv[u][0] * xv[u][0]))
+                          : (xv[u][0] * xv[u][0] + xv[u][1] * xv[u][1]) + xv[u][2] * xv[u][2];
     n2[u] = with_hi(a2, alive[u] ? hi_word(a2) : kOneHi);
     l2[u] = with_hi(b2, alive[u] ? hi_word(b2) : kOneHi);
     over = max(over, hi_word(n2[u]) - kN2Lo);
@@ -1332,14 +1337,19 @@ __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint3
   double str[U], t[U][3];
   if (__all(fast & (over < kN2Span))) {
     if (TOL) {
+      // |e| |xi| and the coefficient from the refined reciprocal square roots;
+      // each term is added to the sum in one fused multiply-add
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const double re = rsq_refined(n2[u]), rl = rsq_refined(l2[u]);
-        str[u] = (n2[u] * re - l2[u] * rl) * rl;
+        str[u] = __builtin_fma(n2[u], re, -(l2[u] * rl)) * rl;
         const double c = ((cnot * str[u]) * xj[u].w) * re;
-#pragma unroll
-        for (int q = 0; q < 3; q++) t[u][q] = c * ev[u][q];
+        f0 = __builtin_fma(c, ev[u][0], f0);
+        f1 = __builtin_fma(c, ev[u][1], f1);
+        f2 = __builtin_fma(c, ev[u][2], f2);
+        if (alive[u] & can_break & (str[u] > snot)) S.broke(k + u, e[u]);
       }
+      return;
     } else {
 #pragma unroll
       for (int u = 0; u < U; u++) {
