@@ -1335,21 +1335,17 @@ __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint3
     over = max(over, hi_word(n2[u]) - kN2Lo);
   }
   double str[U], t[U][3];
+  bool fused = false;   // t[u][0] holds the coefficient, added as fused multiply-adds
   if (__all(fast & (over < kN2Span))) {
     if (TOL) {
-      // |e| |xi| and the coefficient from the refined reciprocal square roots;
-      // each term is added to the sum in one fused multiply-add
+      // |e| |xi| and the coefficient from the refined reciprocal square roots
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const double re = rsq_refined(n2[u]), rl = rsq_refined(l2[u]);
         str[u] = __builtin_fma(n2[u], re, -(l2[u] * rl)) * rl;
-        const double c = ((cnot * str[u]) * xj[u].w) * re;
-        f0 = __builtin_fma(c, ev[u][0], f0);
-        f1 = __builtin_fma(c, ev[u][1], f1);
-        f2 = __builtin_fma(c, ev[u][2], f2);
-        if (alive[u] & can_break & (str[u] > snot)) S.broke(k + u, e[u]);
+        t[u][0] = ((cnot * str[u]) * xj[u].w) * re;
       }
-      return;
+      fused = true;
     } else {
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -1376,9 +1372,15 @@ __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint3
   }
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    f0 += t[u][0];
-    f1 += t[u][1];
-    f2 += t[u][2];
+    if (TOL && fused) {
+      f0 = __builtin_fma(t[u][0], ev[u][0], f0);
+      f1 = __builtin_fma(t[u][0], ev[u][1], f1);
+      f2 = __builtin_fma(t[u][0], ev[u][2], f2);
+    } else {
+      f0 += t[u][0];
+      f1 += t[u][1];
+      f2 += t[u][2];
+    }
     if (alive[u] & can_break & (str[u] > snot)) S.broke(k + u, e[u]);
   }
 }
@@ -1526,8 +1528,9 @@ __global__ __launch_bounds__(256, 2) void k_bond_lds(Dev d) {
     if (blk < d.S / kBlockSlices) cnt = min(d.binfo[blk].x, kLdsCap);
     const uint32_t* tab = d.btab + blk * (uint64_t)kLdsCap;
     // kStage table entries per thread in flight: their index loads go out
-    // together, then their gathers (a clamped index rereads entry i0)
-    constexpr uint32_t kStage = 4;
+    // together, then their gathers (a clamped index rereads entry i0); eight
+    // cover a table in one round
+    constexpr uint32_t kStage = 8;
     for (uint32_t i0 = threadIdx.x; i0 < cnt; i0 += kStage * blockDim.x) {
       uint32_t id[kStage];
       double2 xy[kStage], zv[kStage];
