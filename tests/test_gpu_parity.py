@@ -15,11 +15,13 @@ from peridem_amd import abi, gen
 pytestmark = pytest.mark.gpu
 
 
-def run_pair(system, checkpoints, oracle_lib, label, reaction=False):
+def run_pair(system, checkpoints, oracle_lib, label, reaction=False, prune=False):
+    """prune: the oracle's cell-grid contact candidates, bitwise its all-pairs
+    loops (test_pruned_contact_is_bitwise_the_all_pairs_loop), for long runs."""
     print(f"[parity] {label}: nodes={system.nodes} bonds={system.bonds} checkpoints={checkpoints}",
           flush=True)
     g = abi.gpu_engine(system)
-    o = oracle_lib.engine(system)
+    o = oracle_lib.engine(system, prune=prune)
     done = 0
     try:
         for c in checkpoints:
@@ -47,10 +49,10 @@ def test_free_flight_two_spheres(oracle_lib):
 
 def test_config0_two_sphere_impact(oracle_lib):
     # the reference's small config: two R=1e-3 spheres, h=1e-4, 20 m/s, fixed B;
-    # contact begins near step 100
+    # contact begins near step 100; all 2000 steps of the configuration
     s = gen.two_sphere_impact()
-    cg, co = run_pair(s, [1, 120, 300], oracle_lib, "config0")
-    assert cg["pair_computations"] > 0
+    cg, co = run_pair(s, [1, 120, 300, 1000, 2000], oracle_lib, "config0", prune=True)
+    assert cg["pair_computations"] > 0 and cg["steps"] == 2000
 
 
 def test_small_impact_breaks_bonds_and_walls(oracle_lib):
