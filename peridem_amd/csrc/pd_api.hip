@@ -165,6 +165,10 @@ struct pd_ctx {
   uint32_t lds_bad = 0;         // resident jobs whose block tables exceed kLdsCap
   uint64_t tuned_nodes = 0;     // resident nodes when the shape was tuned
   double tune_ms[kBondShapes] = {};
+  uint32_t* conn_buf = nullptr;   // connectivity read-back: pairs of one particle, then their count
+  uint64_t conn_cap = 0;          // in pairs
+  char* stage = nullptr;          // pinned host staging of read-backs
+  size_t stage_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_par = 0;            // displacement buffer the captured graph starts from
@@ -1941,6 +1945,28 @@ int pd_reset_partzero_y(pd_ctx* c, double wheel_rad) {
   return edit_inst(c, job, [](DevInst& I) { I.dirty = 1; });
 }
 
+// Pinned host staging for read-backs, grown on demand: device-to-host copies
+// into pinned memory run at the link's rate without the runtime's bounce
+// buffers, and all of a call's copies go out before one wait.
+static int stage_reserve(pd_ctx* c, size_t bytes) {
+  if (bytes <= c->stage_cap) return PD_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->stage) {
+    HIPCHK(c, hipHostFree(c->stage));
+    c->stage = nullptr;
+    c->stage_cap = 0;
+  }
+  const size_t cap = std::max<size_t>(bytes, 2 * c->stage_cap);
+  const hipError_t e = hipHostMalloc((void**)&c->stage, cap, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    c->stage = nullptr;
+    return fail(c, e == hipErrorOutOfMemory ? PD_E_NOMEM : PD_E_HIP, "hipHostMalloc(%zu) failed: %s", cap,
+                hipGetErrorString(e));
+  }
+  c->stage_cap = cap;
+  return PD_OK;
+}
+
 int pd_job_get_state(pd_ctx* c, uint32_t job, uint32_t pi, double* disp, double* currpos, double* vel,
                      double* acc, double* force) {
   if (!c) return PD_E_ARG;
@@ -1949,16 +1975,30 @@ int pd_job_get_state(pd_ctx* c, uint32_t job, uint32_t pi, double* disp, double*
   if (rc) return rc;
   const HostPart& p = *Hp;
   const Job& J = c->jobs[job];
-  if (disp || currpos) {
+  // every requested array goes to the pinned stage in one stream of copies:
+  // U and X (double4 per node) for disp / CurrPos, then V, A, F (3 per node)
+  const bool pos = disp || currpos;
+  const size_t n4 = (size_t)p.n * sizeof(double4), n3 = 3ull * p.n * sizeof(double);
+  if ((rc = stage_reserve(c, (pos ? 2 * n4 : 0) + 3 * n3))) return rc;
+  const double4* U = (const double4*)c->stage;
+  const double4* X = (const double4*)(c->stage + n4);
+  char* rest = c->stage + (pos ? 2 * n4 : 0);
+  if (pos) {
     // the displacement lives in the split copies; U is its transfer form
     hipLaunchKernelGGL(k_join, dim3(grid_of(p.n, 256)), dim3(256), 0, c->stream, c->d, p.off, (uint64_t)p.n);
     HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->stage, c->d.U + p.off, n4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->stage + n4, c->d.X + p.off, n4, hipMemcpyDeviceToHost, c->stream));
   }
+  double* const dst[3] = {vel, acc, force};
+  const double* const src[3] = {c->d.V, c->d.A, c->d.F};
+  for (int a = 0; a < 3; a++)
+    if (dst[a])
+      HIPCHK(c, hipMemcpyAsync(rest + a * n3, src[a] + 3 * p.off, n3, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (disp || currpos) {
-    std::vector<double4> U(p.n), X(p.n);
-    HIPCHK(c, hipMemcpy(U.data(), c->d.U + p.off, p.n * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(X.data(), c->d.X + p.off, p.n * sizeof(double4), hipMemcpyDeviceToHost));
+  for (int a = 0; a < 3; a++)
+    if (dst[a]) memcpy(dst[a], rest + a * n3, n3);
+  if (pos) {
     for (uint32_t i = 0; i < p.n; i++) {
       const double u[3] = {U[i].x, U[i].y, U[i].z};
       const double x[3] = {X[i].x, X[i].y, X[i].z};
@@ -1982,14 +2022,6 @@ int pd_job_get_state(pd_ctx* c, uint32_t job, uint32_t pi, double* disp, double*
       }
     }
   }
-  auto down3 = [&](double* dst, const double* src) -> int {
-    if (!dst) return PD_OK;
-    HIPCHK(c, hipMemcpy(dst, src + 3 * p.off, 3ull * p.n * sizeof(double), hipMemcpyDeviceToHost));
-    return PD_OK;
-  };
-  if ((rc = down3(vel, c->d.V))) return rc;
-  if ((rc = down3(acc, c->d.A))) return rc;
-  if ((rc = down3(force, c->d.F))) return rc;
   return PD_OK;
 }
 
@@ -2045,45 +2077,43 @@ int pd_job_get_connectivity(pd_ctx* c, uint32_t job, uint32_t pi, uint32_t* pair
   int rc = locate(c, job, pi, &Hp);
   if (rc) return rc;
   const HostPart& p = *Hp;
-  // the slices of the particle's node blocks and their bond entries (a node's
-  // row sits at the slot node_lane gives it in its block)
-  const uint64_t bn = (uint64_t)kBlockSlices * kSlice;
-  const uint64_t s0 = p.off / bn * kBlockSlices, s1 = (p.off + p.n + bn - 1) / bn * kBlockSlices;
-  std::vector<uint64_t> soff(s1 - s0);
-  std::vector<uint32_t> swid(s1 - s0), deg(p.n);
-  std::vector<uint8_t> nl(p.n);
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(soff.data(), c->d.soff + s0, soff.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(swid.data(), c->d.swid + s0, swid.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(deg.data(), c->d.deg + p.off, p.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(nl.data(), c->d.node_lane + p.off, p.n, hipMemcpyDeviceToHost));
-  const uint64_t lo = soff.front(), hi = soff.back() + (uint64_t)swid.back() * kSlice;
-  std::vector<uint32_t> ent(hi - lo);
-  if (!ent.empty())
-    HIPCHK(c, hipMemcpy(ent.data(), c->d.bond + lo, ent.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  // NbdArr2conn (timeloop.h:20-44): pairs (i, q) with i < q, in row order,
-  // written straight into the caller's buffer while they fit (a buffer of
-  // the particle's initial bond count always does: bonds only break)
-  uint64_t n = 0;
-  for (uint32_t i = 0; i < p.n; i++) {
-    const uint64_t g = p.off + i, t = (g & ~(uint64_t)255) + nl[i], s = t / kSlice, l = t % kSlice;
-    for (uint32_t k = 0; k < deg[i]; k++) {
-      const uint32_t e = ent[soff[s - s0] - lo + (uint64_t)k * kSlice + l];
-      if (e & kBroken) continue;
-      const uint32_t q = e - (uint32_t)p.off;
-      if (i < q) {
-        if (pairs && n < cap) {
-          pairs[2 * n] = i;
-          pairs[2 * n + 1] = q;
-        }
-        n++;
-      }
+  // NbdArr2conn (timeloop.h:20-44) on the device: pairs (i, q) with i < q in
+  // row order, compacted into a buffer of the context (grown to the largest
+  // particle: its entries as pairs, then per node the pair offsets and the
+  // total) and copied through the pinned stage into the caller's buffer
+  const uint64_t need = p.nbonds + ((uint64_t)p.n + 2) / 2 + 1;   // in pairs
+  if (need > c->conn_cap) {
+    if (c->conn_buf) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->conn_buf);
+      if (it != c->allocs.end()) c->allocs.erase(it);
+      HIPCHK(c, hipFree(c->conn_buf));
+      c->conn_buf = nullptr;
+      c->conn_cap = 0;
     }
+    if ((rc = dalloc(c, &c->conn_buf, 2 * need))) return rc;
+    c->conn_cap = need;
   }
+  uint32_t* offs = c->conn_buf + 2 * std::max<uint64_t>(p.nbonds, 1);   // p.n + 1 entries
+  hipLaunchKernelGGL(k_conn_count, dim3(grid_of(p.n, 256)), dim3(256), 0, c->stream, c->d, p.off, p.n, offs);
+  hipLaunchKernelGGL(k_conn_scan, dim3(1), dim3(kConnThreads), 0, c->stream, offs, p.n);
+  hipLaunchKernelGGL(k_conn_write, dim3(grid_of(p.n, 256)), dim3(256), 0, c->stream, c->d, p.off, p.n, offs,
+                     c->conn_buf);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = stage_reserve(c, 2 * need * sizeof(uint32_t)))) return rc;
+  uint32_t* hs = (uint32_t*)c->stage;
+  HIPCHK(c, hipMemcpyAsync(hs, offs + p.n, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const uint32_t n = hs[0];
   *m = n;
   if (pairs && cap < n)
     return fail(c, PD_E_ARG, "connectivity buffer holds %llu pairs, %llu needed", (unsigned long long)cap,
                 (unsigned long long)n);
+  if (pairs && n) {
+    HIPCHK(c, hipMemcpyAsync(hs, c->conn_buf, 2ull * n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    memcpy(pairs, hs, 2ull * n * sizeof(uint32_t));
+  }
   return PD_OK;
 }
 
@@ -2269,6 +2299,7 @@ void pd_destroy(pd_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)drop_graph(c);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stage) (void)hipHostFree(c->stage);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (int s = 0; s < kProfRing; s++)

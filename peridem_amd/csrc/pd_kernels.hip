@@ -1846,6 +1846,68 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n, uint3
 
 
 // alive directed bonds (for counters and read-back)
+// NbdArr2conn (timeloop.h:20-44) of one particle (nodes [off, off + n)): its
+// intact bonds as pairs (i, q), i < q, particle-relative, in row order -- node
+// by node, each row in its own order.  k_conn_count counts each node's pairs,
+// k_conn_scan turns the counts into offsets (one workgroup; the total goes to
+// cnt[n]), k_conn_write writes each node's pairs at its offset.
+constexpr uint32_t kConnThreads = 1024;
+__device__ __forceinline__ uint32_t conn_entry(const Dev& d, uint64_t g, uint32_t k) {
+  const uint64_t t = (g & ~255ull) + d.node_lane[g];   // the node's slot
+  return d.bond[d.soff[t >> 6] + (uint64_t)k * kSlice + (t & 63)];
+}
+__global__ __launch_bounds__(256) void k_conn_count(Dev d, uint64_t off, uint32_t n, uint32_t* cnt) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t deg = d.deg[off + i];
+  uint32_t c = 0;
+  for (uint32_t k = 0; k < deg; k++) {
+    const uint32_t e = conn_entry(d, off + i, k);
+    c += !(e & kBroken) && e - (uint32_t)off > i;
+  }
+  cnt[i] = c;
+}
+__global__ __launch_bounds__(kConnThreads) void k_conn_scan(uint32_t* cnt, uint32_t n) {
+  __shared__ uint32_t wsum[kConnThreads / 64];
+  const uint32_t per = (n + kConnThreads - 1) / kConnThreads;
+  const uint32_t i0 = min(n, threadIdx.x * per), i1 = min(n, i0 + per);
+  uint32_t c = 0;
+  for (uint32_t i = i0; i < i1; i++) c += cnt[i];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t x = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  for (uint32_t w = 0; w < wv; w++) x += wsum[w];
+  uint32_t at = x - c;
+  for (uint32_t i = i0; i < i1; i++) {
+    const uint32_t v = cnt[i];
+    cnt[i] = at;
+    at += v;
+  }
+  if (threadIdx.x == kConnThreads - 1) cnt[n] = x;
+}
+__global__ __launch_bounds__(256) void k_conn_write(Dev d, uint64_t off, uint32_t n, const uint32_t* at0,
+                                                    uint32_t* pairs) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t deg = d.deg[off + i];
+  uint64_t at = at0[i];
+  for (uint32_t k = 0; k < deg; k++) {
+    const uint32_t e = conn_entry(d, off + i, k);
+    const uint32_t q = e - (uint32_t)off;
+    if (!(e & kBroken) && q > i) {
+      pairs[2 * at] = i;
+      pairs[2 * at + 1] = q;
+      at++;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_count_alive(Dev d, uint64_t nent, unsigned long long* out) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // padding entries carry the broken bit, so they are not counted
