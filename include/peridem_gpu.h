@@ -282,7 +282,8 @@ int pd_set_graphs(pd_ctx* ctx, int on);
  * first pd_step of a pool with at least 2^18 resident nodes times every
  * usable shape on the resident state (scratch copies of the force and bond
  * arrays, so nothing changes) and keeps the fastest: every usable shape is
- * timed once, and the shapes within 15% of the fastest three more times in
+ * timed once (while staged shapes are usable, of the unstaged ones only shape
+ * 4), and the shapes within 15% of the fastest three more times in
  * interleaved rounds, each keeping its smallest time; smaller pools use shape
  * 10 (staged), or 4 while that is not usable.  pd_autotune re-runs that with
  * reps timed launches per timing, returning the choice and (when ms != NULL)
