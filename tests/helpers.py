@@ -185,3 +185,50 @@ def wheel_on_bulk(seed=31, wheel_y_h=9.0, wheel_vy=0.0, walls=False, self_contac
                   wall_lrtb=(-5 * h, -4 * h, -4 * h, 15 * h, 14 * h, 4 * h),
                   wall_speed=(0.0, 0.0, 0.0, 0.0, -2.0, 0.0))
     return gen.System(particles=[W, B, T], **kw)
+
+
+def unbonded(pos, like, vel=None):
+    """A particle of the given nodes with no bonds (empty Connectivity), the
+    material of `like`."""
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    h3 = float(like.vol[0])
+    p = gen.Particle(pos=pos, vol=np.full(pos.shape[0], h3), conn=np.zeros((0, 2), dtype=np.uint32),
+                     delta=like.delta, rho=like.rho, cnot=like.cnot, snot=like.snot)
+    p.vel = np.zeros_like(pos) if vel is None else np.asarray(vel, dtype=np.float64).reshape(-1, 3)
+    return p
+
+
+def degenerate_particles(seed=41, walls=True):
+    """Particles at the edges of the input space, all in contact within a few
+    hundred steps: a bonded sphere (0); a single node flying into it (1);
+    three unbonded nodes 2h apart catching it from behind (2); a 27-node cube
+    with an empty Connectivity whose nodes collide with each other and with
+    the top wall (3, self-contact among unbonded nodes); a bonded sphere one
+    of whose nodes sits 1.6h off its surface with its bonds removed (4: a
+    row of length zero among full ones)."""
+    h = 1e-4
+    rng = np.random.default_rng(seed)
+    A = gen.make_particle(gen.lattice_sphere(4.0), h, np.array([0.0, 0.0, 0.0]), rng)
+    A.vel = np.zeros((A.n, 3))
+    A.vel[:, 0] = 20.0
+    B = unbonded([[5.2 * h, 0.2 * h, 0.1 * h]], A, vel=[[-10.0, 0.0, 0.0]])
+    C = unbonded([[-5.2 * h, 0.0, 0.0], [-5.2 * h, 2 * h, 0.0], [-5.2 * h, 0.0, 2 * h]], A,
+                 vel=[[30.0, 0.0, 0.0]] * 3)
+    L = gen.lattice_cube(1.0)
+    D = unbonded(np.array([0.0, 9.0 * h, 0.0]) + h * (L + rng.uniform(-0.05, 0.05, size=L.shape)), A,
+                 vel=rng.uniform(-6.0, 6.0, size=(L.shape[0], 3)))
+    E = gen.make_particle(gen.lattice_sphere(3.0), h, np.array([0.0, -9.0 * h, 0.0]), rng)
+    top = int(np.argmax(E.pos[:, 2]))
+    E.pos[top, 2] += 1.6 * h
+    E.conn = E.conn[(E.conn[:, 0] != top) & (E.conn[:, 1] != top)]
+    E.vel = np.zeros((E.n, 3))
+    E.vel[top, 2] = -40.0
+    Rc = 0.9 * h
+    kw = dict(dt=1e-8, contact_rad=Rc, normal_stiffness=gen.contact_stiffness(Rc),
+              friction_coefficient=0.3, damping_ratio=0.1, enable_fracture=1, self_contact=1,
+              allow_damping=1, allow_friction=1, name="degenerate_particles")
+    if walls:
+        kw.update(allow_wall=1, compute_wall_reaction=1,
+                  wall_lrtb=(-12 * h, -14 * h, -6 * h, 12 * h, 10.6 * h, 6 * h),
+                  wall_speed=(0.0, 0.0, 0.0, 0.0, -1.0, 0.0))
+    return gen.System(particles=[A, B, C, D, E], **kw)

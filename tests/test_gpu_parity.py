@@ -9,7 +9,7 @@ evaluates every term in the reference's order with IEEE FP64 and no FMA).
 import numpy as np
 import pytest
 
-from helpers import assert_bitwise, compare_engines, mixed_system, small_impact
+from helpers import assert_bitwise, compare_engines, degenerate_particles, mixed_system, small_impact
 from peridem_amd import abi, gen
 
 pytestmark = pytest.mark.gpu
@@ -378,3 +378,13 @@ def test_wheel_reset_matches_oracle(oracle_lib, first_reset):
     finally:
         g.close()
         o.close()
+
+
+def test_degenerate_particles_match_oracle(oracle_lib):
+    """Empty Connectivities (a single-node particle, three unbonded nodes, a
+    bond-free cube whose nodes meet each other and the moving top wall) and an
+    empty row inside a bonded sphere: slices of width zero, block tables of
+    the block's own nodes only, self-contact between unbonded nodes, broken
+    bonds, wall reaction -- bitwise against the oracle."""
+    s = degenerate_particles()
+    run_pair(s, [1, 100, 300, 600, 800], oracle_lib, "degenerate particles", reaction=True)
