@@ -1845,25 +1845,34 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n, uint3
 }
 
 
-// alive directed bonds (for counters and read-back)
 // NbdArr2conn (timeloop.h:20-44) of one particle (nodes [off, off + n)): its
 // intact bonds as pairs (i, q), i < q, particle-relative, in row order -- node
 // by node, each row in its own order.  k_conn_count counts each node's pairs,
 // k_conn_scan turns the counts into offsets (one workgroup; the total goes to
-// cnt[n]), k_conn_write writes each node's pairs at its offset.
+// cnt[n]), k_conn_write writes each node's pairs at its offset.  A thread
+// walks its row kConnBatch entries at a time, their loads in flight together.
 constexpr uint32_t kConnThreads = 1024;
-__device__ __forceinline__ uint32_t conn_entry(const Dev& d, uint64_t g, uint32_t k) {
+constexpr uint32_t kConnBatch = 8;
+__device__ __forceinline__ const uint32_t* conn_row(const Dev& d, uint64_t g) {
   const uint64_t t = (g & ~255ull) + d.node_lane[g];   // the node's slot
-  return d.bond[d.soff[t >> 6] + (uint64_t)k * kSlice + (t & 63)];
+  return d.bond + d.soff[t >> 6] + (t & 63);
+}
+__device__ __forceinline__ void conn_batch(const uint32_t* row, uint32_t k0, uint32_t deg, uint32_t (&e)[kConnBatch]) {
+#pragma unroll
+  for (uint32_t u = 0; u < kConnBatch; u++)
+    e[u] = k0 + u < deg ? row[(uint64_t)(k0 + u) * kSlice] : kBroken;
 }
 __global__ __launch_bounds__(256) void k_conn_count(Dev d, uint64_t off, uint32_t n, uint32_t* cnt) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t deg = d.deg[off + i];
+  const uint32_t* row = conn_row(d, off + i);
   uint32_t c = 0;
-  for (uint32_t k = 0; k < deg; k++) {
-    const uint32_t e = conn_entry(d, off + i, k);
-    c += !(e & kBroken) && e - (uint32_t)off > i;
+  for (uint32_t k0 = 0; k0 < deg; k0 += kConnBatch) {
+    uint32_t e[kConnBatch];
+    conn_batch(row, k0, deg, e);
+#pragma unroll
+    for (uint32_t u = 0; u < kConnBatch; u++) c += !(e[u] & kBroken) && e[u] - (uint32_t)off > i;
   }
   cnt[i] = c;
 }
@@ -1896,18 +1905,24 @@ __global__ __launch_bounds__(256) void k_conn_write(Dev d, uint64_t off, uint32_
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t deg = d.deg[off + i];
+  const uint32_t* row = conn_row(d, off + i);
   uint64_t at = at0[i];
-  for (uint32_t k = 0; k < deg; k++) {
-    const uint32_t e = conn_entry(d, off + i, k);
-    const uint32_t q = e - (uint32_t)off;
-    if (!(e & kBroken) && q > i) {
-      pairs[2 * at] = i;
-      pairs[2 * at + 1] = q;
-      at++;
+  for (uint32_t k0 = 0; k0 < deg; k0 += kConnBatch) {
+    uint32_t e[kConnBatch];
+    conn_batch(row, k0, deg, e);
+#pragma unroll
+    for (uint32_t u = 0; u < kConnBatch; u++) {
+      const uint32_t q = e[u] - (uint32_t)off;
+      if (!(e[u] & kBroken) && q > i) {
+        pairs[2 * at] = i;
+        pairs[2 * at + 1] = q;
+        at++;
+      }
     }
   }
 }
 
+// alive directed bonds (for counters and read-back)
 __global__ __launch_bounds__(256) void k_count_alive(Dev d, uint64_t nent, unsigned long long* out) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // padding entries carry the broken bit, so they are not counted
