@@ -1954,13 +1954,13 @@ int pd_reset_partzero_y(pd_ctx* c, double wheel_rad) {
 // buffers, and all of a call's copies go out before one wait.
 static int stage_reserve(pd_ctx* c, size_t bytes) {
   if (bytes <= c->stage_cap) return PD_OK;
+  const size_t cap = std::max<size_t>(bytes, 2 * c->stage_cap);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->stage) {
     HIPCHK(c, hipHostFree(c->stage));
     c->stage = nullptr;
     c->stage_cap = 0;
   }
-  const size_t cap = std::max<size_t>(bytes, 2 * c->stage_cap);
   const hipError_t e = hipHostMalloc((void**)&c->stage, cap, hipHostMallocDefault);
   if (e != hipSuccess) {
     c->stage = nullptr;
