@@ -207,6 +207,32 @@ def batched(args):
             "results_identical_to_solo": f"{same}/{len(systems)}"}
 
 
+def config0_latency(args):
+    """The two-sphere impact of the reference's 3-D example at its own size
+    (R = 1e-3 m, h = 1e-4 m: 4,169 nodes per sphere, A at 20 m/s toward a
+    fixed B, contact from about step 50), all of its steps in one call: the
+    device time per step of a pool far too small to fill the GPU, where a step
+    is bound by the latency of its kernels."""
+    from peridem_amd import abi, gen
+    s = gen.two_sphere_impact()
+    e = abi.gpu_engine(s)
+    try:
+        t0 = time.perf_counter()
+        e.step(args.config0_steps)
+        wall = time.perf_counter() - t0
+        dev = e.last_step_ms()
+        c = e.counters()
+    finally:
+        e.close()
+    out = {"workload": "two spheres R=1e-3 m, h=1e-4 m (4169 nodes each), A at 20 m/s toward fixed B, dt=1e-8",
+           "nodes": s.nodes, "directed_bonds": s.bonds, "steps": args.config0_steps,
+           "device_ms_per_step": dev / args.config0_steps, "wall_s": wall,
+           "pair_computations": c["pair_computations"], "alive_bonds": c["alive_bonds"]}
+    log(f"[bench] config 0: {args.config0_steps} steps in {wall:.3f} s wall, "
+        f"{out['device_ms_per_step'] * 1e3:.1f} us device per step")
+    return out
+
+
 # wave slots of the bond kernel on MI355X: 256 CUs x 4 SIMDs x 2 waves
 WAVE_SLOTS = 256 * 4 * 2
 
@@ -305,7 +331,7 @@ def crush(args):
     log(f"[crush] {len(s.particles)} particles, {s.nodes} nodes, {s.bonds} directed bonds; box height "
         f"{H0:.4e} m; 30% strain after {n30} steps, then {args.crush_relax} steps at rest")
     e = abi.gpu_engine(s)
-    rows, t, dev_ms = [], 0, 0.0
+    rows, t, dev_ms, phase_rows = [], 0, 0.0, []
     c0 = e.counters()
     t_start = time.perf_counter()
     try:
@@ -317,8 +343,18 @@ def crush(args):
             top, bottom, done = [], [], 0
             while done < n:
                 m = min(args.crush_sample, n - done)
+                # every crush_phase_every steps, the first sample of the
+                # chunk carries a mark around every phase of each step
+                marked = args.crush_phase_every > 0 and done == 0 and t > 0 and t % args.crush_phase_every == 0
+                if marked:
+                    e.set_profiling(abi.PD_PROFILE_PHASES)
                 e.step(m)
                 dev_ms += e.last_step_ms()
+                if marked:
+                    ph = e.profile()
+                    e.set_profiling(0)
+                    phase_rows.append({"step": t, "steps": m, "pairs_per_step": None,
+                                       "ms": {k: v["ms"] / max(v["launches"], 1) for k, v in ph.items()}})
                 done += m
                 _, reac = e.wall()
                 top.append(float(reac[5][2]))
@@ -333,6 +369,8 @@ def crush(args):
                    "broken_fraction": 1.0 - c["alive_bonds"] / c["total_bonds"],
                    "pairs_per_step": (c["pair_computations"] - c0["pair_computations"]) / n,
                    "ms_per_step": dev_ms / max(t, 1), "list_capacity": c["list_capacity"]}
+            if phase_rows and phase_rows[-1]["pairs_per_step"] is None:
+                phase_rows[-1]["pairs_per_step"] = row["pairs_per_step"]
             c0 = c
             rows.append(row)
             log(f"[crush] step {t}: strain {row['strain']:.4f}, top reaction mean {row['reaction_top_z_N']:.4e} N "
@@ -350,7 +388,8 @@ def crush(args):
            "nodes": s.nodes, "directed_bonds": s.bonds, "box_height_m": H0, "steps_to_30pct": n30,
            "relax_steps": args.crush_relax, "steps": t, "wall_s": wall_s,
            "device_s": dev_ms * 1e-3, "bond_updates_per_s": s.movable_bonds() * t / (dev_ms * 1e-3),
-           "list_regrows": cnt["list_regrows"], "cell_rebuilds": cnt["cell_rebuilds"], "trace": rows}
+           "list_regrows": cnt["list_regrows"], "cell_rebuilds": cnt["cell_rebuilds"], "trace": rows,
+           "phases": phase_rows}
     print(json.dumps(out), flush=True)
     if args.crush_md:
         with open(args.crush_md, "w") as f:
@@ -362,6 +401,14 @@ def crush(args):
                         f"{r['reaction_top_z_N']:.4e} | {r['reaction_top_z_max_N']:.4e} | "
                         f"{r['reaction_bottom_z_N']:.4e} | {r['broken_fraction']:.4e} | "
                         f"{r['pairs_per_step']:.0f} |\n")
+            if phase_rows:
+                names = list(phase_rows[0]["ms"])
+                f.write("\n| step | pairs/step | " + " | ".join(f"{k} (ms)" for k in names) + " | step (ms) |\n"
+                        "|---|---|" + "---|" * len(names) + "---|\n")
+                for r in phase_rows:
+                    f.write(f"| {r['step']} | {r['pairs_per_step']:.0f} | " +
+                            " | ".join(f"{r['ms'][k]:.4f}" for k in names) +
+                            f" | {sum(r['ms'].values()):.4f} |\n")
 
 
 def main():
@@ -381,6 +428,8 @@ def main():
                     help="steps of the single-thread CPU sample (0: skip it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--config0-steps", type=int, default=2000,
+                    help="steps of the two-sphere example timed for its per-step latency (0: skip)")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-sequential", action="store_true",
@@ -410,6 +459,8 @@ def main():
                     help="steps between the wall reactions averaged over a chunk")
     ap.add_argument("--crush-relax", type=int, default=10000)
     ap.add_argument("--crush-md", default="", help="also write the trace as a markdown table here")
+    ap.add_argument("--crush-phase-every", type=int, default=32000,
+                    help="every this many steps, one sample of steps with a mark around every phase (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -547,6 +598,7 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copy probe: {args.copy_probe_mib} MiB device-to-device")
 
+    lat0 = None if args.config0_steps <= 0 else config0_latency(args)
     bat = None if args.no_batch else batched(args)
     srv = None if args.no_serve else serving(args)
 
@@ -591,6 +643,7 @@ def main():
             "frac": fi * 64 / (force_ms * 1e-3) / FP64_LANE_INSTR_PEAK},
         "tolerance_mode": tol,
         "cpu_baseline": cpu,
+        "config0_latency": lat0,
         "batched": bat,
         "serving": srv,
     }

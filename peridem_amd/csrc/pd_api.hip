@@ -1317,7 +1317,7 @@ static int launch_step(pd_ctx* c, int par, hipEvent_t* ev = nullptr) {
   hipLaunchKernelGGL(launch_shape(c).fn,
                      dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
   if ((rc = mark(3))) return rc;
-  if (any_reaction(c)) hipLaunchKernelGGL(k_reaction, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
+  if (any_reaction(c)) hipLaunchKernelGGL(k_reaction, dim3(d.ninst), dim3(64), 0, st, d);
   if ((rc = mark(4))) return rc;
   if (any_clamped(c)) hipLaunchKernelGGL(k_clamp, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if (any_vc(c)) {

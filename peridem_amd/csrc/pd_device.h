@@ -338,8 +338,80 @@ __device__ __forceinline__ bool within_interior_fext(const double* A, const doub
 }
 
 // wall_side_force_update<3> (contact.h:385-502) with segment_mass<3>
-// (contact.h:14-39).  Adds the node's terms to f; when reac != nullptr also
-// accumulates the wall reaction in the reference's order.
+// (contact.h:14-39): the terms of one node and one plane, each as the
+// reference forms it -- the force terms added to the node's force, and the
+// same terms times the node volume subtracted from the wall reaction.
+struct WallSide {
+  bool hit, fric, damp;
+  int dir;
+  double fn, ff1, ff2, fd;   // force: normal, friction on the two tangents, damping
+  double rn, rf1, rf2, rd;   // reaction terms (each subtracted)
+};
+__device__ __forceinline__ WallSide wall_side_eval(const DevInst& I, double dt, double dist, int dir,
+                                                   const double* vel, double vol, double rho, double speed) {
+  WallSide w{};
+  w.dir = dir;
+  const double Rc = I.Rc;
+  if (!(fabs(dist) <= Rc)) return w;
+  w.hit = true;
+  double dd = fabs(dist);
+  double Rd = Rc - dd;
+  double area = ((M_PI / 3) * (Rd * Rd)) * (2 * Rc - dd);
+  double q = Rc * Rc - dd * dd;
+  double Im = (M_PI / 4) * (q * q);
+  double cen = Im / area;
+  double contrib = Rc - cen;
+  double pc = contrib > 0 ? contrib : 0;
+  double Fn = (I.Kn * pc) * area;
+  double sg = dist > 0 ? 1.0 : -1.0;
+  w.fn = sg * Fn;
+  w.rn = (vol * sg) * Fn;
+  double rv[3] = {vel[0], vel[1], vel[2]};
+  rv[dir] -= speed;
+  if (I.allow_friction) {
+    int t1 = (dir + 1) % 3, t2 = (dir + 2) % 3;
+    double p1 = rv[t1], p2 = rv[t2];
+    double pn = sqrt(p1 * p1 + p2 * p2);
+    if (pn > 0) {
+      double u1 = p1 / pn, u2 = p2 / pn;
+      double ff1 = ((-I.mu) * fabs(Fn)) * u1;
+      double ff2 = ((-I.mu) * fabs(Fn)) * u2;
+      if (pn < ((I.mu * fabs(Fn)) * dt) / rho) {
+        ff1 = (((-pn) / rho) / dt) * u1;
+        ff2 = (((-pn) / rho) / dt) * u2;
+      }
+      w.fric = true;
+      w.ff1 = ff1;
+      w.ff2 = ff2;
+      w.rf1 = vol * ff1;
+      w.rf2 = vol * ff2;
+    }
+  }
+  if (I.allow_damping) {
+    double dc = ((2 * I.zeta) * sqrt(I.Kn * rho)) * sqrt(fabs(area));
+    double fd = (-dc) * rv[dir];
+    w.damp = true;
+    w.fd = fd;
+    w.rd = vol * fd;
+  }
+  return w;
+}
+// The reaction terms of one plane subtracted in the reference's order.
+__device__ __forceinline__ void wall_side_react(const WallSide& w, double* reac /* the plane's [3] */) {
+  if (!w.hit) return;
+  reac[w.dir] -= w.rn;
+  if (w.fric) {
+    reac[(w.dir + 1) % 3] -= w.rf1;
+    reac[(w.dir + 2) % 3] -= w.rf2;
+  }
+  if (w.damp) reac[w.dir] -= w.rd;
+}
+// Adds the node's terms to f; when reac != nullptr also accumulates the wall
+// reaction in the reference's order.  The arithmetic of wall_side_eval,
+// written out once more so that the force path inside the bond kernel keeps
+// its own register allocation; both are checked bit for bit against the
+// oracle (forces at every parity checkpoint, reactions where they are
+// formed).
 __device__ __forceinline__ bool wall_side(const DevInst& I, double dt, double dist, int dir,
                                           const double* vel, double vol, double rho, double* f,
                                           double speed, double* reac) {

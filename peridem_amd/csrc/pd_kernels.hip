@@ -1757,58 +1757,106 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d) {
 }
 
 // Wall reaction of one step in the reference order (particle, node, plane,
-// term), recomputed from the same device function as the force.  One thread
-// per instance walks its node range; blocks without a wall contact are
-// skipped through bflag.
-__global__ void k_reaction(Dev d) {
+// term), recomputed from the same device function as the force.  One
+// wavefront per job walks its node range 64 nodes at a time: blocks without
+// a wall contact are skipped through bflag (64 flags per load), the lanes
+// evaluate their nodes' terms together, and the terms are then subtracted
+// from the running sums one node after another in node order, so every
+// component receives its terms in the reference's order.
+// nl_bdry_only: particle by particle, each in boundary-list order
+// (contact.h:1065-1069), one lane
+__device__ void reaction_bdry_only(const Dev& d, const DevInst& I, uint64_t q) {
+  double reac[18];
+  for (int k = 0; k < 18; k++) reac[k] = 0.0;
+  for (uint64_t g = I.node0; g < I.node1;) {
+    const DevPart& P = d.part[d.pid[g]];
+    for (uint32_t r = 0; r < P.nbdry; r++) {
+      const uint32_t h = d.blist[P.boff + r];
+      if (!d.wmask[h]) continue;
+      const double4 x = d.X[h];
+      const double4 u = disp_of(d, h);
+      const double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
+      const double v[3] = {d.Vold[3 * h], d.Vold[3 * h + 1], d.Vold[3 * h + 2]};
+      double fdummy[3] = {0, 0, 0};
+      wall_node(I, I.dt, cp, v, x.w, P.rho, fdummy, reac);
+      d.wmask[h] = 0;
+    }
+    g = P.node_off + P.nnodes;
+  }
+  for (int k = 0; k < 18; k++) d.reaction[18 * q + k] = reac[k];
+}
+
+__global__ __launch_bounds__(64) void k_reaction(Dev d) {
   if (halted(d)) return;
-  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t q = blockIdx.x;
   if (q >= d.ninst) return;
   const DevInst I = d.inst[q];
   if (!live(I) || !reaction_step(d, I)) return;
-  double reac[18];
-  for (int k = 0; k < 18; k++) reac[k] = 0.0;
-  const uint64_t g0 = I.node0, g1 = I.node1;
+  const uint32_t lane = threadIdx.x;
   if (I.nl_bdry_only) {
-    // particle by particle, each in boundary-list order (contact.h:1065-1069)
-    for (uint64_t g = g0; g < g1;) {
-      const DevPart& P = d.part[d.pid[g]];
-      for (uint32_t r = 0; r < P.nbdry; r++) {
-        const uint32_t h = d.blist[P.boff + r];
-        if (!d.wmask[h]) continue;
-        const double4 x = d.X[h];
-        const double4 u = disp_of(d, h);
-        const double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
-        const double v[3] = {d.Vold[3 * h], d.Vold[3 * h + 1], d.Vold[3 * h + 2]};
-        double fdummy[3] = {0, 0, 0};
-        wall_node(I, I.dt, cp, v, x.w, P.rho, fdummy, reac);
-        d.wmask[h] = 0;
-      }
-      g = P.node_off + P.nnodes;
-    }
-    for (int k = 0; k < 18; k++) d.reaction[18 * q + k] = reac[k];
+    if (lane == 0) reaction_bdry_only(d, I, q);
     return;
   }
-  for (uint64_t g = g0; g < g1;) {
-    uint64_t blk = g >> 8;
-    if (!d.bflag[blk]) {
-      g = (blk + 1) << 8;
-      continue;
-    }
-    uint64_t gend = ((blk + 1) << 8) < g1 ? ((blk + 1) << 8) : g1;
-    for (; g < gend; g++) {
-      if (!d.wmask[g]) continue;
-      const DevPart& P = d.part[d.pid[g]];
-      double4 x = d.X[g];
-      double4 u = disp_of(d, g);
-      double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
-      double v[3] = {d.Vold[3 * g], d.Vold[3 * g + 1], d.Vold[3 * g + 2]};
-      double fdummy[3] = {0, 0, 0};
-      wall_node(I, I.dt, cp, v, x.w, P.rho, fdummy, reac);
-      d.wmask[g] = 0;
+  double reac[18];
+#pragma unroll
+  for (int k = 0; k < 18; k++) reac[k] = 0.0;
+  const uint64_t g0 = I.node0, g1 = I.node1;
+  constexpr int plane[6] = {0, 3, 1, 4, 2, 5};   // x_min, x_max, y_min, y_max, z_min, z_max
+  if (g1 > g0) {
+    const uint64_t b0 = g0 >> 8, b1 = (g1 - 1) >> 8;
+    for (uint64_t bb = b0; bb <= b1; bb += 64) {
+      const uint64_t bl = bb + lane;
+      uint64_t bm = __builtin_amdgcn_ballot_w64(bl <= b1 && d.bflag[bl] != 0);
+      while (bm) {
+        const uint64_t blk = bb + (uint64_t)__builtin_ctzll(bm);
+        bm &= bm - 1;
+        for (uint32_t sub = 0; sub < 256; sub += 64) {
+          const uint64_t g = (blk << 8) + sub + lane;
+          const bool on = g >= g0 && g < g1 && d.wmask[g] != 0;
+          uint64_t cm = __builtin_amdgcn_ballot_w64(on);
+          if (!cm) continue;
+          WallSide w[6];
+#pragma unroll
+          for (int k = 0; k < 6; k++) w[k] = WallSide{};
+          if (on) {
+            const DevPart& P = d.part[d.pid[g]];
+            const double4 x = d.X[g];
+            const double4 u = disp_of(d, g);
+            const double cp[3] = {x.x + u.x, x.y + u.y, x.z + u.z};
+            const double v[3] = {d.Vold[3 * g], d.Vold[3 * g + 1], d.Vold[3 * g + 2]};
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+              const int pl = plane[k], dir = pl % 3;
+              w[k] = wall_side_eval(I, I.dt, cp[dir] - I.wall[pl], dir, v, x.w, P.rho, I.speed[pl]);
+            }
+            d.wmask[g] = 0;
+          }
+          // the nodes of this run in node order, their planes in the reference order
+          while (cm) {
+            const int l = __builtin_ctzll(cm);
+            cm &= cm - 1;
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+              const int pl = plane[k];
+              WallSide b{};
+              b.hit = __shfl((int)w[k].hit, l, 64) != 0;
+              if (!b.hit) continue;
+              b.dir = pl % 3;
+              b.fric = __shfl((int)w[k].fric, l, 64) != 0;
+              b.damp = __shfl((int)w[k].damp, l, 64) != 0;
+              b.rn = __shfl(w[k].rn, l, 64);
+              b.rf1 = __shfl(w[k].rf1, l, 64);
+              b.rf2 = __shfl(w[k].rf2, l, 64);
+              b.rd = __shfl(w[k].rd, l, 64);
+              wall_side_react(b, reac + 3 * pl);
+            }
+          }
+        }
+      }
     }
   }
-  for (int k = 0; k < 18; k++) d.reaction[18 * q + k] = reac[k];
+  if (lane == 0)
+    for (int k = 0; k < 18; k++) d.reaction[18 * q + k] = reac[k];
 }
 
 // Back to the end of the last complete step after a halt (the host makes
