@@ -1299,7 +1299,7 @@ static int launch_step(pd_ctx* c, int par, hipEvent_t* ev = nullptr) {
   if ((rc = mark(1))) return rc;
   // pair box test, wall prefilter and (for rebuilding jobs) cell keys
   hipLaunchKernelGGL(k_cellkey, dim3(grid_of(std::max({d.N, d.ninst, d.npairs, d.P}), 256)), dim3(256), 0, st, d);
-  if (any_torque(c)) hipLaunchKernelGGL(k_centroid, dim3(grid_of(d.P, 64)), dim3(64), 0, st, d);
+  if (any_torque(c)) hipLaunchKernelGGL(k_centroid, dim3(d.P), dim3(64), 0, st, d);
   if (d.lists) {
     // every kernel of the list rebuild exits at once unless a job rebuilds
     const uint32_t nb = d.T / kScanBlock;
@@ -1321,7 +1321,7 @@ static int launch_step(pd_ctx* c, int par, hipEvent_t* ev = nullptr) {
   if ((rc = mark(4))) return rc;
   if (any_clamped(c)) hipLaunchKernelGGL(k_clamp, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   if (any_vc(c)) {
-    hipLaunchKernelGGL(k_pv_mean, dim3(grid_of(d.ninst, 64)), dim3(64), 0, st, d);
+    hipLaunchKernelGGL(k_pv_mean, dim3(d.ninst), dim3(64), 0, st, d);
     hipLaunchKernelGGL(k_pv_apply, dim3(grid_of(Np, 256)), dim3(256), 0, st, d);
   }
   // at least 64 blocks: after a rebuild they also clear the cell table

@@ -314,26 +314,65 @@ __device__ __forceinline__ bool rebuilds_node(const Dev& d, uint32_t job) {
 
 // ---------------------------------------------------------------------------
 // mean_CurrPos (particle2.h:94-103) of every particle of a job with torque on:
-// one thread per particle sums pos + disp in node order, as the reference.
+// pos + disp summed in node order, as the reference.
+// The sums run one node after another, left to right, as the reference's
+// loop: one wavefront per particle loads 64 consecutive nodes' values at
+// once (the next 64 while the sums take these), then the sums take them lane
+// after lane through scalar reads of the lanes (no memory wait between the
+// additions).
+__device__ __forceinline__ double lane_value(double v, uint32_t l) {
+  const long long b = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ void add_lanes3(uint32_t cnt, double c0, double c1, double c2, double& m0, double& m1,
+                                           double& m2) {
+  for (uint32_t l = 0; l < cnt; l++) {
+    m0 += lane_value(c0, l);
+    m1 += lane_value(c1, l);
+    m2 += lane_value(c2, l);
+  }
+}
+__device__ __forceinline__ void currpos_of(const Dev& d, uint64_t g, uint64_t g1, double& c0, double& c1,
+                                           double& c2) {
+  c0 = c1 = c2 = 0.0;
+  if (g < g1) {
+    const double4 x = d.X[g];
+    const double4 u = disp_of(d, g);
+    c0 = x.x + u.x;
+    c1 = x.y + u.y;
+    c2 = x.z + u.z;
+  }
+}
+__device__ __forceinline__ void sum_currpos(const Dev& d, uint64_t g0, uint64_t g1, double& m0, double& m1,
+                                            double& m2) {
+  double c0, c1, c2;
+  currpos_of(d, g0 + threadIdx.x, g1, c0, c1, c2);
+  for (uint64_t b = g0; b < g1; b += 64) {
+    double n0, n1, n2;   // the next 64 nodes, loading meanwhile
+    currpos_of(d, b + 64 + threadIdx.x, g1, n0, n1, n2);
+    add_lanes3((uint32_t)(g1 - b < 64 ? g1 - b : 64), c0, c1, c2, m0, m1, m2);
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+  }
+}
 __global__ __launch_bounds__(64) void k_centroid(Dev d) {
   if (halted(d)) return;
-  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t q = blockIdx.x;
   if (q >= d.P) return;
   const DevPart& P = d.part[q];
   const DevInst& I = d.inst[P.inst];
   if (!live(I) || !I.enable_torque) return;
   double m0 = 0.0, m1 = 0.0, m2 = 0.0;
-  for (uint64_t g = P.node_off; g < P.node_off + P.nnodes; g++) {
-    const double4 x = d.X[g];
-    const double4 u = disp_of(d, g);
-    m0 += (x.x + u.x);
-    m1 += (x.y + u.y);
-    m2 += (x.z + u.z);
+  sum_currpos(d, P.node_off, P.node_off + P.nnodes, m0, m1, m2);
+  if (threadIdx.x == 0) {
+    const double n = (double)P.nnodes;
+    d.cent[3 * q] = m0 / n;
+    d.cent[3 * q + 1] = m1 / n;
+    d.cent[3 * q + 2] = m2 / n;
   }
-  const double n = (double)P.nnodes;
-  d.cent[3 * q] = m0 / n;
-  d.cent[3 * q + 1] = m1 / n;
-  d.cent[3 * q + 2] = m2 / n;
 }
 
 // ---------------------------------------------------------------------------
@@ -1636,31 +1675,35 @@ __global__ __launch_bounds__(256) void k_clamp(Dev d) {
   set_disp(d, g, 0.0, 0.0, 0.0);
 }
 
-// Prescribed velocity (timeloop.h:685-764), one thread per job: the mean
+// Prescribed velocity (timeloop.h:685-764), one wavefront per job: the mean
 // velocity and, for the angular part, mean_CurrPos of the particle, summed
 // in node order like the reference.
-__global__ void k_pv_mean(Dev d) {
+__global__ __launch_bounds__(64) void k_pv_mean(Dev d) {
   if (halted(d)) return;
-  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t q = blockIdx.x;
   if (q >= d.ninst) return;
   DevInst& I = d.inst[q];
   if (!live(I) || !I.vc_on || I.vc_slot == kNoKey) return;
   const DevPart& P = d.part[I.vc_slot];
   if (!P.movable) return;
+  const uint64_t g0 = P.node_off, g1 = P.node_off + P.nnodes;
   double m[3] = {0, 0, 0}, c[3] = {0, 0, 0};
-  for (uint64_t g = P.node_off; g < P.node_off + P.nnodes; g++)
-    for (int k = 0; k < 3; k++) m[k] += d.V[3 * g + k];
-  const double n = (double)P.nnodes;
-  for (int k = 0; k < 3; k++) I.vc_mean[k] = m[k] / n;
-  if (I.vc_fix_ang) {
-    for (uint64_t g = P.node_off; g < P.node_off + P.nnodes; g++) {
-      const double4 x = d.X[g];
-      const double4 u = disp_of(d, g);
-      c[0] += (x.x + u.x);
-      c[1] += (x.y + u.y);
-      c[2] += (x.z + u.z);
+  for (uint64_t b = g0; b < g1; b += 64) {
+    const uint64_t g = b + threadIdx.x;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (g < g1) {
+      v0 = d.V[3 * g];
+      v1 = d.V[3 * g + 1];
+      v2 = d.V[3 * g + 2];
     }
-    for (int k = 0; k < 3; k++) I.vc_cent[k] = c[k] / n;
+    add_lanes3((uint32_t)(g1 - b < 64 ? g1 - b : 64), v0, v1, v2, m[0], m[1], m[2]);
+  }
+  if (I.vc_fix_ang) sum_currpos(d, g0, g1, c[0], c[1], c[2]);
+  if (threadIdx.x == 0) {
+    const double n = (double)P.nnodes;
+    for (int k = 0; k < 3; k++) I.vc_mean[k] = m[k] / n;
+    if (I.vc_fix_ang)
+      for (int k = 0; k < 3; k++) I.vc_cent[k] = c[k] / n;
   }
 }
 
