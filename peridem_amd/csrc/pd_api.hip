@@ -763,6 +763,8 @@ __global__ __launch_bounds__(256) void k_rebase(uint32_t* b, uint64_t n, uint32_
 }  // namespace pd
 
 static thread_local std::string t_err;
+static int stage_reserve(pd_ctx* c, size_t bytes);
+constexpr size_t kStageJobMax = 256ull << 20;   // largest job uploaded through the pinned stage
 
 extern "C" {
 
@@ -1060,7 +1062,30 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
   const double zero18[18] = {0};
   Dev& d = c->d;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-#define UP(dst, src, n) HIPCHK(c, hipMemcpy((dst), (src), (n) * sizeof(*(dst)), hipMemcpyHostToDevice))
+  // The job's arrays go out through the pinned stage when they fit
+  // kStageJobMax (bounded by its footprint): one host copy each, then
+  // asynchronous copies in stream order with the kernels between them and
+  // one wait at the end; a larger job is copied synchronously.
+  const size_t bound = (size_t)(Np * kBytesPerNode + B->entries * kBytesPerEntry) +
+                       (size_t)np * (sizeof(DevPart) + 6 * sizeof(double) + 512) + sizeof(DevInst) + 64 * 256;
+  const bool staged = bound <= kStageJobMax;
+  if (staged && (rc = stage_reserve(c, bound))) return rc;
+  size_t at = 0;
+  auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (!bytes) return PD_OK;
+    if (staged && at + bytes <= c->stage_cap) {
+      char* h = (char*)c->stage + at;
+      memcpy(h, src, bytes);
+      at += (bytes + 255) & ~(size_t)255;
+      HIPCHK(c, hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, c->stream));
+    } else {
+      HIPCHK(c, hipStreamSynchronize(c->stream));   // after the copies and kernels before it
+      HIPCHK(c, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    }
+    return PD_OK;
+  };
+#define UP(dst, src, n) \
+  if ((rc = up((dst), (src), (n) * sizeof(*(dst))))) return rc
   UP(d.X + g0, B->X.data(), Np);
   UP(d.U + g0, B->U.data(), Np);
   hipLaunchKernelGGL(k_split, dim3(grid_of(Np, 256)), dim3(256), 0, c->stream, d, g0, Np, 1);
@@ -1104,6 +1129,7 @@ int pd_add_prepared(pd_ctx* c, const pd_job_blob* B, uint64_t nsteps, uint32_t* 
   HIPCHK(c, hipMemset(d.du2s + (uint64_t)jid * kDuSlots * kDuStride, 0,
                       kDuSlots * kDuStride * sizeof(unsigned long long)));
 #undef UP
+  HIPCHK(c, hipStreamSynchronize(c->stream));   // the stage is free again
   if (B->fl) {
     uint32_t cur = 0;
     HIPCHK(c, hipStreamSynchronize(c->stream));

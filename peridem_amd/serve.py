@@ -97,16 +97,16 @@ def run_batch(systems, nsteps, digests=False):
     try:
         budgets = nsteps if isinstance(nsteps, (list, tuple)) else [nsteps] * len(systems)
         # the host layout of every job (pd_job_prepare, thread-safe) on worker
-        # threads, then placement in arrival order
-        with ThreadPoolExecutor(max_workers=PREPARE_WORKERS) as ex:
-            blobs = list(ex.map(abi.prepare_job, systems))
+        # threads, placed in arrival order as each layout completes (placing
+        # overlaps the later layouts)
         jobs = []
-        for b, s, n in zip(blobs, systems, budgets):
-            j = pool.add_prepared(b, s, n)
-            b.free()
-            if j is None:
-                raise RuntimeError("job does not fit the pool sized for it")
-            jobs.append(j)
+        with ThreadPoolExecutor(max_workers=PREPARE_WORKERS) as ex:
+            for b, s, n in zip(ex.map(abi.prepare_job, systems), systems, budgets):
+                j = pool.add_prepared(b, s, n)
+                b.free()
+                if j is None:
+                    raise RuntimeError("job does not fit the pool sized for it")
+                jobs.append(j)
         t1 = time.perf_counter()
         dev_ms = 0.0
         left = max(budgets)
