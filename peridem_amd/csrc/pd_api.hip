@@ -2131,8 +2131,14 @@ int pd_job_get_connectivity(pd_ctx* c, uint32_t job, uint32_t pi, uint32_t* pair
                      c->conn_buf);
   HIPCHK(c, hipGetLastError());
   if ((rc = stage_reserve(c, 2 * need * sizeof(uint32_t)))) return rc;
+  // the count and, when the caller wants the pairs and can hold every
+  // possible one (half the directed entries), all of them in one wait
   uint32_t* hs = (uint32_t*)c->stage;
+  const uint64_t most = p.nbonds / 2;
+  const bool whole = pairs && cap >= most && most;
   HIPCHK(c, hipMemcpyAsync(hs, offs + p.n, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  if (whole)
+    HIPCHK(c, hipMemcpyAsync(hs + 2, c->conn_buf, 2ull * most * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const uint32_t n = hs[0];
   *m = n;
@@ -2140,9 +2146,11 @@ int pd_job_get_connectivity(pd_ctx* c, uint32_t job, uint32_t pi, uint32_t* pair
     return fail(c, PD_E_ARG, "connectivity buffer holds %llu pairs, %llu needed", (unsigned long long)cap,
                 (unsigned long long)n);
   if (pairs && n) {
-    HIPCHK(c, hipMemcpyAsync(hs, c->conn_buf, 2ull * n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    memcpy(pairs, hs, 2ull * n * sizeof(uint32_t));
+    if (!whole) {
+      HIPCHK(c, hipMemcpyAsync(hs + 2, c->conn_buf, 2ull * n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    memcpy(pairs, hs + 2, 2ull * n * sizeof(uint32_t));
   }
   return PD_OK;
 }
