@@ -308,6 +308,25 @@ def serving(args):
             f"peak {srv.peak_live} resident; {same}/{len(jobs)} identical; {srv.steps_run} pool steps, "
             f"{srv.device_ms * 1e-3:.2f} s on the device (alone: {sum(dev_seq) * 1e-3:.2f} s)")
         out["rates"].append(rec)
+    if args.serve_phases:
+        # where the served pool's device time goes: the highest rate once
+        # more with a mark around every phase of every step (steps launched
+        # one by one, so this run is not one of the timed ones above)
+        rate = max(args.serve_rates)
+        arr = serve.poisson_arrivals(len(jobs), rate, seed=int(rate * 1000) + 1)
+        srv = serve.Server(chunk=args.serve_chunk, digests=False, **cap)
+        srv.pool.set_profiling(abi.PD_PROFILE_PHASES)
+        try:
+            srv.serve([(a, x, n) for a, (x, n) in zip(arr, jobs)])
+            ph = srv.pool.profile()
+        finally:
+            srv.close()
+        tot = sum(v["ms"] for v in ph.values())
+        out["phases_at_rate"] = {"rate_jobs_per_s": rate, "pool_steps": srv.steps_run,
+                                 "device_ms": {k: v["ms"] for k, v in ph.items()},
+                                 "share": {k: v["ms"] / tot for k, v in ph.items()}}
+        log(f"[bench] serving phases at {rate}/s over {srv.steps_run} pool steps: " +
+            ", ".join(f"{k} {v['ms'] * 1e-3:.2f} s ({v['ms'] / tot:.1%})" for k, v in ph.items()))
     return out
 
 
@@ -443,6 +462,8 @@ def main():
     ap.add_argument("--serve-step-scale", type=float, default=0.05)
     ap.add_argument("--serve-pool-gib", type=float, default=1.0)
     ap.add_argument("--serve-chunk", type=int, default=256)
+    ap.add_argument("--serve-phases", action="store_true",
+                    help="serve the highest rate once more with every phase of every step timed")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (a plain copy "
                          "kernel of known size that calibrates FETCH_SIZE/WRITE_SIZE)")
