@@ -1,7 +1,7 @@
 """Python loader of the CPU oracle (oracle/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's CPU-baseline leg.  The product path never
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check the HIP library's results and by bench.py's CPU-baseline leg.  The product path never
 imports it.  The oracle exports the pd_ ABI under the or_ prefix, so the
 same peridem_amd.abi.Engine class drives it.
 """

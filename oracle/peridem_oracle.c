@@ -1,8 +1,8 @@
 /*
  * peridem_oracle.c -- CPU oracle for the PeriDEM explicit step.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the CPU
- * baseline of the benchmark.  Nothing in the product path links or calls it.
+ * TEST INFRASTRUCTURE ONLY: what the HIP library's results are checked
+ * against, and the CPU baseline of the benchmark.  Nothing in the product path links or calls it.
  *
  * A plain-C restatement of the reference algorithm, operation for operation
  * (IEEE FP64, no FMA contraction: build with -ffp-contract=off).  Each routine

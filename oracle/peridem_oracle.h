@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * algorithm (src/particle/particle2.h, src/particle/contact.h,
  * src/particle/timeloop.h, src/compat/overloads.h, src/read/load_hdf5.h),
- * used as the checker for the HIP library and as the CPU baseline in the
+ * used to check the HIP library's results and as the CPU baseline in the
  * benchmark.  Product code never links or calls it.
  *
  * The API mirrors include/peridem_gpu.h (or_* for pd_*) and takes the same
