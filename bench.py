@@ -484,7 +484,7 @@ def main():
                     help="every this many steps, one sample of steps with a mark around every phase (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+        print(json.dumps({"error": "the library steps on one GPU; sharding over several is a design "
                                    "note only (particles shard by spatial block with a surface "
                                    "halo exchange)", "n_gpus": args.gpus}))
         sys.exit(2)
