@@ -274,8 +274,9 @@ int pd_set_graphs(pd_ctx* ctx, int on);
  * The library carries several instantiations of the bond kernel (entries
  * per batch, software pipelining of the index and neighbour loads, and
  * "staged" shapes that first copy each 256-node block's table of referenced
- * nodes and their positions and volumes into LDS); each sums every row in
- * row order, so all give the same bits.
+ * nodes and their positions and volumes into LDS, and a shape with four
+ * lanes per node, pipelined = 3, for pools too small to fill the GPU); each
+ * sums every row in row order, so all give the same bits.
  * Staged shapes are usable while every resident job's block tables hold at
  * most 2240 nodes (pools of 2^27 node slots or more use one unstaged shape);
  * a selected shape that is not usable falls back to shape 4.  By default the
@@ -284,8 +285,10 @@ int pd_set_graphs(pd_ctx* ctx, int on);
  * arrays, so nothing changes) and keeps the fastest: every usable shape is
  * timed once (while staged shapes are usable, of the unstaged ones only shape
  * 4), and the shapes within 15% of the fastest three more times in
- * interleaved rounds, each keeping its smallest time; smaller pools use shape
- * 10 (staged), or 4 while that is not usable.  pd_autotune re-runs that with
+ * interleaved rounds, each keeping its smallest time (the four-lane shape
+ * only on pools below 2^18 nodes); smaller pools use shape 14 (four lanes per
+ * node) while fewer than 2^15 nodes are resident, else shape 10 (staged), or
+ * 4 while that is not usable.  pd_autotune re-runs that with
  * reps timed launches per timing, returning the choice and (when ms != NULL)
  * the time of each shape [count], -1 for a shape not usable on this pool; pd_set_bond_shape forces one (-1: tune
  * at the next step); pd_get_bond_shape reports the shape launches use.

@@ -567,7 +567,8 @@ class Pool:
 def bond_shapes():
     """[(unroll, pipeline, staged)] of the bond-kernel shapes the library was
     built with (pipeline: 0 none, 1 software-pipelined, 2 pipelined and unrolled
-    twice; staged: block node tables and static records in LDS)."""
+    twice, 3 pipelined with four lanes per node; staged: block node tables and
+    static records in LDS)."""
     lib, _ = load_gpu_lib()
     return [bond_shape_info(k) for k in range(lib.pd_bond_shape_count())]
 

@@ -38,6 +38,12 @@ constexpr uint64_t kUnlimited = ~0ull;
 constexpr uint64_t kTuneMinNodes = 1u << 18;  // autotune pools of at least this many nodes
 constexpr int kDefaultBondShape = 4;          // unstaged, pipelined, 4 entries per batch (always usable)
 constexpr int kSmallPoolShape = 10;           // staged, pipelined and unrolled twice, 4 entries per batch
+constexpr int kQuadShape = 14;                // four lanes per node slot, 2 entries per lane and batch
+// Pools below this many resident nodes fill too few SIMDs for one lane per
+// node: the automatic small-pool shape is then kQuadShape (two-sphere impact
+// of 8,338 nodes 73.0 -> 56.4 us per step, 25,461 nodes 57.6 -> 49.7; even
+// at ~37k nodes, slower above).
+constexpr uint64_t kQuadMaxNodes = 1u << 15;
 constexpr double kSkinFrac = 0.3;             // candidate-list skin / R_c
 constexpr int kGraphSteps = 8;  // steps per captured graph
 constexpr int kProfRing = 16;   // event sets of profiled steps in flight
@@ -161,6 +167,7 @@ struct pd_ctx {
   bool use_graph = true, graph_dirty = true;
   uint64_t hwm_slices = 0;      // end of the highest node range in use
   int bond_shape = -1;          // index into bond_shapes() or kWideShape; -1: tune at the next step
+  bool small_auto = false;      // bond_shape is the automatic small-pool choice
   int arith = PD_ARITH_BITWISE; // bond-term arithmetic (pd_set_arithmetic)
   uint32_t lds_bad = 0;         // resident jobs whose block tables exceed kLdsCap
   uint64_t tuned_nodes = 0;     // resident nodes when the shape was tuned
@@ -1274,8 +1281,15 @@ static bool shape_usable(const pd_ctx* c, int v) {
 // kWideShape (one past the tuned shapes) selects bond_wide on any pool.
 static int effective_shape(const pd_ctx* c) {
   if (c->bond_shape == kWideShape) return kWideShape;
-  const int v = c->bond_shape < 0 ? kDefaultBondShape : c->bond_shape;
+  int v = c->bond_shape < 0 ? kDefaultBondShape : c->bond_shape;
+  if (c->small_auto && c->cnt.nodes < kQuadMaxNodes && shape_usable(c, kQuadShape)) v = kQuadShape;
   return shape_usable(c, v) ? v : kDefaultBondShape;
+}
+
+// Workgroups of a bond-kernel launch over Np node slots: a multiple of the
+// XCD count (the bond kernels remap blocks), lanes() threads per slot.
+static dim3 bond_grid(uint64_t Np, const BondShape& b) {
+  return dim3((grid_of(Np, 256) * b.lanes() + kXcds - 1) / kXcds * kXcds);
 }
 
 static BondShape launch_shape(const pd_ctx* c) {
@@ -1340,8 +1354,8 @@ static int launch_step(pd_ctx* c, int par, hipEvent_t* ev = nullptr) {
   // wall, external and torque terms and takes the Verlet velocity half;
   // grid padded to a multiple of the XCD count (the bond kernel remaps
   // blocks)
-  hipLaunchKernelGGL(launch_shape(c).fn,
-                     dim3((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds), dim3(256), 0, st, d);
+  const BondShape bs = launch_shape(c);
+  hipLaunchKernelGGL(bs.fn, bond_grid(Np, bs), dim3(256), 0, st, d);
   if ((rc = mark(3))) return rc;
   if (any_reaction(c)) hipLaunchKernelGGL(k_reaction, dim3(d.ninst), dim3(64), 0, st, d);
   if ((rc = mark(4))) return rc;
@@ -1364,6 +1378,7 @@ static int launch_step(pd_ctx* c, int par, hipEvent_t* ev = nullptr) {
 // wall masks), so the state is untouched.  Pools too large for 32-bit record
 // offsets have one shape only.
 static int autotune(pd_ctx* c, uint32_t reps) {
+  c->small_auto = false;
   if (wide_pool(c)) {
     c->bond_shape = kDefaultBondShape;
     return PD_OK;
@@ -1415,9 +1430,9 @@ static int autotune(pd_ctx* c, uint32_t reps) {
     t.AFD = AFD;
     t.wmask = WM;
     t.bflag = BF;
-    const dim3 grid((grid_of(Np, 256) + kXcds - 1) / kXcds * kXcds);
     // mean time of `reps` launches of shape v
     auto timed = [&](int v, double* ms_out) -> int {
+      const dim3 grid = bond_grid(Np, shapes(c)[v]);
       HIPCHK(c, hipEventRecord(e0, c->stream));
       for (uint32_t r = 0; r < reps; r++) hipLaunchKernelGGL(shapes(c)[v].fn, grid, dim3(256), 0, c->stream, t);
       HIPCHK(c, hipEventRecord(e1, c->stream));
@@ -1435,15 +1450,19 @@ static int autotune(pd_ctx* c, uint32_t reps) {
     // shape if none is usable.
     // While the staged shapes are usable the unstaged ones (35-75% slower on
     // every workload measured) are represented by the default one alone.
+    // The four-lanes-per-node shapes are for pools too small to fill the GPU
+    // and are timed only on pools below the automatic tuning size.
     bool staged = false;
     for (int v = 0; v < kBondShapes; v++) staged |= shapes(c)[v].lds && shape_usable(c, v);
     double first_best = HUGE_VAL;
     for (int v = 0; v < kBondShapes; v++) {
-      if (!shape_usable(c, v) || (staged && !shapes(c)[v].lds && v != kDefaultBondShape)) {
+      const bool quad = shapes(c)[v].lanes() > 1;
+      if (!shape_usable(c, v) || (staged && !shapes(c)[v].lds && v != kDefaultBondShape && !quad) ||
+          (quad && c->cnt.nodes >= kTuneMinNodes)) {
         c->tune_ms[v] = -1.0;   // not timed
         continue;
       }
-      hipLaunchKernelGGL(shapes(c)[v].fn, grid, dim3(256), 0, c->stream, t);   // warm-up
+      hipLaunchKernelGGL(shapes(c)[v].fn, bond_grid(Np, shapes(c)[v]), dim3(256), 0, c->stream, t);   // warm-up
       if ((rc = timed(v, &c->tune_ms[v]))) return rc;
       first_best = std::min(first_best, c->tune_ms[v]);
     }
@@ -1638,6 +1657,7 @@ int pd_step(pd_ctx* c, uint32_t nsteps) {
       if ((rc = autotune(c, 3))) return rc;
     } else {
       c->bond_shape = kSmallPoolShape;   // kDefaultBondShape while it is not usable
+      c->small_auto = true;              // kQuadShape while the pool is below kQuadMaxNodes
     }
   }
   uint32_t left = nsteps;
@@ -1685,6 +1705,7 @@ int pd_set_arithmetic(pd_ctx* c, int mode) {
   if (mode != c->arith) {
     c->arith = mode;
     c->bond_shape = -1;   // the other table's shapes: tuned again at the next step
+    c->small_auto = false;
     c->graph_dirty = true;
   }
   return PD_OK;
@@ -1709,6 +1730,7 @@ int pd_set_bond_shape(pd_ctx* c, int shape) {
   if (!c) return PD_E_ARG;
   if (shape < -1 || shape > kWideShape) return fail(c, PD_E_ARG, "no bond shape %d", shape);
   c->bond_shape = shape;
+  c->small_auto = false;
   c->graph_dirty = true;
   return PD_OK;
 }

@@ -1347,11 +1347,15 @@ __device__ __forceinline__ double rsq_refined(double x) {
 // |xi|^2 = +0; giving both the high word of 1.0 makes them exactly 1.0, the
 // stretch exactly +0 and the coefficient +-0 without a select on it.  The
 // IEEE branch (safe mode, or a lane outside the domain) selects explicitly.
-template <int U, bool TOL, class Src>
+//
+// STRIDE > 1 (bitwise arithmetic only): the U entries are k, k + STRIDE, ...
+// of the row, and their terms are handed out in tout instead of added.
+template <int U, bool TOL, class Src, int STRIDE = 1>
 __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint32_t (&e)[U], const double4 (&xj)[U],
                                            const double4 (&uj)[U], const double4& xi, const double4& ui, bool fast,
                                            bool can_break, double cnot, double snot, double& f0, double& f1,
-                                           double& f2) {
+                                           double& f2, double (*tout)[3] = nullptr) {
+  static_assert(STRIDE == 1 || !TOL, "terms handed out only in bitwise arithmetic");
   bool alive[U];
   double ev[U][3], xv[U][3], n2[U], l2[U];
   uint32_t over = 0;
@@ -1411,7 +1415,11 @@ __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint3
   }
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    if (TOL && fused) {
+    if (STRIDE > 1) {
+      tout[u][0] = t[u][0];
+      tout[u][1] = t[u][1];
+      tout[u][2] = t[u][2];
+    } else if (TOL && fused) {
       f0 = __builtin_fma(t[u][0], ev[u][0], f0);
       f1 = __builtin_fma(t[u][0], ev[u][1], f1);
       f2 = __builtin_fma(t[u][0], ev[u][2], f2);
@@ -1420,7 +1428,7 @@ __device__ __forceinline__ void bond_terms(const Src& S, uint32_t k, const uint3
       f1 += t[u][1];
       f2 += t[u][2];
     }
-    if (alive[u] & can_break & (str[u] > snot)) S.broke(k + u, e[u]);
+    if (alive[u] & can_break & (str[u] > snot)) S.broke(k + u * STRIDE, e[u]);
   }
 }
 
@@ -1622,6 +1630,105 @@ __global__ __launch_bounds__(256, 2) void k_bond_lds(Dev d) {
   d.F[3 * g + 2] = f2;
 }
 
+// Entry s of every quad of lanes, broadcast to the quad (DPP quad_perm).
+template <int SRC>
+__device__ __forceinline__ double quad_lane(double v) {
+  constexpr int ctrl = SRC * 0x55;   // every lane of the quad reads lane SRC
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, ctrl, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), ctrl, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Bond kernel for pools too small to fill the GPU (one wavefront per 64
+// nodes leaves most SIMDs idle and each lane's row a long dependent chain):
+// four lanes per node slot, lane s of the quad evaluating entries s, s + 4,
+// s + 8, ... of the row, U per batch, with the next batch's entries and
+// records loading while a batch is evaluated.  Every lane of the quad then
+// adds the batch's terms in row order, entry 4m + s taken from lane s, so
+// the sum is the one-lane row sum operation for operation; breaks are
+// recorded by the lane that evaluated the entry, and the quad's first lane
+// finishes the node.  The four lanes of a quad share their node, so they
+// take the same branches and the exchanges always read active lanes.
+template <int U>
+__global__ __launch_bounds__(256) void k_bond_quad(Dev d) {
+  if (halted(d)) return;
+  const uint64_t T = xcd_block() * blockDim.x + threadIdx.x;
+  const uint64_t t = T >> 2;                    // slot
+  const uint32_t sub = (uint32_t)(T & 3);       // lane in the quad
+  if (t >= d.S * kSlice) return;
+  const uint64_t g = (t & ~255ull) + d.lane_node[t];
+  const uint32_t p = d.pid[g];
+  const NodeHead H = node_head(d, p);
+  if (!H.live) return;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  const ContactIn cin = contact_in(d, g, p, H.movable);
+  if (H.movable) {
+    uint64_t base;
+    uint32_t W;
+    slice_of(d, t, base, W);   // a wavefront's 16 slots lie in one slice
+    const uint32_t lane = (uint32_t)(t & 63);
+    const GlobalRows<true> S{d,
+                             {(const char*)(d.bond + base), lane * 4u},
+                             {(const char*)(d.bloc + base), lane * 2u},
+                             (uint32_t)g};
+    double4 xi, ui;
+    rec<true>(d, (uint32_t)g, xi, ui);
+    const bool fast = (*d.flags & kFlagSafeMath) == 0;
+    // entries k0 + sub + 4u of the batch at k0 (pad past the width)
+    auto idx = [&](uint32_t k0, uint32_t(&e)[U]) {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const uint32_t k = k0 + sub + 4u * u;
+        const bool in = k < W;
+        const uint32_t v = __builtin_nontemporal_load(S.R.at(in ? k : W - 1));
+        e[u] = in ? v : GlobalRows<true>::kPadE;
+      }
+    };
+    if (W) {
+      uint32_t e0[U], e1[U];
+      double4 xa[U], ua[U];
+      idx(0, e0);
+      idx(4 * U, e1);
+      S.template gather<U>(e0, e0, xa, ua);
+      for (uint32_t k = 0; k < W; k += 4 * U) {
+        uint32_t e2[U];
+        double4 xb[U], ub[U];
+        idx(k + 8 * U, e2);
+        S.template gather<U>(e1, e1, xb, ub);
+        double tt[U][3];
+        bond_terms<U, false, GlobalRows<true>, 4>(S, k + sub, e0, xa, ua, xi, ui, fast, H.frac, H.cnot, H.snot, f0,
+                                                  f1, f2, tt);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            const double t0 = quad_lane<0>(tt[u][q]), t1 = quad_lane<1>(tt[u][q]);
+            const double t2 = quad_lane<2>(tt[u][q]), t3 = quad_lane<3>(tt[u][q]);
+            double& f = q == 0 ? f0 : (q == 1 ? f1 : f2);
+            f += t0;
+            f += t1;
+            f += t2;
+            f += t3;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          e0[u] = e1[u];
+          e1[u] = e2[u];
+          xa[u] = xb[u];
+          ua[u] = ub[u];
+        }
+      }
+    }
+  }
+  if (sub) return;
+  finish_node(d, g, p, cin, f0, f1, f2);
+  d.F[3 * g] = f0;
+  d.F[3 * g + 1] = f1;
+  d.F[3 * g + 2] = f2;
+}
+
 // Instantiated shapes of the bond kernel; the autotuner times them on the
 // resident pool and keeps the fastest.  Every shape sums each row in row order,
 // so all give the same bits.  They address node records with 32-bit offsets;
@@ -1629,9 +1736,11 @@ __global__ __launch_bounds__(256, 2) void k_bond_lds(Dev d) {
 // usable while every job's block tables fit kLdsCap.
 struct BondShape {
   void (*fn)(Dev);
-  int unroll, pipe, lds;   // pipe: 0 none, 1 pipelined, 2 pipelined and unrolled twice
+  int unroll, pipe, lds;   // pipe: 0 none, 1 pipelined, 2 pipelined and unrolled twice,
+                           // 3 pipelined with four lanes per node slot (k_bond_quad)
+  __host__ int lanes() const { return pipe == 3 ? 4 : 1; }   // threads per node slot
 };
-constexpr int kBondShapes = 14;
+constexpr int kBondShapes = 15;
 constexpr uint64_t kOff32MaxNodes = 1ull << 27;   // 16-byte records far below 4 GiB
 __host__ inline const BondShape* bond_shapes() {
   static const BondShape t[kBondShapes] = {
@@ -1639,7 +1748,7 @@ __host__ inline const BondShape* bond_shapes() {
       {k_bond_t<3, 1, true>, 3, 1, 0}, {k_bond_t<4, 1, true>, 4, 1, 0}, {k_bond_t<5, 1, true>, 5, 1, 0},
       {k_bond_t<3, 2, true>, 3, 2, 0}, {k_bond_t<4, 2, true>, 4, 2, 0}, {k_bond_t<2, 2, true>, 2, 2, 0},
       {k_bond_lds<3, 2>, 3, 2, 1},     {k_bond_lds<4, 2>, 4, 2, 1},     {k_bond_lds<2, 2>, 2, 2, 1},
-      {k_bond_lds<4, 1>, 4, 1, 1},     {k_bond_lds<5, 2>, 5, 2, 1}};
+      {k_bond_lds<4, 1>, 4, 1, 1},     {k_bond_lds<5, 2>, 5, 2, 1},     {k_bond_quad<2>, 2, 3, 0}};
   return t;
 }
 // The same shapes in tolerance arithmetic (the default unstaged shape and the
@@ -1650,7 +1759,7 @@ __host__ inline const BondShape* bond_shapes_tol() {
       {nullptr, 3, 1, 0}, {k_bond_t<4, 1, true, true>, 4, 1, 0}, {nullptr, 5, 1, 0},
       {nullptr, 3, 2, 0}, {nullptr, 4, 2, 0}, {nullptr, 2, 2, 0},
       {k_bond_lds<3, 2, true>, 3, 2, 1}, {k_bond_lds<4, 2, true>, 4, 2, 1}, {k_bond_lds<2, 2, true>, 2, 2, 1},
-      {k_bond_lds<4, 1, true>, 4, 1, 1}, {nullptr, 5, 2, 1}};
+      {k_bond_lds<4, 1, true>, 4, 1, 1}, {nullptr, 5, 2, 1}, {nullptr, 2, 3, 0}};
   return t;
 }
 // The default shape with 64-bit record addressing, for pools too large for

@@ -139,6 +139,39 @@ def test_every_bond_shape_matches_oracle(oracle_lib):
             g.close()
 
 
+def test_small_pools_take_the_four_lane_shape(oracle_lib):
+    """A pool below 2^15 resident nodes runs the bond kernel with four lanes
+    per node slot (shape 14, pipelined = 3) unless a shape is forced; a larger
+    one below the tuning size keeps the staged shape 10.  Through contact and
+    breaks the four-lane run is the oracle's bit for bit."""
+    quad = 14
+    assert abi.bond_shape_info(quad)[1] == 3
+    s = small_impact(seed=13)
+    assert s.nodes < 1 << 15
+    g = abi.gpu_engine(s)
+    o = oracle_lib.engine(s)
+    try:
+        for n in (1, 30, 60):
+            print(f"[parity] four-lane default shape, {n} steps", flush=True)
+            g.step(n)
+            o.step(n)
+            assert g.bond_shape() == quad
+            compare_engines(g, o, s, "four-lane shape")
+        c = g.counters()
+        assert c["alive_bonds"] < c["total_bonds"] and c["pair_computations"] > 0
+    finally:
+        g.close()
+        o.close()
+    big = gen.aggregate_job(9)
+    assert 1 << 15 <= big.nodes < 1 << 18
+    g = abi.gpu_engine(big)
+    try:
+        g.step(1)
+        assert g.bond_shape() == 10
+    finally:
+        g.close()
+
+
 def test_wide_bond_shape_matches_oracle(oracle_lib):
     """The 64-bit-offset bond kernel of large pools, forced on a small one,
     gives the oracle's bits through a run that breaks bonds."""
@@ -231,8 +264,9 @@ def test_autotune_leaves_state_untouched(oracle_lib):
         best, ms = g.autotune(2)
         shapes = abi.bond_shapes()
         assert 0 <= best < len(shapes) and g.bond_shape() == best
-        # staged shapes usable here: all of them timed, of the unstaged ones only the default (4)
-        assert all((ms[k] > 0) == (sh[2] or k == 4) for k, sh in enumerate(shapes))
+        # staged shapes usable here: all of them timed, of the unstaged ones the
+        # default (4) and, on a pool below 2^18 nodes, the four-lane shape
+        assert all((ms[k] > 0) == (sh[2] or k == 4 or sh[1] == 3) for k, sh in enumerate(shapes))
         assert ms[best] == ms[ms > 0].min()          # the tuner keeps the fastest shape
         for p in range(len(s.particles)):
             after = g.state(p)
